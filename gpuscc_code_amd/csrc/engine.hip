@@ -1,0 +1,608 @@
+// Host engine of the FB-SCC library: context, upload, the FB step loop and the
+// phase-level entry points of include/gpuscc.h.  It restates computeSCC
+// (scc.cu:150-1022) with one change of mechanism: the reference sets a flag,
+// launches a level and copies the flag back for every BFS level, trim sweep
+// and WCC sweep (reach_q.cu:98-109, scc_routines.cu:597-606, wcc.cu:166-171);
+// here levels are chained on the device -- a level whose predecessor found
+// nothing exits at once -- and the host launches them in growing batches,
+// reading the control block back once per batch.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fb_device.hpp"
+#include "gpuscc.h"
+#include "host/graph.hpp"
+#include "kernels.hpp"
+
+using namespace gpuscc;
+
+struct scc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    scc_opts opts{};
+    std::string err;
+    int cu_count = 256;
+    Graph g{};
+    int32_t *d_row = nullptr, *d_col = nullptr, *d_rowT = nullptr, *d_colT = nullptr;
+    uint32_t *d_word = nullptr;
+    int32_t *d_root = nullptr, *d_label = nullptr, *d_tmp = nullptr;
+    int32_t *d_q[2] = {nullptr, nullptr};
+    int32_t *d_piv = nullptr;
+    unsigned long long *d_slots[2] = {nullptr, nullptr};
+    int slot_bits = 10;
+    int prev_tab = 0;        // table holding the last election's winners
+    uint32_t epoch = 0;
+    Ctl *d_ctl = nullptr;
+    Ctl *h_ctl = nullptr;    // pinned mirror
+    uint16_t *d_score = nullptr;  // learned-rule scores (per vertex), may be null
+    bool uploaded = false, labels_valid = false;
+    int guess[3] = {4, 4, 2};     // chain batch guesses: fwd, bwd, trim
+    int64_t syncs = 0;
+    int instr = 0;
+    unsigned long long st_expanded = 0, st_edges = 0, st_visited = 0;
+};
+
+namespace {
+
+scc_status fail(scc_ctx *c, scc_status s, const std::string &what) {
+    if (c) c->err = what;
+    return s;
+}
+
+scc_status hip_fail(scc_ctx *c, hipError_t e, const char *what) {
+    scc_status s = (e == hipErrorOutOfMemory) ? SCC_E_OOM : SCC_E_HIP;
+    return fail(c, s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, x)                                  \
+    do {                                                \
+        hipError_t e_ = (x);                            \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #x); \
+    } while (0)
+
+#define CHK(x)                          \
+    do {                                \
+        scc_status s_ = (x);            \
+        if (s_ != SCC_OK) return s_;    \
+    } while (0)
+
+Launch launch_for(const scc_ctx *c) {
+    uint32_t block = (uint32_t)(c->opts.block_size > 0 ? c->opts.block_size : 256);
+    block = std::max<uint32_t>(64, std::min<uint32_t>(1024, block & ~63u));
+    uint32_t grid = c->opts.grid_blocks > 0 ? (uint32_t)c->opts.grid_blocks
+                                            : (uint32_t)c->cu_count * 4u;
+    uint64_t need = ((uint64_t)std::max<int32_t>(c->g.n, 1) + block - 1) / block;
+    if (grid > need) grid = (uint32_t)std::max<uint64_t>(need, 1);
+    grid = (grid + 7u) & ~7u;
+    return Launch{c->stream, grid, block};
+}
+
+Launch fixed_launch(const scc_ctx *c) {  // kernels with LDS sized for 256 work-items
+    Launch l = launch_for(c);
+    l.block = 256;
+    uint64_t need = ((uint64_t)std::max<int32_t>(c->g.n, 1) + 255) / 256;
+    uint32_t grid = c->opts.grid_blocks > 0 ? (uint32_t)c->opts.grid_blocks : (uint32_t)c->cu_count * 4u;
+    if (grid > need) grid = (uint32_t)std::max<uint64_t>(need, 1);
+    l.grid = (grid + 7u) & ~7u;
+    return l;
+}
+
+scc_status read_ctl(scc_ctx *c) {
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->syncs++;
+    return SCC_OK;
+}
+
+// Runs a chain of level kernels to its fixpoint; *levels = kernels that did work.
+template <class F>
+scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int64_t *levels) {
+    launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
+    int64_t launched = 0;
+    int L = std::max(1, std::min(guess, 64));
+    const int64_t limit = (int64_t)c->g.n + 4;
+    for (;;) {
+        for (int i = 0; i < L; i++) launch_level((int)(launched + i));
+        launched += L;
+        CHK(read_ctl(c));
+        uint32_t ran = c->h_ctl->ran;
+        if ((int64_t)ran < launched || c->h_ctl->changed[(launched - 1) & 3] == 0) {
+            *levels = ran;
+            break;
+        }
+        if (launched > limit) return fail(c, SCC_E_STATE, "level chain did not converge");
+        L = std::min(L * 2, 1024);
+    }
+    guess = (int)std::max<int64_t>(1, *levels);
+    return SCC_OK;
+}
+
+scc_status do_trim(scc_ctx *c, int64_t *iters) {
+    Launch l = launch_for(c);
+    return run_chain(
+        c, [&](int it) { launch_trim_sweep(l, c->g, c->d_word, c->d_root, c->d_ctl, it); },
+        c->guess[2], 0, iters);
+}
+
+// seeds: queue schedules read the pivot list (d_piv, length ctl->npiv)
+scc_status do_reach(scc_ctx *c, int dir, int sched, int64_t *depth) {
+    const int32_t *row = dir == 0 ? c->g.row : c->g.rowT;
+    const int32_t *col = dir == 0 ? c->g.col : c->g.colT;
+    Launch l = launch_for(c);
+    Launch lf = fixed_launch(c);
+    int &guess = c->guess[dir];
+    int instr = c->instr;
+    if (instr) launch_ctl_reset(c->stream, c->d_ctl, 32);
+    scc_status s;
+    switch (sched) {
+    case SCC_SCHED_Q:
+        s = run_chain(
+            c, [&](int it) { launch_reach_sweep(l, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
+            guess, 0, depth);
+        break;
+    case SCC_SCHED_T:
+        s = run_chain(
+            c, [&](int it) { launch_reach_topo(lf, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
+            guess, 0, depth);
+        break;
+    case SCC_SCHED_D:
+    case SCC_SCHED_L:
+    case SCC_SCHED_H: {
+        uint32_t thr = c->opts.hybrid_threshold > 0 ? (uint32_t)c->opts.hybrid_threshold
+                                                    : (uint32_t)std::max<int64_t>(1024, c->g.n / 32);
+        auto inq = [&](int it) -> const int32_t * { return it == 0 ? c->d_piv : c->d_q[(it - 1) & 1]; };
+        auto outq = [&](int it) -> int32_t * { return c->d_q[it & 1]; };
+        if (sched == SCC_SCHED_D)
+            s = run_chain(c, [&](int it) {
+                    launch_reach_queue(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
+                }, guess, 2, depth);
+        else if (sched == SCC_SCHED_L)
+            s = run_chain(c, [&](int it) {
+                    launch_reach_lb(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
+                }, guess, 2, depth);
+        else
+            s = run_chain(c, [&](int it) {
+                    launch_reach_hybrid(lf, row, col, c->d_word, c->g.n, inq(it), outq(it), dir,
+                                        c->d_ctl, it, thr, instr);
+                }, guess, 2, depth);
+        break;
+    }
+    default:
+        return fail(c, SCC_E_INPUT, "unknown schedule");
+    }
+    if (s == SCC_OK && instr) {
+        c->st_expanded += c->h_ctl->st_expanded;
+        c->st_edges += c->h_ctl->st_edges;
+        c->st_visited += c->h_ctl->st_visited;
+    }
+    return s;
+}
+
+// one election into the table that is not `prev`; winners -> state 19 + pivot list
+scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv) {
+    Launch l = launch_for(c);
+    c->epoch++;
+    unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
+    int64_t nslots = (int64_t)1 << c->slot_bits;
+    if ((c->epoch & 0xFFFFu) < 2u) launch_fill_u64(l, cur, nslots, ~0ull);  // tag wrap
+    launch_ctl_reset(c->stream, c->d_ctl, 4);
+    launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
+                  recolour, c->d_score);
+    launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
+    CHK(read_ctl(c));
+    c->prev_tab ^= 1;
+    *npiv = c->h_ctl->npiv;
+    return SCC_OK;
+}
+
+scc_status do_count(scc_ctx *c, int what, int64_t *out) {
+    launch_ctl_reset(c->stream, c->d_ctl, 8);
+    launch_count_state(launch_for(c), c->d_word, c->g.n, what, c->d_ctl);
+    CHK(read_ctl(c));
+    *out = (int64_t)c->h_ctl->count;
+    return SCC_OK;
+}
+
+scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
+    launch_ctl_reset(c->stream, c->d_ctl, 8);
+    launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
+    CHK(read_ctl(c));
+    *pairs = (int64_t)c->h_ctl->count;
+    return SCC_OK;
+}
+
+// update_wcc (wcc.cu:146-226): labels to fixpoint, fresh colours, election
+scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv) {
+    Launch l = launch_for(c);
+    launch_ctl_reset(c->stream, c->d_ctl, 16);
+    launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
+    launch_wcc_init(l, c->d_tmp, c->g.n);
+    int wg = 4;
+    CHK(run_chain(c, [&](int it) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
+                  wg, 0, iters));
+    launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
+    return do_elect(c, 0, npiv);
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void free_graph(scc_ctx *c) {
+    int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
+                        &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_piv};
+    for (auto b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    if (c->d_word) (void)hipFree(c->d_word);
+    c->d_word = nullptr;
+    for (auto &t : c->d_slots) {
+        if (t) (void)hipFree(t);
+        t = nullptr;
+    }
+    if (c->d_score) (void)hipFree(c->d_score);
+    c->d_score = nullptr;
+    c->uploaded = false;
+    c->labels_valid = false;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *scc_status_string(scc_status s) {
+    switch (s) {
+    case SCC_OK: return "ok";
+    case SCC_E_HIP: return "HIP runtime error";
+    case SCC_E_OOM: return "out of memory";
+    case SCC_E_INPUT: return "invalid input";
+    case SCC_E_OVERFLOW: return "size limit exceeded";
+    case SCC_E_STATE: return "invalid call order";
+    }
+    return "unknown status";
+}
+
+const char *scc_last_error(const scc_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
+    if (!out) return SCC_E_INPUT;
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return SCC_E_HIP;
+    if (device < 0 || device >= ndev) return SCC_E_INPUT;
+    scc_ctx *c = new (std::nothrow) scc_ctx();
+    if (!c) return SCC_E_OOM;
+    c->device = device;
+    if (opts) c->opts = *opts;
+    if (c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
+                                    (c->opts.block_size & 63) != 0)) {
+        delete c;
+        return SCC_E_INPUT;
+    }
+    if (hipSetDevice(device) != hipSuccess) { delete c; return SCC_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SCC_E_HIP;
+    }
+    if (hipMalloc(&c->d_ctl, sizeof(Ctl)) != hipSuccess ||
+        hipHostMalloc(&c->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
+        scc_destroy(c);
+        return SCC_E_OOM;
+    }
+    std::memset(c->h_ctl, 0, sizeof(Ctl));
+    if (hipMemcpy(c->d_ctl, c->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess) {
+        scc_destroy(c);
+        return SCC_E_HIP;
+    }
+    const char *ins = std::getenv("GPUSCC_INSTRUMENT");
+    c->instr = (ins && ins[0] == '1') ? 1 : 0;
+    *out = c;
+    return SCC_OK;
+}
+
+void scc_destroy(scc_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    if (c->d_ctl) (void)hipFree(c->d_ctl);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
+                      const int32_t *rowT, const int32_t *colT) {
+    if (!c) return SCC_E_INPUT;
+    if (n < 0 || m < 0) return fail(c, SCC_E_INPUT, "negative size");
+    if (m >= (int64_t)INT32_MAX || n >= INT32_MAX - 1) return fail(c, SCC_E_OVERFLOW, "graph too large for int32 CSR");
+    if (!row || (m > 0 && !col)) return fail(c, SCC_E_INPUT, "null CSR");
+    if (row[0] != 0 || row[n] != m) return fail(c, SCC_E_INPUT, "row[0] != 0 or row[n] != m");
+    if ((rowT == nullptr) != (colT == nullptr)) return fail(c, SCC_E_INPUT, "rowT/colT: both or neither");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_graph(c);
+    // validate rows and build the transpose on the host when it is not given
+    std::vector<int32_t> hrT, hcT;
+    for (int32_t v = 0; v < n; v++)
+        if (row[v + 1] < row[v]) return fail(c, SCC_E_INPUT, "row offsets decrease");
+    for (int64_t i = 0; i < m; i++)
+        if (col[i] < 0 || col[i] >= n) return fail(c, SCC_E_INPUT, "column id out of range");
+    if (!rowT) {
+        scc_host_graph tg;
+        std::vector<int32_t> src((size_t)m);
+        for (int32_t v = 0; v < n; v++)
+            for (int32_t i = row[v]; i < row[v + 1]; i++) src[i] = v;
+        scc_status s = csr_from_arcs(n, m, col, src.data(), 0, &tg);
+        if (s != SCC_OK) return fail(c, s, "transpose failed");
+        if (tg.m != m) {
+            graph_free(&tg);
+            return fail(c, SCC_E_INPUT, "CSR rows are not duplicate-free");
+        }
+        hrT.assign(tg.row, tg.row + n + 1);
+        hcT.assign(tg.col, tg.col + m);
+        graph_free(&tg);
+        rowT = hrT.data();
+        colT = hcT.data();
+    } else if (rowT[0] != 0 || rowT[n] != m) {
+        return fail(c, SCC_E_INPUT, "rowT[0] != 0 or rowT[n] != m");
+    }
+    size_t nb = sizeof(int32_t) * ((size_t)n + 1), mb = sizeof(int32_t) * std::max<size_t>((size_t)m, 1);
+    size_t vb = sizeof(int32_t) * std::max<size_t>((size_t)n, 1);
+    HIPCHK(c, hipMalloc(&c->d_row, nb));
+    HIPCHK(c, hipMalloc(&c->d_col, mb));
+    HIPCHK(c, hipMalloc(&c->d_rowT, nb));
+    HIPCHK(c, hipMalloc(&c->d_colT, mb));
+    HIPCHK(c, hipMalloc(&c->d_word, vb));
+    HIPCHK(c, hipMalloc(&c->d_root, vb));
+    HIPCHK(c, hipMalloc(&c->d_label, vb));
+    HIPCHK(c, hipMalloc(&c->d_tmp, vb));
+    HIPCHK(c, hipMalloc(&c->d_q[0], vb));
+    HIPCHK(c, hipMalloc(&c->d_q[1], vb));
+    HIPCHK(c, hipMalloc(&c->d_piv, vb));
+    // election table: about n slots (at least 2^10, at most 2^22), or the option
+    int sb = 10;
+    while (sb < 22 && ((int64_t)1 << sb) < (int64_t)n) sb++;
+    if (c->opts.slot_bits > 0) sb = std::min(24, std::max(1, (int)c->opts.slot_bits));
+    c->slot_bits = sb;
+    size_t sbytes = sizeof(unsigned long long) << sb;
+    HIPCHK(c, hipMalloc(&c->d_slots[0], sbytes));
+    HIPCHK(c, hipMalloc(&c->d_slots[1], sbytes));
+    HIPCHK(c, hipMemcpyAsync(c->d_row, row, nb, hipMemcpyHostToDevice, c->stream));
+    if (m > 0) HIPCHK(c, hipMemcpyAsync(c->d_col, col, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_rowT, rowT, nb, hipMemcpyHostToDevice, c->stream));
+    if (m > 0) HIPCHK(c, hipMemcpyAsync(c->d_colT, colT, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
+    c->g = Graph{n, m, c->d_row, c->d_col, c->d_rowT, c->d_colT};
+    Launch l = launch_for(c);
+    launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
+    launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
+    c->epoch = 0;
+    c->prev_tab = 0;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    return SCC_OK;
+}
+
+scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_t *labels_out,
+                   scc_stats *stats) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "scc_run before scc_upload");
+    if (schedule < SCC_SCHED_Q || schedule > SCC_SCHED_H) return fail(c, SCC_E_INPUT, "bad schedule");
+    if (variant < SCC_VARIANT_PLAIN || variant > SCC_VARIANT_2W) return fail(c, SCC_E_INPUT, "bad variant");
+    if (pivot_rule != SCC_PIVOT_PLAIN && pivot_rule != SCC_PIVOT_LEARNED)
+        return fail(c, SCC_E_INPUT, "bad pivot rule");
+    if (pivot_rule == SCC_PIVOT_LEARNED && !c->d_score)
+        return fail(c, SCC_E_STATE, "learned pivot rule needs scores (scc_mlp_score)");
+    HIPCHK(c, hipSetDevice(c->device));
+    scc_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.first_pivot = -1;
+    const int32_t n = c->g.n;
+    uint16_t *score_saved = c->d_score;
+    if (pivot_rule == SCC_PIVOT_PLAIN) c->d_score = nullptr;
+    c->syncs = 0;
+    c->st_expanded = c->st_edges = c->st_visited = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto t_start = std::chrono::steady_clock::now();
+    scc_status s = SCC_OK;
+    auto body = [&]() -> scc_status {
+        Launch l = launch_for(c);
+        launch_init(l, c->d_word, c->d_root, n, nullptr);
+        if (n == 0) return SCC_OK;
+        int64_t it = 0, npiv = 0, ne = 0;
+        auto tp = std::chrono::steady_clock::now();
+        CHK(do_trim(c, &it));  // trim1 (scc.cu:232)
+        st.trim_iters += it;
+        st.ms_trim += ms_since(tp);
+        tp = std::chrono::steady_clock::now();
+        CHK(do_elect(c, 0, &npiv));  // first pivot: lowest live id (scc.cu:235-245)
+        st.ms_update += ms_since(tp);
+        if (npiv > 0) {
+            int32_t fp = -1;
+            HIPCHK(c, hipMemcpy(&fp, c->d_piv, sizeof(int32_t), hipMemcpyDeviceToHost));
+            st.first_pivot = fp;
+        }
+        st.pivots_total += npiv;
+        bool wcc_done = false;
+        const int64_t one_pct = n / 100;
+        while (npiv > 0) {
+            st.steps++;
+            int sched = schedule;
+            if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1)
+                sched = ne < one_pct ? SCC_SCHED_T : SCC_SCHED_Q;  // scc.cu:493-563
+            int64_t d = 0;
+            tp = std::chrono::steady_clock::now();
+            CHK(do_reach(c, 0, sched, &d));
+            st.fwd_depths += d;
+            st.ms_fwd += ms_since(tp);
+            tp = std::chrono::steady_clock::now();
+            CHK(do_reach(c, 1, sched, &d));
+            st.bwd_depths += d;
+            st.ms_bwd += ms_since(tp);
+            tp = std::chrono::steady_clock::now();
+            CHK(do_trim(c, &it));
+            st.trim_iters += it;
+            st.ms_trim += ms_since(tp);
+            tp = std::chrono::steady_clock::now();
+            CHK(do_elect(c, 1, &npiv));
+            if (variant != SCC_VARIANT_PLAIN && !wcc_done) {
+                CHK(do_count(c, 0, &ne));
+                if (ne >= one_pct) {  // scc.cu:787-821
+                    if (ne == n) { st.pivots_total += npiv; st.ms_update += ms_since(tp); break; }
+                    wcc_done = true;
+                    st.wcc_step = st.steps;
+                    launch_unset_pivots(l, c->d_word, n);
+                    if (variant == SCC_VARIANT_2W) {
+                        int64_t pairs = 0;
+                        CHK(do_trim2(c, &pairs));
+                        st.trim2_pairs += pairs;
+                    }
+                    int64_t wi = 0;
+                    CHK(do_wcc(c, &wi, &npiv));
+                }
+            } else if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1 && ne < one_pct) {
+                CHK(do_count(c, 0, &ne));  // scc.cu:879-890
+            }
+            st.ms_update += ms_since(tp);
+            st.pivots_total += npiv;
+        }
+        // canonical labels + #SCC (scc.cu:912-919)
+        launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
+        CHK(do_count(c, 1, &st.num_scc));
+        return SCC_OK;
+    };
+    s = body();
+    c->d_score = score_saved;
+    if (s != SCC_OK) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st.ms_total = ms_since(t_start);
+    st.host_syncs = c->syncs;
+    c->labels_valid = true;
+    if (labels_out && n > 0) {
+        HIPCHK(c, hipMemcpy(labels_out, c->d_label, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+        std::vector<int32_t> cnt((size_t)n, 0);
+        for (int32_t v = 0; v < n; v++) cnt[labels_out[v]]++;
+        for (int32_t v = 0; v < n; v++) {
+            if (labels_out[v] != v) continue;
+            if (cnt[v] == 1) st.trivial++;
+            st.biggest = std::max<int64_t>(st.biggest, cnt[v]);
+        }
+    }
+    if (stats) *stats = st;
+    return SCC_OK;
+}
+
+scc_status scc_labels_device(scc_ctx *c, const int32_t **dptr) {
+    if (!c || !dptr) return SCC_E_INPUT;
+    if (!c->labels_valid) return fail(c, SCC_E_STATE, "no labels yet");
+    *dptr = c->d_label;
+    return SCC_OK;
+}
+
+// ---- phase-level entry points ---------------------------------------------
+scc_status scc_state_set(scc_ctx *c, const uint32_t *words, const int32_t *roots) {
+    if (!c || !words) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    size_t vb = sizeof(uint32_t) * (size_t)c->g.n;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (vb == 0) return SCC_OK;
+    HIPCHK(c, hipMemcpy(c->d_word, words, vb, hipMemcpyHostToDevice));
+    if (roots) {
+        HIPCHK(c, hipMemcpy(c->d_root, roots, vb, hipMemcpyHostToDevice));
+    } else {
+        std::vector<int32_t> r((size_t)c->g.n);
+        for (int32_t v = 0; v < c->g.n; v++) r[v] = v;
+        HIPCHK(c, hipMemcpy(c->d_root, r.data(), vb, hipMemcpyHostToDevice));
+    }
+    return SCC_OK;
+}
+
+scc_status scc_state_get(scc_ctx *c, uint32_t *words, int32_t *roots) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    size_t vb = sizeof(uint32_t) * (size_t)c->g.n;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (vb == 0) return SCC_OK;
+    if (words) HIPCHK(c, hipMemcpy(words, c->d_word, vb, hipMemcpyDeviceToHost));
+    if (roots) HIPCHK(c, hipMemcpy(roots, c->d_root, vb, hipMemcpyDeviceToHost));
+    return SCC_OK;
+}
+
+scc_status scc_phase_trim(scc_ctx *c, int64_t *iterations) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    int64_t it = 0;
+    if (c->g.n > 0) CHK(do_trim(c, &it));
+    if (iterations) *iterations = it;
+    return SCC_OK;
+}
+
+scc_status scc_phase_reach(scc_ctx *c, int dir, int schedule, int64_t *depth) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    if (dir != 0 && dir != 1) return fail(c, SCC_E_INPUT, "dir must be 0 or 1");
+    if (schedule < SCC_SCHED_Q || schedule > SCC_SCHED_H) return fail(c, SCC_E_INPUT, "bad schedule");
+    int64_t d = 0;
+    if (c->g.n > 0) {
+        if (schedule >= SCC_SCHED_D) {  // seeds: live vertices holding the bit
+            launch_ctl_reset(c->stream, c->d_ctl, 4);
+            launch_collect(launch_for(c), c->d_word, c->g.n, dir == 0 ? ST_F : ST_B, c->d_piv, c->d_ctl);
+        }
+        CHK(do_reach(c, dir, schedule, &d));
+    }
+    if (depth) *depth = d;
+    return SCC_OK;
+}
+
+scc_status scc_phase_update(scc_ctx *c, int recolour, int64_t *pivots) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    int64_t np = 0;
+    if (c->g.n > 0) CHK(do_elect(c, recolour ? 1 : 0, &np));
+    if (pivots) *pivots = np;
+    return SCC_OK;
+}
+
+scc_status scc_phase_trim2(scc_ctx *c, int64_t *pairs) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    int64_t p = 0;
+    if (c->g.n > 0) CHK(do_trim2(c, &p));
+    if (pairs) *pairs = p;
+    return SCC_OK;
+}
+
+scc_status scc_phase_wcc(scc_ctx *c, int64_t *iterations, int64_t *pivots) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    int64_t it = 0, np = 0;
+    if (c->g.n > 0) CHK(do_wcc(c, &it, &np));
+    if (iterations) *iterations = it;
+    if (pivots) *pivots = np;
+    return SCC_OK;
+}
+
+scc_status scc_phase_count_elimmed(scc_ctx *c, int64_t *count) {
+    if (!c || !count) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    *count = 0;
+    if (c->g.n > 0) CHK(do_count(c, 0, count));
+    return SCC_OK;
+}
+
+}  // extern "C"

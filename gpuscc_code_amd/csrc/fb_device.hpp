@@ -1,0 +1,108 @@
+// Device-side definitions shared by the FB kernels (gfx950, wave64).
+//
+// Vertex word (one u32 per vertex in HBM): bits 0..7 hold the reference's
+// state byte (scc.h:48-93: B=1 F=2 E=4 T=8 P=16 BExt=32 FExt=64; bit 128 marks
+// a trim-2 pair awaiting commit), bits 8..31 the colour.  Packing colour and
+// state into one word makes the per-edge test of reachability -- "live, not
+// yet visited, same colour" (reach_q.cu:32-33,69-70) -- a single 4-byte gather
+// instead of the reference's two (states[] u8 + ranges[] u32).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpuscc {
+
+constexpr uint32_t ST_B = 1, ST_F = 2, ST_E = 4, ST_T = 8, ST_P = 16, ST_BX = 32, ST_FX = 64;
+constexpr uint32_t ST_PAIR = 128;
+constexpr uint32_t COLOUR_MASK = 0x00FFFFFFu;
+constexpr int WAVE = 64;
+
+__host__ __device__ inline uint32_t colour_of(uint32_t w) { return w >> 8; }
+__host__ __device__ inline uint32_t make_word(uint32_t c, uint32_t s) { return (c << 8) | s; }
+
+// Election slot of a colour: top slot_bits bits of colour * 0x9E3779B1 (mod 2^32).
+__host__ __device__ inline uint32_t slot_of(uint32_t c, int slot_bits) {
+    return slot_bits == 0 ? 0u : (uint32_t)(c * 0x9E3779B1u) >> (32 - slot_bits);
+}
+
+// Election key, smaller wins: [63:48] inverted epoch tag, [47:32] inverted
+// score (0 for the plain rule), [31:0] vertex id.  A newer epoch has a smaller
+// inverted tag, so stale slots never need clearing between elections.
+__host__ __device__ inline uint64_t elect_tag(uint32_t epoch) {
+    return (uint64_t)(0xFFFFu - (epoch & 0xFFFFu)) << 48;
+}
+
+// Device control block: level-chain flags, queue sizes and reduction scratch.
+struct Ctl {
+    uint32_t changed[4];   // chain ring: kernel i sets [i&3] on progress, clears [(i+1)&3]
+    uint32_t qsize[4];     // queue chain ring: input size of level i is [i&3]
+    uint32_t ran;          // chain kernels that did work (depth / iterations)
+    uint32_t npiv;         // pivots elected by the last election = pivot list length
+    uint32_t maxc;         // max colour (reduction scratch)
+    uint32_t mode_sweeps;  // hybrid: levels that ran as sweeps
+    unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
+    unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
+};
+
+// Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;
+// remapping the block index gives each XCD one contiguous share of the vertex
+// range, so neighbouring ids (mesh rows, CSR locality) share that XCD's L2.
+__device__ inline uint32_t xcd_block() {
+    uint32_t b = blockIdx.x, g = gridDim.x;
+    if ((g & 7u) != 0) return b;
+    return (b & 7u) * (g >> 3) + (b >> 3);
+}
+
+// contiguous share of [0, n) for the calling workgroup (XCD-aware block id)
+__device__ inline void my_range(int64_t n, int64_t &beg, int64_t &end) {
+    uint32_t g = gridDim.x, lb = xcd_block();
+    int64_t chunk = ((n + g - 1) / g + 63) & ~(int64_t)63;
+    beg = (int64_t)lb * chunk;
+    end = beg + chunk < n ? beg + chunk : n;
+}
+
+// Gate of a chained level kernel: block 0 clears the flag of the next level;
+// a level whose predecessor made no progress exits at once.
+__device__ inline bool chain_exit(Ctl *ctl, int iter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->changed[(iter + 1) & 3] = 0;
+    if (iter > 0 && __atomic_load_n(&ctl->changed[(iter - 1) & 3], __ATOMIC_RELAXED) == 0)
+        return true;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->ran = (uint32_t)(iter + 1);
+    return false;
+}
+
+__device__ inline void chain_progress(Ctl *ctl, int iter, bool found) {
+    if (__any(found) && (threadIdx.x & (WAVE - 1)) == 0) ctl->changed[iter & 3] = 1;
+}
+
+// Block-wide sum / max through LDS (blockDim.x <= 1024).
+template <typename T>
+__device__ inline T block_sum(T v, T *sh) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
+    int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x + WAVE - 1) / WAVE; i++) r += sh[i];
+    return r;  // valid in thread 0
+}
+
+template <typename T>
+__device__ inline T block_max(T v, T *sh) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        T x = __shfl_down(v, o, WAVE);
+        v = x > v ? x : v;
+    }
+    int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x + WAVE - 1) / WAVE; i++) r = sh[i] > r ? sh[i] : r;
+    return r;
+}
+
+}  // namespace gpuscc

@@ -1,0 +1,468 @@
+// FB bookkeeping kernels for gfx950: vertex-word init, trim (trim-1 within
+// colours), colour update + pivot election, trim-2, the WCC relabel pass,
+// canonical labels and the small reductions.  All are grid-stride kernels on a
+// persistent grid (a few workgroups per CU), so a level that has no work costs
+// one short launch.
+#include "fb_device.hpp"
+#include "kernels.hpp"
+
+namespace gpuscc {
+
+// ---------------------------------------------------------------------------
+// init: colour from init_colour (or 1, thrust::fill scc.cu:214), state 0,
+// root = self
+// ---------------------------------------------------------------------------
+__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const uint32_t *init_colour) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t c = init_colour ? (init_colour[v] & COLOUR_MASK) : 1u;
+        word[v] = make_word(c, 0);
+        root[v] = (int32_t)v;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t count, unsigned long long val) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t count, int32_t val) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+// chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in graphs)
+__global__ void k_ctl_reset(Ctl *ctl, int what) {
+    if (threadIdx.x != 0) return;
+    if (what & 1) {
+        for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
+        ctl->ran = 0;
+        ctl->mode_sweeps = 0;
+    }
+    if (what & 2) {
+        // queue chain: level 0 reads the pivot list (length npiv)
+        ctl->qsize[0] = ctl->npiv;
+        for (int i = 1; i < 4; i++) ctl->qsize[i] = 0;
+    }
+    if (what & 4) ctl->npiv = 0;
+    if (what & 8) ctl->count = 0;
+    if (what & 16) ctl->maxc = 0;
+    if (what & 32) {
+        ctl->st_expanded = 0;
+        ctl->st_edges = 0;
+        ctl->st_visited = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// trim within colours, one sweep: trim_kernel (scc_routines.cu:336-394).
+// Live v with no live same-colour out-neighbour other than itself, or no such
+// in-neighbour, becomes state 28 (P|T|E) and its own SCC.
+// ---------------------------------------------------------------------------
+__device__ inline bool has_live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                                   const uint32_t *word, int32_t v, uint32_t c) {
+    int32_t e0 = row[v], e1 = row[v + 1];
+    for (int32_t e = e0; e < e1; e++) {
+        int32_t u = col[e];
+        if (u == v) continue;
+        uint32_t wu = word[u];
+        if (!(wu & ST_E) && colour_of(wu) == c) return true;
+    }
+    return false;
+}
+
+__global__ void k_trim_sweep(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                             const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
+                             uint32_t *word, int32_t *root, int32_t n, Ctl *ctl, int iter) {
+    if (chain_exit(ctl, iter)) return;
+    int64_t beg, end;
+    my_range(n, beg, end);
+    bool found = false;
+    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
+        int32_t v = (int32_t)vv;
+        uint32_t w = word[v];
+        if (w & ST_E) continue;
+        uint32_t c = colour_of(w);
+        if (!has_live_nb(row, col, word, v, c) || !has_live_nb(rowT, colT, word, v, c)) {
+            word[v] = make_word(c, 28);
+            root[v] = v;
+            found = true;
+        }
+    }
+    chain_progress(ctl, iter, found);
+}
+
+// ---------------------------------------------------------------------------
+// colour update + election: update_kernel (scc_routines.cu:187-224).
+// recolour: live v with F and B joins the SCC of the pivot elected for its
+// colour last time (E set, root = that pivot); other live v take colour
+// 3c+[F]+2[B] and state 0.  Every live v then offers its key to the slot of
+// its colour with a 64-bit atomicMin (lowest key wins, deterministic).
+// ---------------------------------------------------------------------------
+__global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
+                         const unsigned long long *__restrict__ prev, unsigned long long *cur,
+                         int slot_bits, unsigned long long tag, int recolour,
+                         const uint16_t *__restrict__ score) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    int lane = threadIdx.x & (WAVE - 1);
+    for (int64_t base = beg; base < end; base += blockDim.x) {
+        int64_t vv = base + threadIdx.x;
+        bool cand = false;
+        uint32_t sl = 0;
+        unsigned long long key = ~0ull;
+        if (vv < end) {
+            int32_t v = (int32_t)vv;
+            uint32_t w = word[v];
+            if (!(w & ST_E)) {
+                uint32_t c = colour_of(w);
+                bool elim = false;
+                if (recolour) {
+                    uint32_t k = ((w & ST_F) ? 1u : 0u) + ((w & ST_B) ? 2u : 0u);
+                    if (k == 3) {
+                        word[v] = w | ST_E;
+                        uint32_t pv = (uint32_t)prev[slot_of(c, slot_bits)];
+                        root[v] = pv < (uint32_t)n ? (int32_t)pv : v;
+                        elim = true;
+                    } else {
+                        c = (3u * c + k) & COLOUR_MASK;
+                        word[v] = make_word(c, 0);
+                    }
+                }
+                if (!elim) {
+                    cand = true;
+                    sl = slot_of(c, slot_bits);
+                    uint32_t sc = score ? (uint32_t)(0xFFFFu - score[v]) : 0u;
+                    key = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
+                }
+            }
+        }
+        // wave-level combine when every candidate lane targets the same slot
+        // (the first election sends all live vertices to one slot)
+        unsigned long long mask = __ballot(cand);
+        if (mask == 0) continue;
+        int first = __ffsll((long long)mask) - 1;
+        uint32_t sl0 = __shfl(sl, first, WAVE);
+        bool same = __all(!cand || sl == sl0);
+        if (same) {
+            unsigned long long k = key;
+            for (int o = WAVE / 2; o > 0; o >>= 1) {
+                unsigned long long x = __shfl_xor(k, o, WAVE);
+                k = x < k ? x : k;
+            }
+            if (lane == first && k < cur[sl0]) atomicMin(&cur[sl0], k);
+        } else if (cand && key < cur[sl]) {
+            atomicMin(&cur[sl], key);
+        }
+    }
+}
+
+// Winners become pivots (state 19 = P|F|B, scc_routines.cu:219) and are
+// appended to the pivot list through an LDS-staged block queue.
+__global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *__restrict__ cur,
+                          int slot_bits, int32_t *piv, Ctl *ctl) {
+    __shared__ int32_t sq[1024];
+    __shared__ uint32_t scount, sbase;
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t base = beg; base < end; base += blockDim.x) {
+        if (threadIdx.x == 0) scount = 0;
+        __syncthreads();
+        int64_t vv = base + threadIdx.x;
+        if (vv < end) {
+            int32_t v = (int32_t)vv;
+            uint32_t w = word[v];
+            if (!(w & ST_E)) {
+                uint32_t c = colour_of(w);
+                if ((uint32_t)cur[slot_of(c, slot_bits)] == (uint32_t)v) {
+                    word[v] = make_word(c, 19);
+                    sq[atomicAdd(&scount, 1u)] = v;
+                }
+            }
+        }
+        __syncthreads();
+        if (scount > 0) {
+            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, scount);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < scount; i += blockDim.x) piv[sbase + i] = sq[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// what 0: FB-eliminated not trimmed ((s&12)==4, countElimmed scc.cu:42-51)
+// what 1: P bits (#SCC, scc.cu:914-919)
+__global__ void k_count_state(const uint32_t *word, int32_t n, int what, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
+    int64_t beg, end;
+    my_range(n, beg, end);
+    unsigned long long c = 0;
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t s = word[v] & 0xFFu;
+        if (what == 0) c += ((s & 12u) == 4u);
+        else c += ((s & ST_P) != 0);
+    }
+    c = block_sum(c, sh);
+    if (threadIdx.x == 0 && c) atomicAdd(&ctl->count, c);
+}
+
+// max colour over all vertices (thrust::reduce maximum, scc.cu:802)
+__global__ void k_max_colour(const uint32_t *word, int32_t n, Ctl *ctl) {
+    __shared__ uint32_t sh[16];
+    int64_t beg, end;
+    my_range(n, beg, end);
+    uint32_t m = 0;
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t c = colour_of(word[v]);
+        m = c > m ? c : m;
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) atomicMax(&ctl->maxc, m);
+}
+
+// state 19 -> 0: unsetPivots_kernel (scc_routines.cu:10-19)
+__global__ void k_unset_pivots(uint32_t *word, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t w = word[v];
+        if ((w & 0xFFu) == 19u) word[v] = w & ~0xFFu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// trim-2, one pass: trim2_kernel (scc_routines.cu:397-523).  Pass A marks
+// pairs with ST_PAIR (not E), so every test sees the states as they were
+// before the pass; pass B commits 20 (P|E) to the lower id and 4 (E) to the
+// other.
+// ---------------------------------------------------------------------------
+__device__ inline int32_t only_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                                  const uint32_t *word, int32_t v, uint32_t c) {
+    int32_t found = -1;
+    for (int32_t e = row[v]; e < row[v + 1]; e++) {
+        int32_t u = col[e];
+        if (u == v) continue;
+        uint32_t wu = word[u];
+        if ((wu & ST_E) || colour_of(wu) != c) continue;
+        if (found >= 0) return -2;
+        found = u;
+    }
+    return found;
+}
+
+__global__ void k_trim2_mark(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                             const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
+                             uint32_t *word, int32_t *root, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
+        int32_t v = (int32_t)vv;
+        uint32_t w = word[v];
+        if (w & ST_E) continue;
+        uint32_t c = colour_of(w);
+        int32_t u = -1;
+        int32_t a = only_nb(row, col, word, v, c);
+        if (a >= 0 && only_nb(row, col, word, a, c) == v) u = a;
+        if (u < 0) {
+            a = only_nb(rowT, colT, word, v, c);
+            if (a >= 0 && only_nb(rowT, colT, word, a, c) == v) u = a;
+        }
+        if (u < 0) continue;
+        int32_t lo = v < u ? v : u;
+        word[v] = make_word(c, ST_PAIR | (v == lo ? ST_P : 0u));
+        root[v] = lo;
+    }
+}
+
+__global__ void k_trim2_commit(uint32_t *word, int32_t n, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
+    int64_t beg, end;
+    my_range(n, beg, end);
+    unsigned long long pairs = 0;
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t w = word[v];
+        if (!(w & ST_PAIR)) continue;
+        uint32_t p = w & ST_P;
+        word[v] = make_word(colour_of(w), p | ST_E);
+        pairs += p ? 1 : 0;
+    }
+    pairs = block_sum(pairs, sh);
+    if (threadIdx.x == 0 && pairs) atomicAdd(&ctl->count, pairs);
+}
+
+// ---------------------------------------------------------------------------
+// one WCC pass: computeWCC (wcc.cu:7-52) to its fixpoint gives each live v
+// the minimum id among live vertices it reaches over same-colour out-edges;
+// pointer jumping (wcc.cu:42-46) shortcuts chains.
+// ---------------------------------------------------------------------------
+__global__ void k_wcc_init(int32_t *lab, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) lab[v] = (int32_t)v;
+}
+
+__global__ void k_wcc_iter(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                           const uint32_t *__restrict__ word, int32_t *lab, int32_t n, Ctl *ctl,
+                           int iter) {
+    if (chain_exit(ctl, iter)) return;
+    int64_t beg, end;
+    my_range(n, beg, end);
+    bool found = false;
+    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
+        int32_t v = (int32_t)vv;
+        uint32_t w = word[v];
+        if (w & ST_E) continue;
+        uint32_t c = colour_of(w);
+        int32_t cur = lab[v], m = cur;
+        for (int32_t e = row[v]; e < row[v + 1]; e++) {
+            int32_t u = col[e];
+            uint32_t wu = word[u];
+            if ((wu & ST_E) || colour_of(wu) != c) continue;
+            int32_t lu = lab[u];
+            m = lu < m ? lu : m;
+        }
+        int32_t j = lab[m];  // pointer jump: the label of a reached vertex is reached too
+        m = j < m ? j : m;
+        if (m < cur) {
+            lab[v] = m;
+            found = true;
+        }
+    }
+    chain_progress(ctl, iter, found);
+}
+
+// fresh colours per class (update_ranges2/3, wcc.cu:79-144): every member of
+// the class rooted at r takes (maxc + 1 + r) & mask, state 0.
+__global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, const Ctl *ctl) {
+    uint32_t maxc = ctl->maxc;
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t w = word[v];
+        if (w & ST_E) continue;
+        word[v] = make_word((maxc + 1u + (uint32_t)lab[v]) & COLOUR_MASK, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// canonical labels: label = minimum id of the class of root[] (root[r] == r
+// for every class representative r)
+// ---------------------------------------------------------------------------
+__global__ void k_label_seed(const int32_t *root, int32_t *mn, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x)
+        if (root[v] == (int32_t)v) mn[v] = (int32_t)v;
+}
+
+__global__ void k_label_min(const int32_t *root, int32_t *mn, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        int32_t r = root[v];
+        if ((int32_t)v < mn[r]) atomicMin(&mn[r], (int32_t)v);
+    }
+}
+
+__global__ void k_label_final(const int32_t *root, const int32_t *mn, int32_t *label, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) label[v] = mn[root[v]];
+}
+
+// seeds of a reach given as vertex words: live vertices holding `bit`
+__global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t *q, Ctl *ctl) {
+    __shared__ int32_t sq[1024];
+    __shared__ uint32_t scount, sbase;
+    int64_t beg, end;
+    my_range(n, beg, end);
+    for (int64_t base = beg; base < end; base += blockDim.x) {
+        if (threadIdx.x == 0) scount = 0;
+        __syncthreads();
+        int64_t v = base + threadIdx.x;
+        if (v < end) {
+            uint32_t w = word[v];
+            if ((w & (bit | ST_E)) == bit) sq[atomicAdd(&scount, 1u)] = (int32_t)v;
+        }
+        __syncthreads();
+        if (scount > 0) {
+            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, scount);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < scount; i += blockDim.x) q[sbase + i] = sq[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
+
+void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const uint32_t *ic) {
+    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, ic);
+}
+void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v) {
+    hipLaunchKernelGGL(k_fill_u64, GRID(l), p, count, v);
+}
+void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
+    hipLaunchKernelGGL(k_fill_i32, GRID(l), p, count, v);
+}
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what);
+}
+void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl,
+                       int iter) {
+    hipLaunchKernelGGL(k_trim_sweep, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl,
+                       iter);
+}
+void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
+                   const unsigned long long *prev, unsigned long long *cur, int slot_bits,
+                   unsigned long long tag, int recolour, const uint16_t *score) {
+    hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, score);
+}
+void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
+                    int slot_bits, int32_t *piv, Ctl *ctl) {
+    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl);
+}
+void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl) {
+    hipLaunchKernelGGL(k_count_state, GRID(l), word, n, what, ctl);
+}
+void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl) {
+    hipLaunchKernelGGL(k_max_colour, GRID(l), word, n, ctl);
+}
+void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n) {
+    hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n);
+}
+void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl) {
+    hipLaunchKernelGGL(k_trim2_mark, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n);
+    hipLaunchKernelGGL(k_trim2_commit, GRID(l), word, g.n, ctl);
+}
+void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n) {
+    hipLaunchKernelGGL(k_wcc_init, GRID(l), lab, n);
+}
+void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,
+                     int iter) {
+    hipLaunchKernelGGL(k_wcc_iter, GRID(l), g.row, g.col, word, lab, g.n, ctl, iter);
+}
+void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
+                         const Ctl *ctl) {
+    hipLaunchKernelGGL(k_wcc_recolour, GRID(l), word, lab, n, ctl);
+}
+void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n) {
+    hipLaunchKernelGGL(k_label_seed, GRID(l), root, mn, n);
+    hipLaunchKernelGGL(k_label_min, GRID(l), root, mn, n);
+    hipLaunchKernelGGL(k_label_final, GRID(l), root, mn, label, n);
+}
+void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
+                    Ctl *ctl) {
+    hipLaunchKernelGGL(k_collect, GRID(l), word, n, bit, q, ctl);
+}
+
+}  // namespace gpuscc

@@ -1,0 +1,175 @@
+/*
+ * gpuscc.h -- C ABI of the MI355X (gfx950) Forward-Backward SCC library.
+ *
+ * The reference (chenxuhao/gpuscc_code) has no FFI: its path sits behind the
+ * `./scc [-t] [-a q|t|d|l|h] [-b N] -f graph` process interface (scc.cu:57-148)
+ * and an internal C++ header API of phase functions on device pointers
+ * (reach_q.h:8-21, reach_d.h:14-25, scc_routines.h:8-50, wcc.h:18-30,
+ * load.h:5, tarjan.h:5).  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions: plain C types only; vertex ids are 0-based (the reference's
+ * files and arrays are 1-based, load.cpp:128-139); the CSR is int32
+ * (row[n+1], col[m]), rows sorted ascending and duplicate-free.  Host buffers
+ * are caller-owned; device buffers are context-owned; one HIP stream per
+ * context; a context is not thread-safe.  Every function returns an scc_status
+ * and never calls exit() (the reference exits on any error, GPUerrchk.h:6-12).
+ */
+#ifndef GPUSCC_H
+#define GPUSCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SCC_OK = 0,
+    SCC_E_HIP = 1,       /* a HIP runtime call failed (no GPU, launch error, ...) */
+    SCC_E_OOM = 2,       /* device or host allocation failed */
+    SCC_E_INPUT = 3,     /* malformed graph, file or argument */
+    SCC_E_OVERFLOW = 4,  /* a size exceeds a documented limit (never a silent drop) */
+    SCC_E_STATE = 5      /* call out of order (e.g. run before upload) */
+} scc_status;
+
+/* Reachability schedules: reach_strategy (scc.h:7-16), CLI -a q|t|d|l|h (scc.cu:94-104). */
+typedef enum {
+    SCC_SCHED_Q = 0,  /* topology sweep, one lane per vertex (reach_q.cu) */
+    SCC_SCHED_T = 1,  /* topology sweep with workgroup edge balancing (reach_t.cu) */
+    SCC_SCHED_D = 2,  /* data-driven frontier queues (reach_d.cu) */
+    SCC_SCHED_L = 3,  /* frontier queues with load-balanced edge expansion (reach_l.cu) */
+    SCC_SCHED_H = 4   /* hybrid: switches schedule by frontier size (scc.cu:493-563) */
+} scc_schedule;
+
+/* The three binaries of Makefile:12-19. */
+typedef enum {
+    SCC_VARIANT_PLAIN = 0,  /* scc */
+    SCC_VARIANT_W = 1,      /* scc_w: one WCC pass once >= n/100 vertices are FB-eliminated */
+    SCC_VARIANT_2W = 2      /* scc_2_w: scc_w plus one trim-2 pass before the WCC pass */
+} scc_variant;
+
+typedef enum {
+    SCC_PIVOT_PLAIN = 0,    /* lowest live id per colour slot (the reference rule, made deterministic) */
+    SCC_PIVOT_LEARNED = 1   /* highest MLP score per colour slot (needs scc_mlp_set_params) */
+} scc_pivot_rule;
+
+/* Options: alg_opt (scc.h:19-27).  Zero means "default". */
+typedef struct {
+    int32_t block_size;        /* workgroup size for sweeps (-b, scc.cu:82); default 256 */
+    int32_t hybrid_policy;     /* 0: switch per level by frontier size; 1: the reference's rule,
+                                  sweep-with-balancing until >= n/100 FB-eliminated, then q */
+    int32_t hybrid_threshold;  /* h: frontier size (vertices) at or above which a level sweeps */
+    int32_t slot_bits;         /* log2 of the election table size; default from n */
+    int32_t grid_blocks;       /* workgroups of the grid-stride kernels; default from the CU count */
+    int32_t reserved[3];
+} scc_opts;
+
+/* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
+typedef struct {
+    int64_t steps;          /* FB steps (->steps) */
+    int64_t fwd_depths;     /* sum of forward levels (->sum of fwdDepths) */
+    int64_t bwd_depths;     /* sum of backward levels */
+    int64_t trim_iters;     /* sum of trim sweeps */
+    int64_t num_scc;        /* count of P bits (->count of SCC, scc.cu:914-919) */
+    int64_t pivots_total;   /* pivots elected over the run */
+    int64_t wcc_step;       /* step at which the WCC pass ran, 0 if never */
+    int64_t trim2_pairs;    /* pairs removed by trim-2 */
+    int64_t trivial;        /* size-1 SCCs (tarjan.cpp:237) */
+    int64_t biggest;        /* largest SCC size */
+    int64_t host_syncs;     /* device->host waits the run made */
+    int64_t first_pivot;    /* first pivot (lowest live id after trim-1, scc.cu:235-245), -1 if none */
+    double ms_total;        /* upload-excluded wall time, labels ready (scc.cu:217->1011) */
+    double ms_fwd, ms_bwd, ms_trim, ms_update; /* per phase, host clock (STOPWATCH, timing.h:14) */
+} scc_stats;
+
+typedef struct scc_ctx scc_ctx;
+
+/* Host-side graph owned by the library (scc_graph_free releases it). */
+typedef struct {
+    int32_t n;
+    int64_t m;              /* arcs after dedup */
+    int32_t *row, *col;     /* CSR, row[n+1] */
+    int32_t *rowT, *colT;   /* transposed CSR */
+    int64_t m_reported;     /* load.cpp's countOfEdges (nnz, or nnz + mirrored for symmetric) */
+    int32_t pattern, symmetric, mtx;
+    int32_t max_in, min_in, max_out, min_out;
+    double average;         /* nnz / n, as load.cpp:60 */
+    int32_t num_graphs;     /* batched graphs only: number of graphs, else 0 */
+    int32_t *graph_offsets; /* batched graphs only: vertex offsets [num_graphs+1] */
+} scc_host_graph;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the device setup of computeSCC (scc.cu:199-215). */
+scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out);
+void scc_destroy(scc_ctx *ctx);
+const char *scc_status_string(scc_status s);
+const char *scc_last_error(const scc_ctx *ctx);
+
+/* Copies a CSR + CSR^T into HBM (replaces the cudaMalloc/cudaMemcpy of
+ * scc.cu:200-211).  rowT/colT may be NULL: the library then builds the
+ * transpose (on the host) before copying.  n >= 0, 0 <= m < 2^31. */
+scc_status scc_upload(scc_ctx *ctx, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
+                      const int32_t *rowT, const int32_t *colT);
+
+/* Whole FB run: computeSCC (scc.cu:150-1022) for one schedule and variant.
+ * labels_out (host, n entries, may be NULL) receives canonical labels: the
+ * minimum vertex id of each vertex's SCC.  stats may be NULL. */
+scc_status scc_run(scc_ctx *ctx, int schedule, int variant, int pivot_rule, int32_t *labels_out,
+                   scc_stats *stats);
+
+/* Device address of the labels of the last run (valid until the next upload). */
+scc_status scc_labels_device(scc_ctx *ctx, const int32_t **dptr);
+
+/* ---- phase-level entry points (tests, autotuner) -------------------------
+ * They act on the context's vertex words: word = colour << 8 | state, with the
+ * state bits of scc.h:48-93 (B=1 F=2 E=4 T=8 P=16 BExt=32 FExt=64) and a
+ * 24-bit colour. */
+scc_status scc_state_set(scc_ctx *ctx, const uint32_t *words, const int32_t *roots);
+scc_status scc_state_get(scc_ctx *ctx, uint32_t *words, int32_t *roots);
+/* trim1/trim (scc_routines.cu:584-643): trim within colours to the fixpoint. */
+scc_status scc_phase_trim(scc_ctx *ctx, int64_t *iterations);
+/* fwd_reach_X / bwd_reach_X (reach_q.h:8-21, reach_t.h, reach_d.h, reach_l.h):
+ * dir 0 = forward over the CSR (bit F), 1 = backward over CSR^T (bit B).
+ * Seeds are the live vertices holding the bit. */
+scc_status scc_phase_reach(scc_ctx *ctx, int dir, int schedule, int64_t *depth);
+/* update (scc_routines.cu:526-555): recolour = 1 splits colours and eliminates
+ * F&B; 0 only elects (first pivot, scc.cu:235-245). */
+scc_status scc_phase_update(scc_ctx *ctx, int recolour, int64_t *pivots);
+/* trim2 (scc_routines.cu:645-668): one pass. */
+scc_status scc_phase_trim2(scc_ctx *ctx, int64_t *pairs);
+/* update_wcc (wcc.cu:146-226): WCC labels, fresh colours, election. */
+scc_status scc_phase_wcc(scc_ctx *ctx, int64_t *iterations, int64_t *pivots);
+/* countElimmed (scc.cu:42-51). */
+scc_status scc_phase_count_elimmed(scc_ctx *ctx, int64_t *count);
+
+/* ---- host input path (no GPU needed) ------------------------------------- */
+/* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
+ * "a b" is the arc b->a; self-loops kept, duplicates dropped. */
+scc_status scc_load_file(const char *path, scc_host_graph *g);
+/* Builds CSR + CSR^T from arcs src[i]->dst[i] (0-based): rows sorted,
+ * duplicates removed, self-loops dropped when drop_self_loops != 0. */
+scc_status scc_csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t *dst,
+                             int drop_self_loops, scc_host_graph *g);
+/* R-MAT (a, b, c, d = 1-a-b-c), 2^scale vertices, edge_factor * 2^scale arcs
+ * drawn; self-loops and duplicates removed; permute != 0 relabels vertices by
+ * a seeded bijection. */
+scc_status scc_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                        int permute, scc_host_graph *g);
+/* side x side grid, id = r*side + c; each lattice edge is bidirectional with
+ * probability p_bidir, otherwise one uniformly random direction. */
+scc_status scc_gen_mesh(int side, double p_bidir, uint64_t seed, scc_host_graph *g);
+/* num_graphs ER digraphs packed into one CSR with graph_offsets: n_i
+ * log-uniform in [n_min, n_max], mean out-degree uniform in [deg_lo, deg_hi],
+ * plus planted directed cycles with Geometric(cycle_p) sizes over a
+ * cycle_cover fraction of each graph's vertices. */
+scc_status scc_gen_batch(int num_graphs, int n_min, int n_max, double deg_lo, double deg_hi,
+                         double cycle_p, double cycle_cover, uint64_t seed, scc_host_graph *g);
+void scc_graph_free(scc_host_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSCC_H */

@@ -1,0 +1,265 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bar: bit-exact.  Whole runs are compared as canonical labels (minimum vertex id
+of each SCC) with the oracle's Tarjan; because every election in the HIP path
+is deterministic (lowest id per slot), the final colour, state and class
+representative of every vertex must also equal the oracle's sequential FB
+restatement run with the same election table size.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gpuscc_code_amd import scc
+
+from helpers import bigrid, complete, cycle, graph_from_arcs, is_canonical, path_dag, planted, \
+    random_digraph
+
+pytestmark = pytest.mark.gpu
+
+SCHEDS = ["q", "t", "d", "l", "h"]
+DEMO_EXPECT = np.array([1, 2, 3, 4, 2, 2, 7, 8, 7, 7, 7, 7, 8]) - 1  # SCCs of demo.graph
+
+
+def default_slot_bits(n):
+    sb = 10
+    while sb < 22 and (1 << sb) < n:
+        sb += 1
+    return sb
+
+
+def demo_graph():
+    # the reference's demo.graph (13 vertices): line "a b" is the arc b -> a
+    lines = [(1, 1), (1, 12), (2, 5), (2, 11), (4, 3), (5, 4), (5, 6), (6, 2), (7, 8), (7, 11),
+             (8, 13), (9, 12), (10, 7), (10, 13), (11, 7), (11, 9), (11, 10), (12, 11), (13, 8)]
+    return graph_from_arcs(13, [(b - 1, a - 1) for a, b in lines])
+
+
+def check_against_oracle(ctx, g, sched, variant, full_state=True):
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    labels, st = ctx.run(sched, variant)
+    assert np.array_equal(labels, ref), f"labels differ (sched {sched}, variant {variant})"
+    assert st["num_scc"] == len(np.unique(ref))
+    if full_state:
+        o = oracle.fb(g.row, g.col, g.rowT, g.colT, variant=variant,
+                      slot_bits=default_slot_bits(g.n), hash_kind=1)
+        words, roots = ctx.get_state()
+        colour, state = scc.split_word(words)
+        assert st["steps"] == o["steps"]
+        assert np.array_equal(colour, o["colour"])
+        assert np.array_equal(state & 0x1C, o["state"] & 0x1C)  # E, T, P bits
+        assert np.array_equal(roots, o["root"])
+        assert st["wcc_step"] == o["wcc_step"]
+        assert st["trim2_pairs"] == o["trim2_pairs"]
+    return labels, st
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_demo_graph(ctx, sched, variant):
+    g = demo_graph()
+    labels, st = check_against_oracle(ctx, g, sched, variant)
+    assert np.array_equal(labels, DEMO_EXPECT)
+    assert st["num_scc"] == 6 and st["trivial"] == 3 and st["biggest"] == 5
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_rmat14_config(ctx, sched, variant):
+    # R-MAT scale 14, edge factor 8, (0.57, 0.19, 0.19), seed 1, dedup, no self-loops
+    g = scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1)
+    check_against_oracle(ctx, g, sched, variant)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_mesh_small(ctx, sched):
+    g = scc.gen_mesh(128, 0.6, seed=3)
+    for variant in (0, 1, 2):
+        check_against_oracle(ctx, g, sched, variant)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_batch_small(ctx, sched):
+    g = scc.gen_batch(48, 64, 2048, seed=4)
+    for variant in (0, 1, 2):
+        check_against_oracle(ctx, g, sched, variant)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_closed_forms(ctx, sched):
+    ctx.upload(cycle(1000))
+    lab, st = ctx.run(sched)
+    assert np.all(lab == 0) and st["num_scc"] == 1
+    ctx.upload(path_dag(777))
+    lab, st = ctx.run(sched)
+    assert np.array_equal(lab, np.arange(777)) and st["num_scc"] == 777
+    ctx.upload(complete(40))
+    lab, st = ctx.run(sched)
+    assert np.all(lab == 0) and st["num_scc"] == 1
+    ctx.upload(bigrid(40))
+    lab, st = ctx.run(sched)
+    assert np.all(lab == 0) and st["num_scc"] == 1
+    g, expect = planted(300, 7, 2000, seed=5)
+    ctx.upload(g)
+    lab, st = ctx.run(sched)
+    assert np.array_equal(lab, expect) and st["num_scc"] == 300
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_random_and_self_loops(ctx, sched):
+    for seed in range(3):
+        g = random_digraph(3000, 4500, seed, self_loops=True)
+        check_against_oracle(ctx, g, sched, seed % 3)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_edge_cases(ctx, sched):
+    # no vertices
+    ctx.upload(scc.csr_from_arcs(0, [], []))
+    lab, st = ctx.run(sched)
+    assert len(lab) == 0 and st["num_scc"] == 0
+    # one vertex, with and without a self-loop
+    ctx.upload(scc.csr_from_arcs(1, [], []))
+    lab, st = ctx.run(sched)
+    assert list(lab) == [0] and st["num_scc"] == 1
+    ctx.upload(graph_from_arcs(1, [(0, 0)]))
+    lab, st = ctx.run(sched)
+    assert list(lab) == [0] and st["num_scc"] == 1
+    # vertices without arcs
+    ctx.upload(scc.csr_from_arcs(500, [], []))
+    lab, st = ctx.run(sched)
+    assert np.array_equal(lab, np.arange(500))
+    # a single 2-cycle plus isolated vertices; trim-2 pair in scc_2_w
+    g = graph_from_arcs(6, [(4, 1), (1, 4)])
+    for variant in (0, 1, 2):
+        check_against_oracle(ctx, g, sched, variant)
+
+
+def test_upload_errors(ctx):
+    with pytest.raises(scc.SccError):
+        ctx.upload(scc.Graph(n=3, row=np.array([0, 1, 2, 5], np.int32), col=np.array([1, 2, 0], np.int32),
+                             rowT=np.array([0, 1, 2, 3], np.int32), colT=np.array([2, 0, 1], np.int32)))
+    with pytest.raises(scc.SccError):  # column id out of range
+        ctx.upload(scc.Graph(n=2, row=np.array([0, 1, 2], np.int32), col=np.array([1, 7], np.int32),
+                             rowT=np.array([0, 1, 2], np.int32), colT=np.array([1, 0], np.int32)))
+    # a good upload afterwards still works; transpose built by the library
+    g = cycle(10)
+    ctx.upload(g, transpose=False)
+    lab, _ = ctx.run("q")
+    assert np.all(lab == 0)
+
+
+# ---------------------------------------------------------------------------
+# phase-level parity on random states
+# ---------------------------------------------------------------------------
+def random_state(n, rng, ncol=3, p_elim=0.2):
+    colour = rng.integers(1, ncol + 1, n).astype(np.uint32)
+    state = np.where(rng.random(n) < p_elim, 4, 0).astype(np.uint8)
+    return colour, state
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+@pytest.mark.parametrize("direction", [0, 1])
+def test_phase_reach(ctx, sched, direction):
+    rng = np.random.default_rng(7 + direction)
+    g = random_digraph(20000, 60000, 11)
+    ctx.upload(g)
+    colour, state = random_state(g.n, rng)
+    bit = 2 if direction == 0 else 1
+    seeds = rng.choice(g.n, 30, replace=False)
+    state[seeds] = np.where(state[seeds] & 4, state[seeds], state[seeds] | bit)
+    ctx.set_state(scc.make_word(colour, state))
+    depth = ctx.reach(direction, sched)
+    assert depth >= 1
+    words, _ = ctx.get_state()
+    _, gst = scc.split_word(words)
+    ost = state.copy()
+    if direction == 0:
+        oracle.reach(g.row, g.col, colour, ost, bit)
+    else:
+        oracle.reach(g.rowT, g.colT, colour, ost, bit)
+    assert np.array_equal(gst & 0x1F, ost & 0x1F)
+    assert np.array_equal(scc.split_word(words)[0], colour)
+
+
+def test_phase_trim(ctx):
+    rng = np.random.default_rng(3)
+    g = random_digraph(20000, 30000, 12, self_loops=True)
+    ctx.upload(g)
+    colour, state = random_state(g.n, rng)
+    roots = np.arange(g.n, dtype=np.int32)
+    ctx.set_state(scc.make_word(colour, state), roots)
+    it = ctx.trim()
+    assert it >= 1
+    words, groots = ctx.get_state()
+    ost, oroot = state.copy(), roots.copy()
+    oracle.trim(g.row, g.col, g.rowT, g.colT, colour, ost, oroot)
+    assert np.array_equal(scc.split_word(words)[1], ost)
+    assert np.array_equal(groots, oroot)
+
+
+def test_phase_update_elect():
+    rng = np.random.default_rng(5)
+    g = random_digraph(5000, 10000, 13)
+    sb = 6  # a small table makes slot collisions certain
+    with scc.Context(0, scc.alg_opt(slot_bits=sb)) as c:
+        c.upload(g)
+        colour, state = random_state(g.n, rng, ncol=200)
+        roots = np.arange(g.n, dtype=np.int32)
+        c.set_state(scc.make_word(colour, state), roots)
+        npiv = c.update(recolour=False)
+        ocol, ost, oroot = colour.copy(), state.copy(), roots.copy()
+        onp, owin = oracle.update_elect(ocol, ost, oroot, oracle.COLOUR_MASK_PACKED, sb, 1, None, False)
+        words, _ = c.get_state()
+        assert npiv == onp
+        assert np.array_equal(scc.split_word(words)[1], ost)
+        # mark F/B on vertices of pivoted colours, then split colours
+        pivc = set(ocol[ost == 19].tolist())
+        fb = rng.integers(0, 4, g.n).astype(np.uint8)
+        live = (ost & 4) == 0
+        sel = live & np.isin(ocol, list(pivc)) & (ost != 19)
+        ost2 = ost.copy()
+        ost2[sel] = fb[sel]
+        c.set_state(scc.make_word(ocol, ost2), oroot)
+        npiv2 = c.update(recolour=True)
+        words, groots = c.get_state()
+        onp2, _ = oracle.update_elect(ocol, ost2, oroot, oracle.COLOUR_MASK_PACKED, sb, 1, owin, True)
+        gc, gs = scc.split_word(words)
+        assert npiv2 == onp2
+        assert np.array_equal(gc, ocol)
+        assert np.array_equal(gs, ost2)
+        assert np.array_equal(groots, oroot)
+
+
+def test_phase_trim2_and_wcc(ctx):
+    rng = np.random.default_rng(9)
+    g = random_digraph(20000, 22000, 14)
+    ctx.upload(g)
+    colour, state = random_state(g.n, rng, ncol=2, p_elim=0.1)
+    roots = np.arange(g.n, dtype=np.int32)
+    ctx.set_state(scc.make_word(colour, state), roots)
+    pairs = ctx.trim2()
+    ost, oroot = state.copy(), roots.copy()
+    opairs = oracle.trim2(g.row, g.col, g.rowT, g.colT, colour, ost, oroot)
+    words, groots = ctx.get_state()
+    assert pairs == opairs and pairs > 0
+    assert np.array_equal(scc.split_word(words)[1], ost)
+    assert np.array_equal(groots, oroot)
+    # WCC labels -> fresh colours: members of a class share (maxc + 1 + root)
+    ctx.set_state(scc.make_word(colour, ost), oroot)
+    ctx.update_wcc()
+    words, _ = ctx.get_state()
+    gc, gs = scc.split_word(words)
+    lab = oracle.wcc_labels(g.rowT, g.colT, colour, ost)
+    live = (ost & 4) == 0
+    maxc = int(colour.max())
+    assert np.array_equal(gc[live], ((maxc + 1 + lab[live]) & 0xFFFFFF).astype(np.uint32))
+
+
+def test_canonical_property(ctx):
+    g = random_digraph(50000, 150000, 21)
+    ctx.upload(g)
+    for sched in SCHEDS:
+        lab, _ = ctx.run(sched)
+        assert is_canonical(lab)
