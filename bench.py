@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the FB-SCC hot path on one MI355X.
+
+A "step" is one whole SCC decomposition (scc_run: trim, pivots, forward and
+backward reachability, colour updates, canonical labels) of the workload graph,
+which is already resident in HBM when the timed region starts.  Default
+workload: R-MAT scale 22 (4,194,304 vertices, 33,554,432 arcs drawn, edge
+factor 8, a=0.57 b=0.19 c=0.19, seed 2, duplicates and self-loops removed).
+
+value = MTEPS = arcs after dedup / seconds per decomposition.
+roofline: algorithmic bytes of the reachability level kernels (from device
+work counters in an untimed counting run) over their summed duration, timed
+with HIP events on the library's stream during the timed region.
+cpu_baseline: the oracle's single-threaded Tarjan on the same graph.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SCC MTEPS + ms/graph, learned-pivot vs plain FB on 1 MI355X (bit-exact)"
+HBM_PEAK_GBS = 8000.0
+KERNEL_OF = {"q": "k_reach_sweep", "t": "k_reach_topo", "d": "k_reach_queue", "l": "k_reach_lb",
+             "h": "k_reach_hybrid"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(sched, n, prof):
+    """Bytes the reachability levels must move, per the data layout:
+    a sweep level reads every 4-byte vertex word; an expanded vertex reads its
+    row-offset pair (8) and writes its word (4); a scanned arc reads its column
+    id (4) and gathers the target's word (4); a new visit is one 4-byte atomic,
+    plus a 4-byte queue push and (on queue levels) a 4-byte queue read."""
+    S, X, E, V = prof["sweep_levels"], prof["expanded"], prof["edges"], prof["visited"]
+    if sched in ("q", "t"):
+        return S * 4 * n + X * 12 + E * 8 + V * 4
+    if sched in ("d", "l"):
+        return X * 16 + E * 8 + V * 8
+    return S * 4 * n + X * 12 + E * 8 + V * 8
+
+
+def cpu_tarjan_baseline(g, reps=3):
+    import oracle
+    try:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+    except (AttributeError, OSError):
+        old = None
+    times = []
+    try:
+        for _ in range(reps):
+            t, _cnt = oracle.tarjan_time(g.row, g.col)
+            times.append(t)
+    finally:
+        if old is not None:
+            os.sched_setaffinity(0, old)
+    t = statistics.median(times)
+    return {"value": round(g.m / t / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
+            "sample": f"oracle Tarjan (single thread, pinned) on the whole graph, median of {reps} "
+                      f"runs, {t * 1e3:.1f} ms each; host has {os.cpu_count()} CPUs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--schedule", default="t", choices=list("qtdlh"))
+    ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2])
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-permute", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the single-GPU figure is the one reported", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from gpuscc_code_amd import scc
+
+    t0 = time.time()
+    g = scc.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed,
+                     permute=not args.no_permute)
+    log(f"generated R-MAT s{args.scale}: n={g.n} m={g.m} in {time.time() - t0:.1f}s")
+    ctx = scc.Context(0)
+    ctx.upload(g)
+
+    # untimed counting run: algorithmic work of the reachability levels
+    ctx.profile(1)
+    ctx.run(args.schedule, args.variant, labels=False)
+    work = ctx.profile_read(reset=True)
+    ctx.profile(0)
+    for _ in range(args.warmup):
+        ctx.run(args.schedule, args.variant, labels=False)
+
+    # timed region: K decompositions, level kernels timed with HIP events
+    ctx.profile(2)
+    ctx.profile_read(reset=True)
+    stats = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = ctx.run(args.schedule, args.variant, labels=False)
+        stats.append(st)
+    t_total = time.perf_counter() - t_start
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(0)
+
+    ms_per_step = t_total * 1e3 / args.steps
+    mteps = g.m / (ms_per_step * 1e3)
+
+    # correctness of the measured configuration
+    labels, st_check = ctx.run(args.schedule, args.variant)
+    import oracle
+    ref, tst = oracle.tarjan(g.row, g.col)
+    bit_exact = bool(np.array_equal(labels, ref))
+
+    per_run_bytes = algorithmic_bytes(args.schedule, g.n, work)
+    launches = prof["reach_launches"] / args.steps
+    kernel_ms = prof["reach_kernel_ms"] / args.steps
+    achieved = per_run_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    roofline = {
+        "bound": "hbm", "kernel": KERNEL_OF[args.schedule],
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "bytes_per_launch": int(per_run_bytes / max(launches, 1)),
+        "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 2),
+        "launches_per_step": launches, "kernel_ms_per_step": round(kernel_ms, 3),
+        "work_per_step": {k: work[k] for k in ("sweep_levels", "queue_levels", "expanded", "edges",
+                                               "visited")},
+    }
+    cpu = None if args.no_cpu else cpu_tarjan_baseline(g)
+    last = stats[-1]
+    out = {
+        "metric": METRIC, "value": round(mteps, 2), "unit": "MTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, "
+                               f"a=0.57 b=0.19 c=0.19, seed {args.seed}, dedup, no self-loops, "
+                               f"{'un' if args.no_permute else ''}permuted ids",
+                   "n": g.n, "m": g.m, "schedule": args.schedule,
+                   "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain"},
+        "bit_exact": bit_exact, "num_scc": st_check["num_scc"], "biggest_scc": tst["biggest"],
+        "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],
+        "trim_iters": last["trim_iters"], "host_syncs": last["host_syncs"],
+        "phase_ms": {k: round(statistics.mean(s[k] for s in stats), 3)
+                     for k in ("ms_fwd", "ms_bwd", "ms_trim", "ms_update")},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

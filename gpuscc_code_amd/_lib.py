@@ -42,6 +42,17 @@ class SccStats(ctypes.Structure):
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
+_PROF_INTS = ["reach_launches", "reach_levels", "sweep_levels", "expanded", "edges", "visited",
+              "queue_levels", "trim_launches"]
+_PROF_DBLS = ["reach_kernel_ms", "trim_kernel_ms"]
+
+
+class SccProfile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in _PROF_INTS] + [(k, ctypes.c_double) for k in _PROF_DBLS]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in _PROF_INTS + _PROF_DBLS}
+
 
 class SccHostGraph(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int64),
@@ -78,6 +89,8 @@ SIGNATURES = {
     "scc_phase_trim2": (_st, [_vp, _i64p]),
     "scc_phase_wcc": (_st, [_vp, _i64p, _i64p]),
     "scc_phase_count_elimmed": (_st, [_vp, _i64p]),
+    "scc_profile_set": (_st, [_vp, ctypes.c_int]),
+    "scc_profile_get": (_st, [_vp, ctypes.POINTER(SccProfile), ctypes.c_int]),
     "scc_load_file": (_st, [ctypes.c_char_p, _gp]),
     "scc_csr_from_arcs": (_st, [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, ctypes.c_int, _gp]),
     "scc_gen_rmat": (_st, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

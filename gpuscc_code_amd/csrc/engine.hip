@@ -43,8 +43,10 @@ struct scc_ctx {
     bool uploaded = false, labels_valid = false;
     int guess[3] = {4, 4, 2};     // chain batch guesses: fwd, bwd, trim
     int64_t syncs = 0;
-    int instr = 0;
-    unsigned long long st_expanded = 0, st_edges = 0, st_visited = 0;
+    int instr = 0;                  // profile flag 1: device work counters
+    int prof_flags = 0;
+    scc_profile_data prof{};
+    std::vector<hipEvent_t> evs;    // flag 2: one start/stop pair per level launch
 };
 
 namespace {
@@ -100,17 +102,45 @@ scc_status read_ctl(scc_ctx *c) {
     return SCC_OK;
 }
 
+scc_status ensure_events(scc_ctx *c, size_t count) {
+    while (c->evs.size() < count) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->evs.push_back(e);
+    }
+    return SCC_OK;
+}
+
 // Runs a chain of level kernels to its fixpoint; *levels = kernels that did work.
+// kind 0 = reachability, 1 = trim, 2 = other (for the profile counters).
 template <class F>
-scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int64_t *levels) {
+scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int64_t *levels,
+                     int kind = 2) {
     launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
     int64_t launched = 0;
     int L = std::max(1, std::min(guess, 64));
     const int64_t limit = (int64_t)c->g.n + 4;
+    const bool timed = (c->prof_flags & 2) && kind < 2;
     for (;;) {
-        for (int i = 0; i < L; i++) launch_level((int)(launched + i));
+        if (timed) CHK(ensure_events(c, 2 * (size_t)L));
+        for (int i = 0; i < L; i++) {
+            if (timed) HIPCHK(c, hipEventRecord(c->evs[2 * i], c->stream));
+            launch_level((int)(launched + i));
+            if (timed) HIPCHK(c, hipEventRecord(c->evs[2 * i + 1], c->stream));
+        }
         launched += L;
         CHK(read_ctl(c));
+        if (timed) {
+            double ms = 0;
+            for (int i = 0; i < L; i++) {
+                float t = 0;
+                HIPCHK(c, hipEventElapsedTime(&t, c->evs[2 * i], c->evs[2 * i + 1]));
+                ms += t;
+            }
+            if (kind == 0) c->prof.reach_kernel_ms += ms; else c->prof.trim_kernel_ms += ms;
+        }
+        if (kind == 0) c->prof.reach_launches += L;
+        if (kind == 1) c->prof.trim_launches += L;
         uint32_t ran = c->h_ctl->ran;
         if ((int64_t)ran < launched || c->h_ctl->changed[(launched - 1) & 3] == 0) {
             *levels = ran;
@@ -127,7 +157,7 @@ scc_status do_trim(scc_ctx *c, int64_t *iters) {
     Launch l = launch_for(c);
     return run_chain(
         c, [&](int it) { launch_trim_sweep(l, c->g, c->d_word, c->d_root, c->d_ctl, it); },
-        c->guess[2], 0, iters);
+        c->guess[2], 0, iters, 1);
 }
 
 // seeds: queue schedules read the pivot list (d_piv, length ctl->npiv)
@@ -144,12 +174,12 @@ scc_status do_reach(scc_ctx *c, int dir, int sched, int64_t *depth) {
     case SCC_SCHED_Q:
         s = run_chain(
             c, [&](int it) { launch_reach_sweep(l, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
-            guess, 0, depth);
+            guess, 0, depth, 0);
         break;
     case SCC_SCHED_T:
         s = run_chain(
             c, [&](int it) { launch_reach_topo(lf, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
-            guess, 0, depth);
+            guess, 0, depth, 0);
         break;
     case SCC_SCHED_D:
     case SCC_SCHED_L:
@@ -161,25 +191,32 @@ scc_status do_reach(scc_ctx *c, int dir, int sched, int64_t *depth) {
         if (sched == SCC_SCHED_D)
             s = run_chain(c, [&](int it) {
                     launch_reach_queue(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
-                }, guess, 2, depth);
+                }, guess, 2, depth, 0);
         else if (sched == SCC_SCHED_L)
             s = run_chain(c, [&](int it) {
                     launch_reach_lb(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
-                }, guess, 2, depth);
+                }, guess, 2, depth, 0);
         else
             s = run_chain(c, [&](int it) {
                     launch_reach_hybrid(lf, row, col, c->d_word, c->g.n, inq(it), outq(it), dir,
                                         c->d_ctl, it, thr, instr);
-                }, guess, 2, depth);
+                }, guess, 2, depth, 0);
         break;
     }
     default:
         return fail(c, SCC_E_INPUT, "unknown schedule");
     }
-    if (s == SCC_OK && instr) {
-        c->st_expanded += c->h_ctl->st_expanded;
-        c->st_edges += c->h_ctl->st_edges;
-        c->st_visited += c->h_ctl->st_visited;
+    if (s != SCC_OK) return s;
+    c->prof.reach_levels += *depth;
+    int64_t sw = 0;
+    if (sched == SCC_SCHED_Q || sched == SCC_SCHED_T) sw = *depth;
+    else if (sched == SCC_SCHED_H) sw = c->h_ctl->mode_sweeps;
+    c->prof.sweep_levels += sw;
+    if (sched != SCC_SCHED_Q && sched != SCC_SCHED_T) c->prof.queue_levels += *depth - sw;
+    if (instr) {
+        c->prof.expanded += (int64_t)c->h_ctl->st_expanded;
+        c->prof.edges += (int64_t)c->h_ctl->st_edges;
+        c->prof.visited += (int64_t)c->h_ctl->st_visited;
     }
     return s;
 }
@@ -308,8 +345,6 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
         scc_destroy(c);
         return SCC_E_HIP;
     }
-    const char *ins = std::getenv("GPUSCC_INSTRUMENT");
-    c->instr = (ins && ins[0] == '1') ? 1 : 0;
     *out = c;
     return SCC_OK;
 }
@@ -319,6 +354,7 @@ void scc_destroy(scc_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
+    for (auto e : c->evs) (void)hipEventDestroy(e);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -416,7 +452,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     uint16_t *score_saved = c->d_score;
     if (pivot_rule == SCC_PIVOT_PLAIN) c->d_score = nullptr;
     c->syncs = 0;
-    c->st_expanded = c->st_edges = c->st_visited = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t_start = std::chrono::steady_clock::now();
     scc_status s = SCC_OK;
@@ -504,6 +539,20 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         }
     }
     if (stats) *stats = st;
+    return SCC_OK;
+}
+
+scc_status scc_profile_set(scc_ctx *c, int flags) {
+    if (!c || flags < 0 || flags > 3) return SCC_E_INPUT;
+    c->prof_flags = flags;
+    c->instr = flags & 1;
+    return SCC_OK;
+}
+
+scc_status scc_profile_get(scc_ctx *c, scc_profile_data *out, int reset) {
+    if (!c || !out) return SCC_E_INPUT;
+    *out = c->prof;
+    if (reset) std::memset(&c->prof, 0, sizeof(c->prof));
     return SCC_OK;
 }
 

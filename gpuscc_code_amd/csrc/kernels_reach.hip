@@ -145,16 +145,27 @@ struct Items {
 template <bool PUSH>
 __device__ inline void expand_items(const int32_t *__restrict__ col, uint32_t *word, uint32_t bit,
                                     Items &it, uint32_t total, int32_t *sbuf, uint32_t *scount,
-                                    int32_t *outq, Ctl *ctl, int iter, bool &found,
+                                    uint32_t *sbase, int32_t *outq, Ctl *ctl, int iter, bool &found,
                                     unsigned long long &cv) {
     uint32_t cnt = blockDim.x;
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        uint32_t k = find_item(it.off, cnt, e);
-        int32_t u = col[it.start[k] + (int32_t)(e - it.off[k])];
-        if (try_claim(word, u, it.colour[k], bit)) {
-            found = true;
-            cv++;
-            if (PUSH) stage_push(sbuf, scount, u, outq, ctl, iter);
+    // block-uniform trip count: every pass adds at most blockDim entries to the
+    // staging buffer, which is flushed before it can overflow
+    for (uint32_t base = 0; base < total; base += blockDim.x) {
+        uint32_t e = base + threadIdx.x;
+        if (e < total) {
+            uint32_t k = find_item(it.off, cnt, e);
+            int32_t u = col[it.start[k] + (int32_t)(e - it.off[k])];
+            if (try_claim(word, u, it.colour[k], bit)) {
+                found = true;
+                cv++;
+                if (PUSH) stage_push(sbuf, scount, u, outq, ctl, iter);
+            }
+        }
+        if (PUSH) {
+            __syncthreads();
+            bool full = *scount > SQ_CAP - blockDim.x;  // read by all before anyone pushes again
+            __syncthreads();
+            if (full) stage_flush(sbuf, scount, sbase, outq, ctl, iter);
         }
     }
 }
@@ -193,7 +204,8 @@ __device__ inline void topo_level(const int32_t *__restrict__ row, const int32_t
         it.colour[threadIdx.x] = c;
         __syncthreads();
         if (total > 0)
-            expand_items<PUSH>(col, word, bit, it, total, sbuf, scount, outq, ctl, iter, found, cv);
+            expand_items<PUSH>(col, word, bit, it, total, sbuf, scount, sbase, outq, ctl, iter, found,
+                               cv);
         if (PUSH) stage_flush(sbuf, scount, sbase, outq, ctl, iter);
         else __syncthreads();
     }
@@ -285,7 +297,8 @@ __device__ inline void lb_level(const int32_t *__restrict__ row, const int32_t *
         it.colour[threadIdx.x] = c;
         __syncthreads();
         if (total > 0)
-            expand_items<true>(col, word, bit, it, total, sbuf, scount, outq, ctl, iter, found, cv);
+            expand_items<true>(col, word, bit, it, total, sbuf, scount, sbase, outq, ctl, iter, found,
+                               cv);
         stage_flush(sbuf, scount, sbase, outq, ctl, iter);
     }
 }

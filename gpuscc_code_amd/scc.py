@@ -211,6 +211,15 @@ class Context:
     def count_elimmed(self):
         return self._i64call(lib().scc_phase_count_elimmed)
 
+    # measurement: flags 1 = device work counters, 2 = HIP-event kernel timing
+    def profile(self, flags):
+        self._chk(lib().scc_profile_set(self._h, int(flags)))
+
+    def profile_read(self, reset=True):
+        p = _lib.SccProfile()
+        self._chk(lib().scc_profile_get(self._h, ctypes.byref(p), 1 if reset else 0))
+        return p.as_dict()
+
 
 def computeSCC(g: Graph, opt: alg_opt = None, variant=0, ctx: Context = None):  # noqa: N802
     """computeSCC (scc.cu:150-1022): returns (#SCC, seconds, labels, stats)."""

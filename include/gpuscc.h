@@ -144,6 +144,27 @@ scc_status scc_phase_wcc(scc_ctx *ctx, int64_t *iterations, int64_t *pivots);
 /* countElimmed (scc.cu:42-51). */
 scc_status scc_phase_count_elimmed(scc_ctx *ctx, int64_t *count);
 
+/* ---- measurement ----------------------------------------------------------
+ * Work counters and kernel timing of the reachability levels, for the roofline
+ * report (the reference times phases with gettimeofday + cudaDeviceSynchronize,
+ * timing.h:14).  flags: 1 = count expanded vertices / scanned edges / new
+ * visits with device counters; 2 = time every level kernel with HIP events on
+ * the context's stream. */
+typedef struct {
+    int64_t reach_launches;   /* level kernels launched, including ones that exited at once */
+    int64_t reach_levels;     /* level kernels that did work */
+    int64_t sweep_levels;     /* of those, levels that read every vertex word (q, t, h sweeps) */
+    int64_t expanded;         /* vertices whose rows were scanned (flag 1) */
+    int64_t edges;            /* arcs scanned (flag 1) */
+    int64_t visited;          /* vertices newly reached (flag 1) */
+    int64_t queue_levels;     /* levels that expanded a frontier queue (d, l, h) */
+    int64_t trim_launches;
+    double reach_kernel_ms;   /* sum of level-kernel durations (flag 2) */
+    double trim_kernel_ms;    /* sum of trim-sweep durations (flag 2) */
+} scc_profile_data;
+scc_status scc_profile_set(scc_ctx *ctx, int flags);
+scc_status scc_profile_get(scc_ctx *ctx, scc_profile_data *out, int reset);
+
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
  * "a b" is the arc b->a; self-loops kept, duplicates dropped. */

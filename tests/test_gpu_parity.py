@@ -234,9 +234,23 @@ def test_phase_update_elect():
 
 def test_phase_trim2_and_wcc(ctx):
     rng = np.random.default_rng(9)
-    g = random_digraph(20000, 22000, 14)
+    # sparse random arcs among 20000 vertices plus 2-cycles on 4000 further vertices, some
+    # of them with one extra arc in or out (still a pair by one of the two tests)
+    n0, n1 = 20000, 4000
+    base = random_digraph(n0, 22000, 14)
+    src = list(np.repeat(np.arange(n0), np.diff(base.row)))
+    dst = list(base.col)
+    for k in range(0, n1, 2):
+        a, b = n0 + k, n0 + k + 1
+        src += [a, b]
+        dst += [b, a]
+        if k % 6 == 2:
+            src.append(b)
+            dst.append(int(rng.integers(0, n0)))
+    g = scc.csr_from_arcs(n0 + n1, np.array(src), np.array(dst))
     ctx.upload(g)
     colour, state = random_state(g.n, rng, ncol=2, p_elim=0.1)
+    colour[n0:] = 1
     roots = np.arange(g.n, dtype=np.int32)
     ctx.set_state(scc.make_word(colour, state), roots)
     pairs = ctx.trim2()
