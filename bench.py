@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-permute", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sequential", action="store_true",
+                    help="forward reach to its fixpoint, then backward (the reference's order)")
+    ap.add_argument("--pull-threshold", type=int, default=0,
+                    help="hybrid: frontier size at which a direction pulls (0 = n/20)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -96,7 +100,8 @@ def main():
     g = scc.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed,
                      permute=not args.no_permute)
     log(f"generated R-MAT s{args.scale}: n={g.n} m={g.m} in {time.time() - t0:.1f}s")
-    ctx = scc.Context(0)
+    ctx = scc.Context(0, scc.alg_opt(sequential_reach=1 if args.sequential else 0,
+                                     hybrid_threshold=args.pull_threshold))
     ctx.upload(g)
 
     # untimed counting run: algorithmic work of the reachability levels
@@ -153,7 +158,8 @@ def main():
                                f"a=0.57 b=0.19 c=0.19, seed {args.seed}, dedup, no self-loops, "
                                f"{'un' if args.no_permute else ''}permuted ids",
                    "n": g.n, "m": g.m, "schedule": args.schedule,
-                   "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain"},
+                   "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
+                   "reach": "sequential" if args.sequential else "fused"},
         "bit_exact": bit_exact, "num_scc": st_check["num_scc"], "biggest_scc": tst["biggest"],
         "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],
         "trim_iters": last["trim_iters"], "host_syncs": last["host_syncs"],

@@ -25,7 +25,8 @@ class SccError(RuntimeError):
 class SccOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("hybrid_policy", ctypes.c_int32),
                 ("hybrid_threshold", ctypes.c_int32), ("slot_bits", ctypes.c_int32),
-                ("grid_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("grid_blocks", ctypes.c_int32), ("sequential_reach", ctypes.c_int32),
+                ("huge_row", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",

@@ -31,8 +31,10 @@ struct scc_ctx {
     int32_t *d_row = nullptr, *d_col = nullptr, *d_rowT = nullptr, *d_colT = nullptr;
     uint32_t *d_word = nullptr;
     int32_t *d_root = nullptr, *d_label = nullptr, *d_tmp = nullptr;
-    int32_t *d_q[2] = {nullptr, nullptr};
-    int32_t *d_piv = nullptr;
+    int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
+    int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
+    int32_t *d_seedB = nullptr;  // backward seeds of a phase-level reach
+    int32_t *d_huge[4] = {nullptr, nullptr, nullptr, nullptr};  // deferred huge rows [dir * 2 + parity]
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
     int prev_tab = 0;        // table holding the last election's winners
@@ -112,6 +114,8 @@ scc_status ensure_events(scc_ctx *c, size_t count) {
 }
 
 // Runs a chain of level kernels to its fixpoint; *levels = kernels that did work.
+// launch_level(iter, ev_start, ev_stop) launches level `iter`; with kernel
+// timing on (profile flag 2) the events bracket the kernel itself.
 // kind 0 = reachability, 1 = trim, 2 = other (for the profile counters).
 template <class F>
 scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int64_t *levels,
@@ -124,9 +128,8 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
     for (;;) {
         if (timed) CHK(ensure_events(c, 2 * (size_t)L));
         for (int i = 0; i < L; i++) {
-            if (timed) HIPCHK(c, hipEventRecord(c->evs[2 * i], c->stream));
-            launch_level((int)(launched + i));
-            if (timed) HIPCHK(c, hipEventRecord(c->evs[2 * i + 1], c->stream));
+            if (timed) launch_level((int)(launched + i), c->evs[2 * i], c->evs[2 * i + 1]);
+            else launch_level((int)(launched + i), (hipEvent_t) nullptr, (hipEvent_t) nullptr);
         }
         launched += L;
         CHK(read_ctl(c));
@@ -156,69 +159,67 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
 scc_status do_trim(scc_ctx *c, int64_t *iters) {
     Launch l = launch_for(c);
     return run_chain(
-        c, [&](int it) { launch_trim_sweep(l, c->g, c->d_word, c->d_root, c->d_ctl, it); },
+        c,
+        [&](int it, hipEvent_t e0, hipEvent_t e1) {
+            Launch lt = l;
+            lt.ev_start = e0;
+            lt.ev_stop = e1;
+            launch_trim_sweep(lt, c->g, c->d_word, c->d_root, c->d_ctl, it);
+        },
         c->guess[2], 0, iters, 1);
 }
 
-// seeds: queue schedules read the pivot list (d_piv, length ctl->npiv)
-scc_status do_reach(scc_ctx *c, int dir, int sched, int64_t *depth) {
-    const int32_t *row = dir == 0 ? c->g.row : c->g.rowT;
-    const int32_t *col = dir == 0 ? c->g.col : c->g.colT;
-    Launch l = launch_for(c);
-    Launch lf = fixed_launch(c);
-    int &guess = c->guess[dir];
-    int instr = c->instr;
-    if (instr) launch_ctl_reset(c->stream, c->d_ctl, 32);
-    scc_status s;
-    switch (sched) {
-    case SCC_SCHED_Q:
-        s = run_chain(
-            c, [&](int it) { launch_reach_sweep(l, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
-            guess, 0, depth, 0);
-        break;
-    case SCC_SCHED_T:
-        s = run_chain(
-            c, [&](int it) { launch_reach_topo(lf, row, col, c->d_word, c->g.n, dir, c->d_ctl, it, instr); },
-            guess, 0, depth, 0);
-        break;
-    case SCC_SCHED_D:
-    case SCC_SCHED_L:
-    case SCC_SCHED_H: {
-        uint32_t thr = c->opts.hybrid_threshold > 0 ? (uint32_t)c->opts.hybrid_threshold
-                                                    : (uint32_t)std::max<int64_t>(1024, c->g.n / 32);
-        auto inq = [&](int it) -> const int32_t * { return it == 0 ? c->d_piv : c->d_q[(it - 1) & 1]; };
-        auto outq = [&](int it) -> int32_t * { return c->d_q[it & 1]; };
-        if (sched == SCC_SCHED_D)
-            s = run_chain(c, [&](int it) {
-                    launch_reach_queue(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
-                }, guess, 2, depth, 0);
-        else if (sched == SCC_SCHED_L)
-            s = run_chain(c, [&](int it) {
-                    launch_reach_lb(lf, row, col, c->d_word, inq(it), outq(it), dir, c->d_ctl, it, instr);
-                }, guess, 2, depth, 0);
-        else
-            s = run_chain(c, [&](int it) {
-                    launch_reach_hybrid(lf, row, col, c->d_word, c->g.n, inq(it), outq(it), dir,
-                                        c->d_ctl, it, thr, instr);
-                }, guess, 2, depth, 0);
-        break;
-    }
-    default:
-        return fail(c, SCC_E_INPUT, "unknown schedule");
-    }
+// One reachability chain for the directions in `dirs` (1 = F, 2 = B, 3 = both).
+// Queue schedules start from the seed lists: d_piv (forward, ctl->seeds[0])
+// and bseed (backward, ctl->seeds[1]).
+scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, int64_t *depth) {
+    Launch l = sched == SCC_SCHED_Q ? launch_for(c) : fixed_launch(c);
+    int &guess = c->guess[dirs == 2 ? 1 : 0];
+    if (c->instr) launch_ctl_reset(c->stream, c->d_ctl, 32);
+    ReachArgs a{};
+    a.row = c->g.row;
+    a.col = c->g.col;
+    a.rowT = c->g.rowT;
+    a.colT = c->g.colT;
+    a.word = c->d_word;
+    a.n = c->g.n;
+    a.ctl = c->d_ctl;
+    a.dirs = dirs;
+    a.instr = c->instr;
+    a.pull_thr = c->opts.hybrid_threshold > 0 ? (uint32_t)c->opts.hybrid_threshold
+                                              : (uint32_t)std::max<int64_t>(1024, c->g.n / 20);
+    a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
+                                      : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
+    const bool queues = sched >= SCC_SCHED_D;
+    scc_status s = run_chain(
+        c,
+        [&](int it, hipEvent_t e0, hipEvent_t e1) {
+            a.iter = it;
+            for (int d = 0; d < 2; d++) {
+                a.inq[d] = it == 0 ? (d == 0 ? c->d_piv : bseed) : c->d_q[d * 2 + ((it - 1) & 1)];
+                a.outq[d] = c->d_q[d * 2 + (it & 1)];
+                a.hin[d] = c->d_huge[d * 2 + ((it + 1) & 1)];
+                a.hout[d] = c->d_huge[d * 2 + (it & 1)];
+            }
+            Launch lt = l;
+            lt.ev_start = e0;
+            lt.ev_stop = e1;
+            launch_reach(lt, sched, a);
+        },
+        guess, queues ? 2 : 0, depth, 0);
     if (s != SCC_OK) return s;
     c->prof.reach_levels += *depth;
     int64_t sw = 0;
     if (sched == SCC_SCHED_Q || sched == SCC_SCHED_T) sw = *depth;
-    else if (sched == SCC_SCHED_H) sw = c->h_ctl->mode_sweeps;
+    else if (sched == SCC_SCHED_H) sw = c->h_ctl->pull_levels;
     c->prof.sweep_levels += sw;
-    if (sched != SCC_SCHED_Q && sched != SCC_SCHED_T) c->prof.queue_levels += *depth - sw;
-    if (instr) {
+    if (queues) c->prof.queue_levels += *depth;
+    if (c->instr) {
         c->prof.expanded += (int64_t)c->h_ctl->st_expanded;
         c->prof.edges += (int64_t)c->h_ctl->st_edges;
         c->prof.visited += (int64_t)c->h_ctl->st_visited;
     }
-    return s;
+    return SCC_OK;
 }
 
 // one election into the table that is not `prev`; winners -> state 19 + pivot list
@@ -232,6 +233,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv) {
     launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                   recolour, c->d_score);
     launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
+    launch_ctl_reset(c->stream, c->d_ctl, 64);  // the pivot list seeds both directions
     CHK(read_ctl(c));
     c->prev_tab ^= 1;
     *npiv = c->h_ctl->npiv;
@@ -261,7 +263,7 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv) {
     launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
     launch_wcc_init(l, c->d_tmp, c->g.n);
     int wg = 4;
-    CHK(run_chain(c, [&](int it) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
+    CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
                   wg, 0, iters));
     launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
     return do_elect(c, 0, npiv);
@@ -273,7 +275,9 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 void free_graph(scc_ctx *c) {
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
-                        &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_piv};
+                        &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
+                        &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
+                        &c->d_huge[3]};
     for (auto b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -407,8 +411,9 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_root, vb));
     HIPCHK(c, hipMalloc(&c->d_label, vb));
     HIPCHK(c, hipMalloc(&c->d_tmp, vb));
-    HIPCHK(c, hipMalloc(&c->d_q[0], vb));
-    HIPCHK(c, hipMalloc(&c->d_q[1], vb));
+    for (auto &q : c->d_q) HIPCHK(c, hipMalloc(&q, vb));
+    for (auto &q : c->d_huge) HIPCHK(c, hipMalloc(&q, vb));
+    HIPCHK(c, hipMalloc(&c->d_seedB, vb));
     HIPCHK(c, hipMalloc(&c->d_piv, vb));
     // election table: about n slots (at least 2^10, at most 2^22), or the option
     int sb = 10;
@@ -482,13 +487,20 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 sched = ne < one_pct ? SCC_SCHED_T : SCC_SCHED_Q;  // scc.cu:493-563
             int64_t d = 0;
             tp = std::chrono::steady_clock::now();
-            CHK(do_reach(c, 0, sched, &d));
-            st.fwd_depths += d;
-            st.ms_fwd += ms_since(tp);
-            tp = std::chrono::steady_clock::now();
-            CHK(do_reach(c, 1, sched, &d));
-            st.bwd_depths += d;
-            st.ms_bwd += ms_since(tp);
+            if (c->opts.sequential_reach) {  // scc.cu:285-704 order: forward, then backward
+                CHK(do_reach(c, 1, sched, c->d_piv, &d));
+                st.fwd_depths += d;
+                st.ms_fwd += ms_since(tp);
+                tp = std::chrono::steady_clock::now();
+                CHK(do_reach(c, 2, sched, c->d_piv, &d));
+                st.bwd_depths += d;
+                st.ms_bwd += ms_since(tp);
+            } else {  // both directions in the same level kernels
+                CHK(do_reach(c, 3, sched, c->d_piv, &d));
+                st.fwd_depths += d;
+                st.bwd_depths += d;
+                st.ms_fwd += ms_since(tp);
+            }
             tp = std::chrono::steady_clock::now();
             CHK(do_trim(c, &it));
             st.trim_iters += it;
@@ -604,15 +616,17 @@ scc_status scc_phase_trim(scc_ctx *c, int64_t *iterations) {
 scc_status scc_phase_reach(scc_ctx *c, int dir, int schedule, int64_t *depth) {
     if (!c) return SCC_E_INPUT;
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
-    if (dir != 0 && dir != 1) return fail(c, SCC_E_INPUT, "dir must be 0 or 1");
+    if (dir < 0 || dir > 2) return fail(c, SCC_E_INPUT, "dir must be 0, 1 or 2");
     if (schedule < SCC_SCHED_Q || schedule > SCC_SCHED_H) return fail(c, SCC_E_INPUT, "bad schedule");
     int64_t d = 0;
+    uint32_t dirs = dir == 0 ? 1u : dir == 1 ? 2u : 3u;
     if (c->g.n > 0) {
         if (schedule >= SCC_SCHED_D) {  // seeds: live vertices holding the bit
-            launch_ctl_reset(c->stream, c->d_ctl, 4);
-            launch_collect(launch_for(c), c->d_word, c->g.n, dir == 0 ? ST_F : ST_B, c->d_piv, c->d_ctl);
+            launch_ctl_reset(c->stream, c->d_ctl, 128);
+            if (dirs & 1u) launch_collect(launch_for(c), c->d_word, c->g.n, ST_F, c->d_piv, c->d_ctl);
+            if (dirs & 2u) launch_collect(launch_for(c), c->d_word, c->g.n, ST_B, c->d_seedB, c->d_ctl);
         }
-        CHK(do_reach(c, dir, schedule, &d));
+        CHK(do_reach(c, dirs, schedule, c->d_seedB, &d));
     }
     if (depth) *depth = d;
     return SCC_OK;

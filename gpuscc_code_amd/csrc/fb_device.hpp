@@ -8,6 +8,7 @@
 // instead of the reference's two (states[] u8 + ranges[] u32).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 namespace gpuscc {
@@ -35,11 +36,13 @@ __host__ __device__ inline uint64_t elect_tag(uint32_t epoch) {
 // Device control block: level-chain flags, queue sizes and reduction scratch.
 struct Ctl {
     uint32_t changed[4];   // chain ring: kernel i sets [i&3] on progress, clears [(i+1)&3]
-    uint32_t qsize[4];     // queue chain ring: input size of level i is [i&3]
+    uint32_t qsize[2][4];  // queue chain rings (forward, backward): input of level i is [d][i&3]
+    uint32_t hsize[2][4];  // rings of the huge-row lists, same indexing
+    uint32_t seeds[2];     // seed-list lengths of the next reach (forward, backward)
     uint32_t ran;          // chain kernels that did work (depth / iterations)
     uint32_t npiv;         // pivots elected by the last election = pivot list length
     uint32_t maxc;         // max colour (reduction scratch)
-    uint32_t mode_sweeps;  // hybrid: levels that ran as sweeps
+    uint32_t pull_levels;  // hybrid: direction-levels that ran bottom-up (pull)
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
 };

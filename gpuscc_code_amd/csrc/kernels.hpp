@@ -12,6 +12,27 @@ struct Launch {
     hipStream_t stream;
     uint32_t grid;   // workgroups (persistent grid-stride)
     uint32_t block;  // work-items per workgroup, multiple of 64
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // set: time the kernel itself
+};
+
+// Arguments of one reachability level (both directions): forward over
+// (row, col) with bit F, backward over (rowT, colT) with bit B.  dirs bit 0 /
+// bit 1 enables the forward / backward direction.  Queue schedules read
+// inq[d] (length ctl->qsize[d][iter & 3]) and append to outq[d].
+struct ReachArgs {
+    const int32_t *row, *col, *rowT, *colT;
+    uint32_t *word;
+    int32_t n;
+    const int32_t *inq[2];
+    int32_t *outq[2];
+    const int32_t *hin[2];  // huge rows deferred by the previous level (ctl->hsize[d][iter & 3])
+    int32_t *hout[2];       // huge rows this level defers to the next
+    Ctl *ctl;
+    int iter;
+    uint32_t dirs;
+    uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
+    uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
+    int instr;          // count work (expanded vertices, scanned arcs, visits)
 };
 
 struct Graph {  // device CSR + CSR^T
@@ -45,22 +66,9 @@ void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *l
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl);
 
-// kernels_reach.hip -- one level of a reachability chain.  dir 0: forward over
-// (row, col) with bit F / FExt; dir 1: backward over (rowT, colT) with B / BExt.
-// Queue levels read `inq` (length ctl->qsize[iter&3]) and append to `outq`.
-void launch_reach_sweep(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                        int32_t n, int dir, Ctl *ctl, int iter, int instr);
-void launch_reach_topo(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                       int32_t n, int dir, Ctl *ctl, int iter, int instr);
-void launch_reach_queue(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                        const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter, int instr);
-void launch_reach_lb(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                     const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter, int instr);
-void launch_reach_hybrid(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                         int32_t n, const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter,
-                         uint32_t threshold, int instr);
-// clears the expanded bit (FExt or BExt) of every vertex: queue levels of the
-// hybrid mark expanded vertices so a later sweep level skips them
+// kernels_reach.hip -- one level of a reachability chain; sched 0..4 = q t d l h
+void launch_reach(const Launch &l, int sched, const ReachArgs &a);
+// clears bits of every vertex word
 void launch_clear_bits(const Launch &l, uint32_t *word, int32_t n, uint32_t bits);
 
 }  // namespace gpuscc

@@ -39,13 +39,25 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
     if (threadIdx.x != 0) return;
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
+        for (int d = 0; d < 2; d++)
+            for (int i = 0; i < 4; i++) ctl->hsize[d][i] = 0;
         ctl->ran = 0;
-        ctl->mode_sweeps = 0;
+        ctl->pull_levels = 0;
     }
     if (what & 2) {
-        // queue chain: level 0 reads the pivot list (length npiv)
-        ctl->qsize[0] = ctl->npiv;
-        for (int i = 1; i < 4; i++) ctl->qsize[i] = 0;
+        // queue chains: level 0 reads the seed lists
+        for (int d = 0; d < 2; d++) {
+            ctl->qsize[d][0] = ctl->seeds[d];
+            for (int i = 1; i < 4; i++) ctl->qsize[d][i] = 0;
+        }
+    }
+    if (what & 64) {  // the pivot list seeds both directions
+        ctl->seeds[0] = ctl->npiv;
+        ctl->seeds[1] = ctl->npiv;
+    }
+    if (what & 128) {
+        ctl->seeds[0] = 0;
+        ctl->seeds[1] = 0;
     }
     if (what & 4) ctl->npiv = 0;
     if (what & 8) ctl->count = 0;
@@ -377,6 +389,7 @@ __global__ void k_label_final(const int32_t *root, const int32_t *mn, int32_t *l
 }
 
 // seeds of a reach given as vertex words: live vertices holding `bit`
+// (seed list length accumulates in ctl->seeds[bit == ST_F ? 0 : 1])
 __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t *q, Ctl *ctl) {
     __shared__ int32_t sq[1024];
     __shared__ uint32_t scount, sbase;
@@ -392,7 +405,7 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
         }
         __syncthreads();
         if (scount > 0) {
-            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, scount);
+            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->seeds[bit == ST_F ? 0 : 1], scount);
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < scount; i += blockDim.x) q[sbase + i] = sq[i];
         }
@@ -419,8 +432,12 @@ void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
 }
 void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl,
                        int iter) {
-    hipLaunchKernelGGL(k_trim_sweep, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl,
-                       iter);
+    if (l.ev_start)
+        hipExtLaunchKernelGGL(k_trim_sweep, dim3(l.grid), dim3(l.block), 0, l.stream, l.ev_start,
+                              l.ev_stop, 0, g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl, iter);
+    else
+        hipLaunchKernelGGL(k_trim_sweep, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl,
+                           iter);
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,

@@ -1,23 +1,36 @@
-// Forward / backward reachability inside colours for gfx950: one kernel per
-// BFS level, chained on the device (a level whose predecessor found nothing
-// exits at once), in the four schedules of the reference plus its default:
+// Forward / backward reachability inside colours for gfx950.
+//
+// One kernel per BFS level, chained on the device (a level whose predecessor
+// found nothing exits at once).  The reference runs the forward reach to its
+// fixpoint and then the backward one (scc.cu:285-704); here one level kernel
+// advances both: the forward frontier over the CSR (bit F) and the backward
+// frontier over CSR^T (bit B) share each launch, so a step costs
+// max(depth_F, depth_B) launches instead of their sum.  Schedules:
 //   q  sweep, one lane per vertex                 (reach_q.cu:12-79)
 //   t  sweep, workgroup-balanced edge expansion   (reach_t.cu:17-270)
-//   d  frontier queue, one lane per item          (reach_d.cu:11-64)
-//   l  frontier queue, workgroup-balanced edges   (reach_l.cu:53-253)
-//   h  per level: queue expansion while the frontier is small, balanced sweep
-//      once it is large (scc.cu:493-563 switches by eliminated count instead)
+//   d  frontier queues, one lane per item         (reach_d.cu:11-64)
+//   l  frontier queues, workgroup-balanced edges  (reach_l.cu:53-253)
+//   h  per level and direction: balanced queue expansion (push) while the
+//      frontier is small, bottom-up (pull) once it is large: every live
+//      unreached vertex looks for one reached in-neighbour and stops at the
+//      first (scc.cu:493-563 switches by eliminated count instead)
 //
 // Every visit is claimed with one atomicOr on the target's word, whose old
 // value says who set the bit first, so a vertex enters a queue exactly once:
 // queues of n entries cannot overflow (the reference's can, worklistc.h:80).
-// Pushes go through an LDS staging buffer, one global atomicAdd per flush.
+// Pushes go through per-direction LDS staging buffers, one global atomicAdd
+// per flush.
 #include "fb_device.hpp"
 #include "kernels.hpp"
 
 namespace gpuscc {
 
-constexpr uint32_t SQ_CAP = 2048;  // LDS staging entries per workgroup (8 KiB)
+constexpr uint32_t SQ_CAP = 1024;  // LDS staging entries per direction per workgroup
+
+__device__ inline uint32_t dbit(int d) { return d == 0 ? ST_F : ST_B; }
+__device__ inline uint32_t dxbit(int d) { return d == 0 ? ST_FX : ST_BX; }
+__device__ inline const int32_t *drow(const ReachArgs &a, int d) { return d == 0 ? a.row : a.rowT; }
+__device__ inline const int32_t *dcol(const ReachArgs &a, int d) { return d == 0 ? a.col : a.colT; }
 
 __device__ inline bool try_claim(uint32_t *word, int32_t u, uint32_t c, uint32_t bit) {
     uint32_t wu = word[u];
@@ -26,32 +39,50 @@ __device__ inline bool try_claim(uint32_t *word, int32_t u, uint32_t c, uint32_t
     return !(old & bit);
 }
 
-__device__ inline void stage_push(int32_t *sbuf, uint32_t *scount, int32_t v, int32_t *outq,
-                                  Ctl *ctl, int iter) {
-    uint32_t i = atomicAdd(scount, 1u);
+// ---------------------------------------------------------------------------
+// LDS staging of the two output queues
+// ---------------------------------------------------------------------------
+struct Stage {
+    int32_t buf[2][SQ_CAP];
+    uint32_t cnt[2];
+    uint32_t base[2];
+};
+
+__device__ inline void st_init(Stage &s) {
+    if (threadIdx.x < 2) s.cnt[threadIdx.x] = 0;
+}
+
+__device__ inline void st_push(Stage &s, int d, int32_t v, const ReachArgs &a) {
+    uint32_t i = atomicAdd(&s.cnt[d], 1u);
     if (i < SQ_CAP) {
-        sbuf[i] = v;
+        s.buf[d][i] = v;
     } else {
-        uint32_t g = atomicAdd(&ctl->qsize[(iter + 1) & 3], 1u);
-        outq[g] = v;
+        uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
+        a.outq[d][g] = v;
     }
 }
 
-// all threads of the workgroup call this
-__device__ inline void stage_flush(int32_t *sbuf, uint32_t *scount, uint32_t *sbase, int32_t *outq,
-                                   Ctl *ctl, int iter) {
+// all work-items of the workgroup call this
+__device__ inline void st_flush(Stage &s, const ReachArgs &a) {
     __syncthreads();
-    uint32_t c = *scount;
-    if (c > SQ_CAP) c = SQ_CAP;
-    if (c > 0) {
-        if (threadIdx.x == 0) *sbase = atomicAdd(&ctl->qsize[(iter + 1) & 3], c);
-        __syncthreads();
-        uint32_t b = *sbase;
-        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) outq[b + i] = sbuf[i];
-    }
+    uint32_t c0 = s.cnt[0] < SQ_CAP ? s.cnt[0] : SQ_CAP;
+    uint32_t c1 = s.cnt[1] < SQ_CAP ? s.cnt[1] : SQ_CAP;
+    if (threadIdx.x == 0 && c0) s.base[0] = atomicAdd(&a.ctl->qsize[0][(a.iter + 1) & 3], c0);
+    if (threadIdx.x == 1 && c1) s.base[1] = atomicAdd(&a.ctl->qsize[1][(a.iter + 1) & 3], c1);
     __syncthreads();
-    if (threadIdx.x == 0) *scount = 0;
+    for (uint32_t i = threadIdx.x; i < c0; i += blockDim.x) a.outq[0][s.base[0] + i] = s.buf[0][i];
+    for (uint32_t i = threadIdx.x; i < c1; i += blockDim.x) a.outq[1][s.base[1] + i] = s.buf[1][i];
     __syncthreads();
+    if (threadIdx.x < 2) s.cnt[threadIdx.x] = 0;
+    __syncthreads();
+}
+
+// all work-items call this; flushes when another pass could overflow a buffer
+__device__ inline void st_maybe_flush(Stage &s, const ReachArgs &a) {
+    __syncthreads();
+    bool full = s.cnt[0] > SQ_CAP - blockDim.x || s.cnt[1] > SQ_CAP - blockDim.x;
+    __syncthreads();  // every work-item has read the counts before anyone pushes again
+    if (full) st_flush(s, a);
 }
 
 __device__ inline void instr_flush(Ctl *ctl, unsigned long long x, unsigned long long e,
@@ -97,102 +128,289 @@ __device__ inline uint32_t find_item(const uint32_t *off, uint32_t cnt, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// q: one lane per vertex; a live vertex with the bit and without the expanded
-// bit sets the expanded bit and scans its row (reach_q.cu:44-79)
-// ---------------------------------------------------------------------------
-__global__ void k_reach_sweep(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                              uint32_t *word, int32_t n, uint32_t bit, uint32_t xbit, Ctl *ctl,
-                              int iter, int instr) {
-    __shared__ unsigned long long sh[16];
-    if (chain_exit(ctl, iter)) return;
-    int64_t beg, end;
-    my_range(n, beg, end);
-    bool found = false;
-    unsigned long long cx = 0, ce = 0, cv = 0;
-    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
-        int32_t v = (int32_t)vv;
-        uint32_t w = word[v];
-        if ((w & (bit | xbit | ST_E)) != bit) continue;
-        word[v] = w | xbit;
-        uint32_t c = colour_of(w);
-        int32_t e0 = row[v], e1 = row[v + 1];
-        cx++;
-        ce += (unsigned long long)(e1 - e0);
-        for (int32_t e = e0; e < e1; e++) {
-            if (try_claim(word, col[e], c, bit)) {
-                found = true;
-                cv++;
-            }
-        }
-    }
-    chain_progress(ctl, iter, found);
-    if (instr) instr_flush(ctl, cx, ce, cv, sh);
-}
-
-// ---------------------------------------------------------------------------
-// balanced expansion of up to blockDim items held in LDS: item k owns edges
-// [start[k], start[k]+deg[k]); work-item t takes edges t, t+blockDim, ... of
-// the concatenation (the CTA gather of reach_t.cu:195-265, without its
-// 256-edge windows)
+// balanced expansion of the items held in LDS: item k owns arcs
+// [start[k], start[k] + deg[k]) of direction cd[k] >> 31; work-item t takes
+// arcs t, t + blockDim, ... of their concatenation (the CTA gather of
+// reach_t.cu:195-265 and reach_l.cu:190-245, without their 256-arc windows)
 // ---------------------------------------------------------------------------
 struct Items {
     uint32_t off[1024];
     int32_t start[1024];
-    uint32_t colour[1024];
+    uint32_t cd[1024];  // colour | direction << 31
+    int32_t lim[1024];  // end of the row (huge rows)
     uint32_t shw[16];
 };
 
 template <bool PUSH>
-__device__ inline void expand_items(const int32_t *__restrict__ col, uint32_t *word, uint32_t bit,
-                                    Items &it, uint32_t total, int32_t *sbuf, uint32_t *scount,
-                                    uint32_t *sbase, int32_t *outq, Ctl *ctl, int iter, bool &found,
-                                    unsigned long long &cv) {
-    uint32_t cnt = blockDim.x;
-    // block-uniform trip count: every pass adds at most blockDim entries to the
-    // staging buffer, which is flushed before it can overflow
-    for (uint32_t base = 0; base < total; base += blockDim.x) {
+__device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nitems, uint32_t total,
+                                    Stage *st, bool &found, unsigned long long &cv) {
+    for (uint32_t base = 0; base < total; base += blockDim.x) {  // block-uniform trip count
         uint32_t e = base + threadIdx.x;
         if (e < total) {
-            uint32_t k = find_item(it.off, cnt, e);
-            int32_t u = col[it.start[k] + (int32_t)(e - it.off[k])];
-            if (try_claim(word, u, it.colour[k], bit)) {
+            uint32_t k = find_item(it.off, nitems, e);
+            uint32_t cd = it.cd[k];
+            int d = (int)(cd >> 31);
+            int32_t u = dcol(a, d)[it.start[k] + (int32_t)(e - it.off[k])];
+            if (try_claim(a.word, u, cd & 0x7FFFFFFFu, dbit(d))) {
                 found = true;
                 cv++;
-                if (PUSH) stage_push(sbuf, scount, u, outq, ctl, iter);
+                if (PUSH) st_push(*st, d, u, a);
             }
         }
-        if (PUSH) {
+        if (PUSH) st_maybe_flush(*st, a);
+    }
+}
+
+// Gate of a reach level: exits when the previous level made no progress;
+// block 0 clears the output counters of the level after next.
+__device__ inline bool level_begin(const ReachArgs &a, bool queues) {
+    if (chain_exit(a.ctl, a.iter)) return false;
+    if (blockIdx.x == 0 && threadIdx.x < 2) {
+        a.ctl->hsize[threadIdx.x][(a.iter + 2) & 3] = 0;
+        if (queues) a.ctl->qsize[threadIdx.x][(a.iter + 2) & 3] = 0;
+    }
+    return true;
+}
+
+// a row of at least huge_thr arcs is not expanded by its own lane or
+// workgroup: it is deferred to the next level, where all workgroups split it
+__device__ inline void push_huge(const ReachArgs &a, int d, int32_t v) {
+    uint32_t g = atomicAdd(&a.ctl->hsize[d][(a.iter + 1) & 3], 1u);
+    a.hout[d][g] = v;
+}
+
+constexpr uint32_t HCHUNK = 1024;  // arcs per chunk of a huge row
+
+// Expands the huge rows deferred by the previous level: their arcs are cut
+// into chunks of HCHUNK, numbered across the list, and workgroup b takes the
+// chunks k with k mod gridDim == b.  All work-items call this.
+template <bool PUSH>
+__device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool &found,
+                                  unsigned long long &cx, unsigned long long &ce,
+                                  unsigned long long &cv) {
+    const uint32_t G = gridDim.x;
+    for (int d = 0; d < 2; d++) {
+        if (!((a.dirs >> d) & 1u)) continue;
+        uint32_t hs = a.ctl->hsize[d][a.iter & 3];
+        if (hs == 0) continue;
+        const int32_t *row = drow(a, d), *col = dcol(a, d);
+        uint32_t bit = dbit(d);
+        uint32_t acc = 0;
+        for (uint32_t base = 0; base < hs; base += blockDim.x) {
+            uint32_t i = base + threadIdx.x;
+            uint32_t nch = 0, c = 0;
+            int32_t e0 = 0, e1 = 0;
+            if (i < hs) {
+                int32_t v = a.hin[d][i];
+                e0 = row[v];
+                e1 = row[v + 1];
+                nch = ((uint32_t)(e1 - e0) + HCHUNK - 1) / HCHUNK;
+                c = colour_of(a.word[v]);
+                if (blockIdx.x == 0) {
+                    cx++;
+                    ce += (unsigned long long)(e1 - e0);
+                }
+            }
+            uint32_t total;
+            uint32_t off = block_excl_scan(nch, it.shw, total);
+            it.off[threadIdx.x] = off;
+            it.start[threadIdx.x] = e0;
+            it.lim[threadIdx.x] = e1;
+            it.cd[threadIdx.x] = c;
             __syncthreads();
-            bool full = *scount > SQ_CAP - blockDim.x;  // read by all before anyone pushes again
+            for (uint32_t k = acc + (blockIdx.x + G - acc % G) % G; k < acc + total; k += G) {
+                uint32_t local = k - acc;
+                uint32_t j = find_item(it.off, blockDim.x, local);
+                int32_t cs = it.start[j] + (int32_t)((local - it.off[j]) * HCHUNK);
+                int32_t cend = cs + (int32_t)HCHUNK < it.lim[j] ? cs + (int32_t)HCHUNK : it.lim[j];
+                uint32_t cc = it.cd[j];
+                for (int32_t pb = cs; pb < cend; pb += (int32_t)blockDim.x) {  // block-uniform
+                    int32_t e = pb + (int32_t)threadIdx.x;
+                    if (e < cend && try_claim(a.word, col[e], cc, bit)) {
+                        found = true;
+                        cv++;
+                        if (PUSH) st_push(*st, d, col[e], a);
+                    }
+                    if (PUSH) st_maybe_flush(*st, a);
+                }
+            }
+            acc += total;
             __syncthreads();
-            if (full) stage_flush(sbuf, scount, sbase, outq, ctl, iter);
         }
     }
 }
 
-// t: balanced sweep.  PUSH=true (hybrid sweep levels) also queues every new
-// visit so a later queue level can continue from it.
-template <bool PUSH>
-__device__ inline void topo_level(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                                  uint32_t *word, int32_t n, uint32_t bit, uint32_t xbit, Ctl *ctl,
-                                  int iter, Items &it, int32_t *sbuf, uint32_t *scount,
-                                  uint32_t *sbase, int32_t *outq, bool &found,
-                                  unsigned long long &cx, unsigned long long &ce,
-                                  unsigned long long &cv) {
+// ---------------------------------------------------------------------------
+// q: one lane per vertex; a live vertex with the bit and without the expanded
+// bit sets the expanded bit and scans its row serially (reach_q.cu:44-79)
+// ---------------------------------------------------------------------------
+__global__ void k_reach_sweep(ReachArgs a) {
+    __shared__ Items it;
+    __shared__ unsigned long long sh[16];
+    if (!level_begin(a, false)) return;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(a.n, beg, end);
+    bool found = false;
+    unsigned long long cx = 0, ce = 0, cv = 0;
+    huge_phase<false>(a, it, nullptr, found, cx, ce, cv);
+    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
+        int32_t v = (int32_t)vv;
+        uint32_t w = a.word[v];
+        if (w & ST_E) continue;
+        uint32_t c = colour_of(w);
+        for (int d = 0; d < 2; d++) {
+            uint32_t bit = dbit(d), xbit = dxbit(d);
+            if (!((a.dirs >> d) & 1u) || (w & (bit | xbit)) != bit) continue;
+            atomicOr(&a.word[v], xbit);
+            const int32_t *row = drow(a, d), *col = dcol(a, d);
+            int32_t e0 = row[v], e1 = row[v + 1];
+            if ((uint32_t)(e1 - e0) >= a.huge_thr) {
+                push_huge(a, d, v);
+                found = true;
+                continue;
+            }
+            cx++;
+            ce += (unsigned long long)(e1 - e0);
+            for (int32_t e = e0; e < e1; e++) {
+                if (try_claim(a.word, col[e], c, bit)) {
+                    found = true;
+                    cv++;
+                }
+            }
+        }
+    }
+    chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
+}
+
+// ---------------------------------------------------------------------------
+// t: sweep with balanced expansion: the workgroup's 256 vertices contribute up
+// to 512 items (forward and backward rows), expanded arc-parallel
+// ---------------------------------------------------------------------------
+__global__ void k_reach_topo(ReachArgs a) {
+    __shared__ Items it;
+    __shared__ unsigned long long sh[16];
+    if (!level_begin(a, false)) return;
+    int64_t beg, end;
+    my_range(a.n, beg, end);
+    bool found = false;
+    unsigned long long cx = 0, ce = 0, cv = 0;
+    huge_phase<false>(a, it, nullptr, found, cx, ce, cv);
     for (int64_t base = beg; base < end; base += blockDim.x) {
         int64_t vv = base + threadIdx.x;
-        uint32_t deg = 0, c = 0;
-        int32_t start = 0;
+        uint32_t deg[2] = {0, 0}, c = 0;
+        int32_t start[2] = {0, 0};
         if (vv < end) {
             int32_t v = (int32_t)vv;
-            uint32_t w = word[v];
-            if ((w & (bit | xbit | ST_E)) == bit) {
-                word[v] = w | xbit;
-                start = row[v];
-                deg = (uint32_t)(row[v + 1] - start);
+            uint32_t w = a.word[v];
+            if (!(w & ST_E)) {
                 c = colour_of(w);
+                uint32_t xb = 0;
+                for (int d = 0; d < 2; d++) {
+                    if (!((a.dirs >> d) & 1u) || (w & (dbit(d) | dxbit(d))) != dbit(d)) continue;
+                    xb |= dxbit(d);
+                    const int32_t *row = drow(a, d);
+                    start[d] = row[v];
+                    deg[d] = (uint32_t)(row[v + 1] - start[d]);
+                    if (deg[d] >= a.huge_thr) {
+                        push_huge(a, d, v);
+                        found = true;
+                        deg[d] = 0;
+                        continue;
+                    }
+                    cx++;
+                    ce += deg[d];
+                }
+                if (xb) atomicOr(&a.word[v], xb);
+            }
+        }
+        uint32_t total;
+        uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total);
+        it.off[2 * threadIdx.x] = off;
+        it.off[2 * threadIdx.x + 1] = off + deg[0];
+        it.start[2 * threadIdx.x] = start[0];
+        it.start[2 * threadIdx.x + 1] = start[1];
+        it.cd[2 * threadIdx.x] = c;
+        it.cd[2 * threadIdx.x + 1] = c | 0x80000000u;
+        __syncthreads();
+        if (total > 0) expand_items<false>(a, it, 2 * blockDim.x, total, nullptr, found, cv);
+        __syncthreads();
+    }
+    chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
+}
+
+// ---------------------------------------------------------------------------
+// d: frontier queues, one lane per item, serial row scan (reach_d.cu:36-64);
+// the item space is the forward queue followed by the backward queue
+// ---------------------------------------------------------------------------
+__global__ void k_reach_queue(ReachArgs a) {
+    __shared__ Items it;
+    __shared__ Stage st;
+    __shared__ unsigned long long sh[16];
+    if (!level_begin(a, true)) return;
+    st_init(st);
+    __syncthreads();
+    bool found = false;
+    unsigned long long cx = 0, ce = 0, cv = 0;
+    huge_phase<true>(a, it, &st, found, cx, ce, cv);
+    st_flush(st, a);  // the queue loop below may not run
+    uint32_t qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
+    uint32_t qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
+    uint32_t tot = qs0 + qs1;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        if (i < tot) {
+            int d = i < qs0 ? 0 : 1;
+            int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
+            uint32_t c = colour_of(a.word[v]), bit = dbit(d);
+            const int32_t *row = drow(a, d), *col = dcol(a, d);
+            int32_t e0 = row[v], e1 = row[v + 1];
+            if ((uint32_t)(e1 - e0) >= a.huge_thr) {
+                push_huge(a, d, v);
+                found = true;
+                e1 = e0;
+            } else {
+                cx++;
+                ce += (unsigned long long)(e1 - e0);
+            }
+            for (int32_t e = e0; e < e1; e++) {
+                int32_t u = col[e];
+                if (try_claim(a.word, u, c, bit)) {
+                    found = true;
+                    cv++;
+                    st_push(st, d, u, a);
+                }
+            }
+        }
+        st_flush(st, a);
+    }
+    chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
+}
+
+// balanced expansion of queue items [0, tot): forward items first (qs0 of them)
+__device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t tot, Items &it,
+                                      Stage &st, bool &found, unsigned long long &cx,
+                                      unsigned long long &ce, unsigned long long &cv) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t deg = 0, cd = 0;
+        int32_t start = 0;
+        if (i < tot) {
+            int d = i < qs0 ? 0 : 1;
+            int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
+            const int32_t *row = drow(a, d);
+            start = row[v];
+            deg = (uint32_t)(row[v + 1] - start);
+            cd = colour_of(a.word[v]) | ((uint32_t)d << 31);
+            if (deg >= a.huge_thr) {
+                push_huge(a, d, v);
+                found = true;
+                deg = 0;
+            } else {
                 cx++;
                 ce += deg;
             }
@@ -201,161 +419,100 @@ __device__ inline void topo_level(const int32_t *__restrict__ row, const int32_t
         uint32_t off = block_excl_scan(deg, it.shw, total);
         it.off[threadIdx.x] = off;
         it.start[threadIdx.x] = start;
-        it.colour[threadIdx.x] = c;
+        it.cd[threadIdx.x] = cd;
         __syncthreads();
-        if (total > 0)
-            expand_items<PUSH>(col, word, bit, it, total, sbuf, scount, sbase, outq, ctl, iter, found,
-                               cv);
-        if (PUSH) stage_flush(sbuf, scount, sbase, outq, ctl, iter);
-        else __syncthreads();
+        if (total > 0) expand_items<true>(a, it, blockDim.x, total, &st, found, cv);
+        st_flush(st, a);
     }
 }
 
-__global__ void k_reach_topo(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                             uint32_t *word, int32_t n, uint32_t bit, uint32_t xbit, Ctl *ctl,
-                             int iter, int instr) {
+// ---------------------------------------------------------------------------
+// l: frontier queues with balanced expansion (reach_l.cu:151-253)
+// ---------------------------------------------------------------------------
+__global__ void k_reach_lb(ReachArgs a) {
     __shared__ Items it;
+    __shared__ Stage st;
     __shared__ unsigned long long sh[16];
-    if (chain_exit(ctl, iter)) return;
+    if (!level_begin(a, true)) return;
+    st_init(st);
+    __syncthreads();
+    uint32_t qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
+    uint32_t qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
-    topo_level<false>(row, col, word, n, bit, xbit, ctl, iter, it, nullptr, nullptr, nullptr,
-                      nullptr, found, cx, ce, cv);
-    chain_progress(ctl, iter, found);
-    if (instr) instr_flush(ctl, cx, ce, cv, sh);
+    huge_phase<true>(a, it, &st, found, cx, ce, cv);
+    st_flush(st, a);
+    queue_balanced(a, qs0, qs0 + qs1, it, st, found, cx, ce, cv);
+    chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }
 
 // ---------------------------------------------------------------------------
-// d: frontier queue, one lane per item, serial row scan (reach_d.cu:36-64)
+// h: a direction whose frontier holds fewer than pull_thr vertices expands its
+// queue (balanced push); a larger one runs bottom-up: every live vertex of
+// the direction that is not reached scans its reverse row for a reached
+// same-colour neighbour and stops at the first.  Pulled visits are queued too,
+// so the next level may push again.
 // ---------------------------------------------------------------------------
-__global__ void k_reach_queue(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                              uint32_t *word, uint32_t bit, const int32_t *__restrict__ inq,
-                              int32_t *outq, Ctl *ctl, int iter, int instr) {
-    __shared__ int32_t sbuf[SQ_CAP];
-    __shared__ uint32_t scount, sbase;
+__global__ void k_reach_hybrid(ReachArgs a) {
+    __shared__ Items it;
+    __shared__ Stage st;
     __shared__ unsigned long long sh[16];
-    if (chain_exit(ctl, iter)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->qsize[(iter + 2) & 3] = 0;
-    if (threadIdx.x == 0) scount = 0;
+    if (!level_begin(a, true)) return;
+    st_init(st);
     __syncthreads();
-    uint32_t qs = ctl->qsize[iter & 3];
+    uint32_t qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
+    uint32_t qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
+    bool pull0 = qs0 > 0 && qs0 >= a.pull_thr, pull1 = qs1 > 0 && qs1 >= a.pull_thr;
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < qs; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        if (i < qs) {
-            int32_t v = inq[i];
-            uint32_t c = colour_of(word[v]);
-            int32_t e0 = row[v], e1 = row[v + 1];
-            cx++;
-            ce += (unsigned long long)(e1 - e0);
-            for (int32_t e = e0; e < e1; e++) {
-                int32_t u = col[e];
-                if (try_claim(word, u, c, bit)) {
-                    found = true;
-                    cv++;
-                    stage_push(sbuf, &scount, u, outq, ctl, iter);
+    huge_phase<true>(a, it, &st, found, cx, ce, cv);
+    st_flush(st, a);
+    // push part: queue items of the directions that do not pull
+    uint32_t p0 = pull0 ? 0u : qs0, p1 = pull1 ? 0u : qs1;
+    if (p0 + p1 > 0) queue_balanced(a, p0, p0 + p1, it, st, found, cx, ce, cv);
+    if (pull0 || pull1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            a.ctl->pull_levels += (pull0 ? 1u : 0u) + (pull1 ? 1u : 0u);
+        int64_t beg, end;
+        my_range(a.n, beg, end);
+        for (int64_t base = beg; base < end; base += blockDim.x) {
+            int64_t vv = base + threadIdx.x;
+            if (vv < end) {
+                int32_t v = (int32_t)vv;
+                uint32_t w = a.word[v];
+                if (!(w & ST_E)) {
+                    uint32_t c = colour_of(w);
+                    for (int d = 0; d < 2; d++) {
+                        if (!(d == 0 ? pull0 : pull1)) continue;
+                        uint32_t bit = dbit(d);
+                        if (w & bit) continue;
+                        // reverse direction: a forward visit needs a reached in-neighbour
+                        const int32_t *row = drow(a, 1 - d), *col = dcol(a, 1 - d);
+                        int32_t e0 = row[v], e1 = row[v + 1];
+                        cx++;
+                        for (int32_t e = e0; e < e1; e++) {
+                            ce++;
+                            uint32_t wu = a.word[col[e]];
+                            if ((wu & (bit | ST_E)) == bit && colour_of(wu) == c) {
+                                uint32_t old = atomicOr(&a.word[v], bit);
+                                if (!(old & bit)) {
+                                    found = true;
+                                    cv++;
+                                    st_push(st, d, v, a);
+                                }
+                                break;
+                            }
+                        }
+                    }
                 }
             }
+            st_maybe_flush(st, a);
         }
-        stage_flush(sbuf, &scount, &sbase, outq, ctl, iter);
+        st_flush(st, a);
     }
-    chain_progress(ctl, iter, found);
-    if (instr) instr_flush(ctl, cx, ce, cv, sh);
-}
-
-// ---------------------------------------------------------------------------
-// l: frontier queue with balanced edge expansion (reach_l.cu:151-253).
-// XBIT != 0 also marks each item expanded (hybrid queue levels).
-// ---------------------------------------------------------------------------
-__device__ inline void lb_level(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                                uint32_t *word, uint32_t bit, uint32_t xbit,
-                                const int32_t *__restrict__ inq, int32_t *outq, Ctl *ctl, int iter,
-                                uint32_t qs, Items &it, int32_t *sbuf, uint32_t *scount,
-                                uint32_t *sbase, bool &found, unsigned long long &cx,
-                                unsigned long long &ce, unsigned long long &cv) {
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < qs; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t deg = 0, c = 0;
-        int32_t start = 0;
-        if (i < qs) {
-            int32_t v = inq[i];
-            uint32_t w = word[v];
-            if (xbit) word[v] = w | xbit;
-            start = row[v];
-            deg = (uint32_t)(row[v + 1] - start);
-            c = colour_of(w);
-            cx++;
-            ce += deg;
-        }
-        uint32_t total;
-        uint32_t off = block_excl_scan(deg, it.shw, total);
-        it.off[threadIdx.x] = off;
-        it.start[threadIdx.x] = start;
-        it.colour[threadIdx.x] = c;
-        __syncthreads();
-        if (total > 0)
-            expand_items<true>(col, word, bit, it, total, sbuf, scount, sbase, outq, ctl, iter, found,
-                               cv);
-        stage_flush(sbuf, scount, sbase, outq, ctl, iter);
-    }
-}
-
-__global__ void k_reach_lb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                           uint32_t *word, uint32_t bit, const int32_t *__restrict__ inq,
-                           int32_t *outq, Ctl *ctl, int iter, int instr) {
-    __shared__ Items it;
-    __shared__ int32_t sbuf[SQ_CAP];
-    __shared__ uint32_t scount, sbase;
-    __shared__ unsigned long long sh[16];
-    if (chain_exit(ctl, iter)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->qsize[(iter + 2) & 3] = 0;
-    if (threadIdx.x == 0) scount = 0;
-    __syncthreads();
-    uint32_t qs = ctl->qsize[iter & 3];
-    bool found = false;
-    unsigned long long cx = 0, ce = 0, cv = 0;
-    lb_level(row, col, word, bit, 0u, inq, outq, ctl, iter, qs, it, sbuf, &scount, &sbase, found,
-             cx, ce, cv);
-    chain_progress(ctl, iter, found);
-    if (instr) instr_flush(ctl, cx, ce, cv, sh);
-}
-
-// ---------------------------------------------------------------------------
-// h: per level, the frontier size in the queue ring picks the expansion: below
-// `threshold` items the level expands the queue (balanced); at or above it the
-// level sweeps all vertices with the bit and without the expanded bit (the
-// same set), which reads the vertex words once, coalesced, instead of
-// gathering them through the queue.  Both forms mark expanded vertices and
-// queue new visits, so consecutive levels may use either.
-// ---------------------------------------------------------------------------
-__global__ void k_reach_hybrid(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                               uint32_t *word, int32_t n, uint32_t bit, uint32_t xbit,
-                               const int32_t *__restrict__ inq, int32_t *outq, Ctl *ctl, int iter,
-                               uint32_t threshold, int instr) {
-    __shared__ Items it;
-    __shared__ int32_t sbuf[SQ_CAP];
-    __shared__ uint32_t scount, sbase;
-    __shared__ unsigned long long sh[16];
-    if (chain_exit(ctl, iter)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->qsize[(iter + 2) & 3] = 0;
-    if (threadIdx.x == 0) scount = 0;
-    __syncthreads();
-    uint32_t qs = ctl->qsize[iter & 3];
-    bool found = false;
-    unsigned long long cx = 0, ce = 0, cv = 0;
-    if (qs >= threshold) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->mode_sweeps++;
-        topo_level<true>(row, col, word, n, bit, xbit, ctl, iter, it, sbuf, &scount, &sbase, outq,
-                         found, cx, ce, cv);
-    } else {
-        lb_level(row, col, word, bit, xbit, inq, outq, ctl, iter, qs, it, sbuf, &scount, &sbase,
-                 found, cx, ce, cv);
-    }
-    chain_progress(ctl, iter, found);
-    if (instr) instr_flush(ctl, cx, ce, cv, sh);
+    chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }
 
 __global__ void k_clear_bits(uint32_t *word, int32_t n, uint32_t bits) {
@@ -368,38 +525,30 @@ __global__ void k_clear_bits(uint32_t *word, int32_t n, uint32_t bits) {
 }
 
 // ---------------------------------------------------------------------------
-#define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
-static inline uint32_t bit_of(int dir) { return dir == 0 ? ST_F : ST_B; }
-static inline uint32_t xbit_of(int dir) { return dir == 0 ? ST_FX : ST_BX; }
+// launchers: with events set, the launch records them around the kernel
+// itself (hipExtLaunchKernelGGL), so their interval is the kernel's duration
+// ---------------------------------------------------------------------------
+template <typename K>
+static void launch_reach_kernel(K kernel, const Launch &l, const ReachArgs &a) {
+    if (l.ev_start)
+        hipExtLaunchKernelGGL(kernel, dim3(l.grid), dim3(l.block), 0, l.stream, l.ev_start, l.ev_stop,
+                              0, a);
+    else
+        hipLaunchKernelGGL(kernel, dim3(l.grid), dim3(l.block), 0, l.stream, a);
+}
 
-void launch_reach_sweep(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                        int32_t n, int dir, Ctl *ctl, int iter, int instr) {
-    hipLaunchKernelGGL(k_reach_sweep, GRID(l), row, col, word, n, bit_of(dir), xbit_of(dir), ctl,
-                       iter, instr);
+void launch_reach(const Launch &l, int sched, const ReachArgs &a) {
+    switch (sched) {
+    case 0: launch_reach_kernel(k_reach_sweep, l, a); break;
+    case 1: launch_reach_kernel(k_reach_topo, l, a); break;
+    case 2: launch_reach_kernel(k_reach_queue, l, a); break;
+    case 3: launch_reach_kernel(k_reach_lb, l, a); break;
+    default: launch_reach_kernel(k_reach_hybrid, l, a); break;
+    }
 }
-void launch_reach_topo(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                       int32_t n, int dir, Ctl *ctl, int iter, int instr) {
-    hipLaunchKernelGGL(k_reach_topo, GRID(l), row, col, word, n, bit_of(dir), xbit_of(dir), ctl,
-                       iter, instr);
-}
-void launch_reach_queue(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                        const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter, int instr) {
-    hipLaunchKernelGGL(k_reach_queue, GRID(l), row, col, word, bit_of(dir), inq, outq, ctl, iter,
-                       instr);
-}
-void launch_reach_lb(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                     const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter, int instr) {
-    hipLaunchKernelGGL(k_reach_lb, GRID(l), row, col, word, bit_of(dir), inq, outq, ctl, iter,
-                       instr);
-}
-void launch_reach_hybrid(const Launch &l, const int32_t *row, const int32_t *col, uint32_t *word,
-                         int32_t n, const int32_t *inq, int32_t *outq, int dir, Ctl *ctl, int iter,
-                         uint32_t threshold, int instr) {
-    hipLaunchKernelGGL(k_reach_hybrid, GRID(l), row, col, word, n, bit_of(dir), xbit_of(dir), inq,
-                       outq, ctl, iter, threshold, instr);
-}
+
 void launch_clear_bits(const Launch &l, uint32_t *word, int32_t n, uint32_t bits) {
-    hipLaunchKernelGGL(k_clear_bits, GRID(l), word, n, bits);
+    hipLaunchKernelGGL(k_clear_bits, dim3(l.grid), dim3(l.block), 0, l.stream, word, n, bits);
 }
 
 }  // namespace gpuscc

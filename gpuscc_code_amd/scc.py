@@ -33,6 +33,8 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     hybrid_threshold: int = 0
     slot_bits: int = 0
     grid_blocks: int = 0
+    sequential_reach: int = 0
+    huge_row: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -41,6 +43,8 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.hybrid_threshold = self.hybrid_threshold
         o.slot_bits = self.slot_bits
         o.grid_blocks = self.grid_blocks
+        o.sequential_reach = self.sequential_reach
+        o.huge_row = self.huge_row
         return o
 
 

@@ -63,7 +63,11 @@ typedef struct {
     int32_t hybrid_threshold;  /* h: frontier size (vertices) at or above which a level sweeps */
     int32_t slot_bits;         /* log2 of the election table size; default from n */
     int32_t grid_blocks;       /* workgroups of the grid-stride kernels; default from the CU count */
-    int32_t reserved[3];
+    int32_t sequential_reach;  /* 1: forward reach to its fixpoint, then backward (scc.cu:285-704);
+                                  0: both directions advance in the same level kernels */
+    int32_t huge_row;          /* rows of at least this many arcs are split over all workgroups
+                                  (deferred one level); default 1024 for q/d, 4096 for t/l/h */
+    int32_t reserved[1];
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -131,8 +135,9 @@ scc_status scc_state_get(scc_ctx *ctx, uint32_t *words, int32_t *roots);
 /* trim1/trim (scc_routines.cu:584-643): trim within colours to the fixpoint. */
 scc_status scc_phase_trim(scc_ctx *ctx, int64_t *iterations);
 /* fwd_reach_X / bwd_reach_X (reach_q.h:8-21, reach_t.h, reach_d.h, reach_l.h):
- * dir 0 = forward over the CSR (bit F), 1 = backward over CSR^T (bit B).
- * Seeds are the live vertices holding the bit. */
+ * dir 0 = forward over the CSR (bit F), 1 = backward over CSR^T (bit B),
+ * 2 = both in the same level kernels.  Seeds are the live vertices holding
+ * the bit.  *depth = level kernels that did work. */
 scc_status scc_phase_reach(scc_ctx *ctx, int dir, int schedule, int64_t *depth);
 /* update (scc_routines.cu:526-555): recolour = 1 splits colours and eliminates
  * F&B; 0 only elects (first pivot, scc.cu:235-245). */

@@ -160,27 +160,67 @@ def random_state(n, rng, ncol=3, p_elim=0.2):
 
 
 @pytest.mark.parametrize("sched", SCHEDS)
-@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("direction", [0, 1, 2])
 def test_phase_reach(ctx, sched, direction):
     rng = np.random.default_rng(7 + direction)
     g = random_digraph(20000, 60000, 11)
     ctx.upload(g)
     colour, state = random_state(g.n, rng)
-    bit = 2 if direction == 0 else 1
-    seeds = rng.choice(g.n, 30, replace=False)
-    state[seeds] = np.where(state[seeds] & 4, state[seeds], state[seeds] | bit)
+    bits = {0: [2], 1: [1], 2: [2, 1]}[direction]
+    for bit in bits:  # separate seed sets per direction
+        seeds = rng.choice(g.n, 30, replace=False)
+        state[seeds] = np.where(state[seeds] & 4, state[seeds], state[seeds] | bit)
     ctx.set_state(scc.make_word(colour, state))
     depth = ctx.reach(direction, sched)
     assert depth >= 1
     words, _ = ctx.get_state()
     _, gst = scc.split_word(words)
     ost = state.copy()
-    if direction == 0:
-        oracle.reach(g.row, g.col, colour, ost, bit)
-    else:
-        oracle.reach(g.rowT, g.colT, colour, ost, bit)
+    for bit in bits:
+        if bit == 2:
+            oracle.reach(g.row, g.col, colour, ost, bit)
+        else:
+            oracle.reach(g.rowT, g.colT, colour, ost, bit)
     assert np.array_equal(gst & 0x1F, ost & 0x1F)
     assert np.array_equal(scc.split_word(words)[0], colour)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_phase_reach_pull_levels(sched):
+    """A tiny pull threshold forces the hybrid's bottom-up levels everywhere."""
+    rng = np.random.default_rng(17)
+    g = random_digraph(30000, 120000, 19)
+    with scc.Context(0, scc.alg_opt(hybrid_threshold=1)) as c:
+        c.upload(g)
+        colour, state = random_state(g.n, rng, ncol=2)
+        for bit in (2, 1):
+            seeds = rng.choice(g.n, 5, replace=False)
+            state[seeds] = np.where(state[seeds] & 4, state[seeds], state[seeds] | bit)
+        c.set_state(scc.make_word(colour, state))
+        c.reach(2, sched)
+        words, _ = c.get_state()
+        ost = state.copy()
+        oracle.reach(g.row, g.col, colour, ost, 2)
+        oracle.reach(g.rowT, g.colT, colour, ost, 1)
+        assert np.array_equal(scc.split_word(words)[1] & 0x1F, ost & 0x1F)
+        ref, _ = oracle.tarjan(g.row, g.col)
+        lab, _ = c.run(sched)
+        assert np.array_equal(lab, ref)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_sequential_reach_matches(sched):
+    """Forward-then-backward chains (the reference's order) give the same run."""
+    g = scc.gen_rmat(13, 8, seed=3)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    o = oracle.fb(g.row, g.col, g.rowT, g.colT, slot_bits=default_slot_bits(g.n), hash_kind=1)
+    with scc.Context(0, scc.alg_opt(sequential_reach=1)) as c:
+        c.upload(g)
+        for variant in (0, 1, 2):
+            lab, st = c.run(sched, variant)
+            assert np.array_equal(lab, ref)
+        lab, st = c.run(sched, 0)
+        assert st["steps"] == o["steps"]
 
 
 def test_phase_trim(ctx):
@@ -277,3 +317,19 @@ def test_canonical_property(ctx):
     for sched in SCHEDS:
         lab, _ = ctx.run(sched)
         assert is_canonical(lab)
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_huge_rows_split(sched):
+    """A tiny huge-row threshold sends most rows through the grid-wide split."""
+    for huge in (2, 5):
+        with scc.Context(0, scc.alg_opt(huge_row=huge)) as c:
+            for g in (scc.gen_rmat(13, 8, seed=5), scc.gen_mesh(64, 0.6, seed=2), demo_graph()):
+                for variant in (0, 1, 2):
+                    ref, _ = oracle.tarjan(g.row, g.col)
+                    c.upload(g)
+                    lab, st = c.run(sched, variant)
+                    assert np.array_equal(lab, ref), (huge, variant)
+                    o = oracle.fb(g.row, g.col, g.rowT, g.colT, variant=variant,
+                                  slot_bits=default_slot_bits(g.n), hash_kind=1)
+                    assert st["steps"] == o["steps"]
