@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--schedule", default="t", choices=list("qtdlh"))
     ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2])
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "mesh", "batch"],
+                    help="rmat: R-MAT (default, scale 22); mesh: 2048x2048 grid; "
+                         "batch: 4096 uneven ER graphs with planted cycles")
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=8)
     ap.add_argument("--seed", type=int, default=2)
@@ -97,12 +100,26 @@ def main():
     from gpuscc_code_amd import scc
 
     t0 = time.time()
-    g = scc.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed,
-                     permute=not args.no_permute)
-    log(f"generated R-MAT s{args.scale}: n={g.n} m={g.m} in {time.time() - t0:.1f}s")
+    if args.workload == "rmat":
+        g = scc.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed,
+                         permute=not args.no_permute)
+        wl = (f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, a=0.57 b=0.19 c=0.19, "
+              f"seed {args.seed}, dedup, no self-loops, {'un' if args.no_permute else ''}permuted ids")
+    elif args.workload == "mesh":
+        g = scc.gen_mesh(2048, 0.6, seed=3)
+        wl = "2048x2048 directed mesh, lattice edge bidirectional w.p. 0.6 else one random direction, seed 3"
+    else:
+        g = scc.gen_batch(4096, 64, 65536, 1.5, 6.0, 0.05, 0.5, seed=4)
+        wl = ("4096 directed ER graphs, n log-uniform in [64, 65536], mean out-degree U(1.5, 6), "
+              "planted cycles Geometric(0.05) over half the vertices, seed 4, packed CSR")
+    log(f"generated {args.workload}: n={g.n} m={g.m} in {time.time() - t0:.1f}s")
     ctx = scc.Context(0, scc.alg_opt(sequential_reach=1 if args.sequential else 0,
                                      hybrid_threshold=args.pull_threshold))
     ctx.upload(g)
+    num_graphs = 1
+    if args.workload == "batch":
+        ctx.set_batch(g.graph_offsets)
+        num_graphs = len(g.graph_offsets) - 1
 
     # untimed counting run: algorithmic work of the reachability levels
     ctx.profile(1)
@@ -154,9 +171,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, "
-                               f"a=0.57 b=0.19 c=0.19, seed {args.seed}, dedup, no self-loops, "
-                               f"{'un' if args.no_permute else ''}permuted ids",
+        "ms_per_graph": round(ms_per_step / num_graphs, 6), "graphs_per_s": round(num_graphs * 1e3 / ms_per_step, 1),
+        "config": {"workload": wl, "graphs": num_graphs,
                    "n": g.n, "m": g.m, "schedule": args.schedule,
                    "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
                    "reach": "sequential" if args.sequential else "fused"},

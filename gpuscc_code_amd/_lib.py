@@ -82,6 +82,7 @@ SIGNATURES = {
     "scc_upload": (_st, [_vp, ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p, _i32p]),
     "scc_run": (_st, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.POINTER(SccStats)]),
     "scc_labels_device": (_st, [_vp, ctypes.POINTER(_i32p)]),
+    "scc_set_batch": (_st, [_vp, _i32p, ctypes.c_int32]),
     "scc_state_set": (_st, [_vp, _u32p, _i32p]),
     "scc_state_get": (_st, [_vp, _u32p, _i32p]),
     "scc_phase_trim": (_st, [_vp, _i64p]),

@@ -42,6 +42,8 @@ struct scc_ctx {
     Ctl *d_ctl = nullptr;
     Ctl *h_ctl = nullptr;    // pinned mirror
     uint16_t *d_score = nullptr;  // learned-rule scores (per vertex), may be null
+    int32_t *d_goff = nullptr;    // batch: graph offsets [num_graphs + 1]
+    int32_t num_graphs = 0;
     bool uploaded = false, labels_valid = false;
     int guess[3] = {4, 4, 2};     // chain batch guesses: fwd, bwd, trim
     int64_t syncs = 0;
@@ -290,6 +292,9 @@ void free_graph(scc_ctx *c) {
     }
     if (c->d_score) (void)hipFree(c->d_score);
     c->d_score = nullptr;
+    if (c->d_goff) (void)hipFree(c->d_goff);
+    c->d_goff = nullptr;
+    c->num_graphs = 0;
     c->uploaded = false;
     c->labels_valid = false;
 }
@@ -462,7 +467,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     scc_status s = SCC_OK;
     auto body = [&]() -> scc_status {
         Launch l = launch_for(c);
-        launch_init(l, c->d_word, c->d_root, n, nullptr);
+        launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();
@@ -565,6 +570,34 @@ scc_status scc_profile_get(scc_ctx *c, scc_profile_data *out, int reset) {
     if (!c || !out) return SCC_E_INPUT;
     *out = c->prof;
     if (reset) std::memset(&c->prof, 0, sizeof(c->prof));
+    return SCC_OK;
+}
+
+scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "scc_set_batch before scc_upload");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_goff) HIPCHK(c, hipFree(c->d_goff));
+    c->d_goff = nullptr;
+    c->num_graphs = 0;
+    if (G == 0) return SCC_OK;
+    if (G < 0 || !off) return fail(c, SCC_E_INPUT, "bad batch");
+    if (off[0] != 0 || off[G] != c->g.n) return fail(c, SCC_E_INPUT, "offsets must run from 0 to n");
+    for (int32_t i = 0; i < G; i++)
+        if (off[i + 1] < off[i]) return fail(c, SCC_E_INPUT, "offsets decrease");
+    HIPCHK(c, hipMalloc(&c->d_goff, sizeof(int32_t) * ((size_t)G + 1)));
+    HIPCHK(c, hipMemcpy(c->d_goff, off, sizeof(int32_t) * ((size_t)G + 1), hipMemcpyHostToDevice));
+    if (c->g.n > 0) {
+        launch_ctl_reset(c->stream, c->d_ctl, 8);
+        launch_batch_check(launch_for(c), c->g, c->d_goff, G, c->d_ctl);
+        CHK(read_ctl(c));
+        if (c->h_ctl->count != 0) {
+            (void)hipFree(c->d_goff);
+            c->d_goff = nullptr;
+            return fail(c, SCC_E_INPUT, "an arc crosses graphs of the batch");
+        }
+    }
+    c->num_graphs = G;
     return SCC_OK;
 }
 

@@ -12,14 +12,42 @@ namespace gpuscc {
 // init: colour from init_colour (or 1, thrust::fill scc.cu:214), state 0,
 // root = self
 // ---------------------------------------------------------------------------
-__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const uint32_t *init_colour) {
+// graph of vertex v in a batch: largest g with off[g] <= v
+__device__ inline uint32_t graph_of(const int32_t *off, int32_t G, int32_t v) {
+    int32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        int32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= v) lo = mid; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+// init: colour 1 (thrust::fill, scc.cu:214) or, for a batch, graph id + 1;
+// state 0; root = self
+__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G) {
     int64_t beg, end;
     my_range(n, beg, end);
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
-        uint32_t c = init_colour ? (init_colour[v] & COLOUR_MASK) : 1u;
+        uint32_t c = off ? ((graph_of(off, G, (int32_t)v) + 1u) & COLOUR_MASK) : 1u;
         word[v] = make_word(c, 0);
         root[v] = (int32_t)v;
     }
+}
+
+// counts arcs whose ends lie in different graphs of a batch
+__global__ void k_batch_check(const int32_t *row, const int32_t *col, int32_t n, const int32_t *off,
+                              int32_t G, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
+    int64_t beg, end;
+    my_range(n, beg, end);
+    unsigned long long bad = 0;
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        uint32_t g = graph_of(off, G, (int32_t)v);
+        int32_t lo = off[g], hi = off[g + 1];
+        for (int32_t e = row[v]; e < row[v + 1]; e++) bad += (col[e] < lo || col[e] >= hi);
+    }
+    bad = block_sum(bad, sh);
+    if (threadIdx.x == 0 && bad) atomicAdd(&ctl->count, bad);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int64_t count, unsigned long long val) {
@@ -418,8 +446,12 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 // ---------------------------------------------------------------------------
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
 
-void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const uint32_t *ic) {
-    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, ic);
+void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
+                 int32_t G) {
+    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G);
+}
+void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
+    hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
 }
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v) {
     hipLaunchKernelGGL(k_fill_u64, GRID(l), p, count, v);

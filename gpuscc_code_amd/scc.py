@@ -167,6 +167,14 @@ class Context:
                                    _p(col, _lib._i32p), pT, cT))
         self.n = g.n
 
+    def set_batch(self, graph_offsets):
+        """Treat the uploaded CSR as a batch of graphs (None or [] clears)."""
+        if graph_offsets is None or len(graph_offsets) == 0:
+            self._chk(lib().scc_set_batch(self._h, None, 0))
+            return
+        off = np.ascontiguousarray(graph_offsets, np.int32)
+        self._chk(lib().scc_set_batch(self._h, _p(off, _lib._i32p), len(off) - 1))
+
     def run(self, schedule="t", variant=0, pivot=PIVOT_PLAIN, labels=True):
         sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
         var = VARIANTS[variant] if isinstance(variant, str) else int(variant)

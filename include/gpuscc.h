@@ -123,6 +123,16 @@ scc_status scc_upload(scc_ctx *ctx, int32_t n, int64_t m, const int32_t *row, co
 scc_status scc_run(scc_ctx *ctx, int schedule, int variant, int pivot_rule, int32_t *labels_out,
                    scc_stats *stats);
 
+/* Batched graphs: the uploaded CSR is the disjoint union of num_graphs graphs,
+ * graph g owning vertices [graph_offsets[g], graph_offsets[g+1]).  Later
+ * scc_run calls give every graph its own initial colour (g + 1), so each
+ * graph gets its own pivot per step and all graphs advance in the same
+ * launches (the reference's colour mechanism, scc_routines.cu:207-221,
+ * applied to the union).  No reference counterpart: the reference solves one
+ * graph per process.  Fails with SCC_E_INPUT if an arc crosses graphs.
+ * num_graphs = 0 clears the setting.  Labels stay global vertex ids. */
+scc_status scc_set_batch(scc_ctx *ctx, const int32_t *graph_offsets, int32_t num_graphs);
+
 /* Device address of the labels of the last run (valid until the next upload). */
 scc_status scc_labels_device(scc_ctx *ctx, const int32_t **dptr);
 

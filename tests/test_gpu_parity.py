@@ -333,3 +333,39 @@ def test_huge_rows_split(sched):
                     o = oracle.fb(g.row, g.col, g.rowT, g.colT, variant=variant,
                                   slot_bits=default_slot_bits(g.n), hash_kind=1)
                     assert st["steps"] == o["steps"]
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_batched_graphs(ctx, sched):
+    """Batch of uneven graphs: colour = graph id + 1; labels and final state vs the oracle."""
+    g = scc.gen_batch(160, 64, 8192, 1.5, 6.0, 0.05, 0.5, seed=4)
+    off = g.graph_offsets
+    gid = np.repeat(np.arange(len(off) - 1), np.diff(off)).astype(np.uint32)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    ctx.set_batch(off)
+    for variant in (0, 1, 2):
+        lab, st = ctx.run(sched, variant)
+        assert np.array_equal(lab, ref)
+        o = oracle.fb(g.row, g.col, g.rowT, g.colT, variant=variant, init_colour=gid + 1,
+                      slot_bits=default_slot_bits(g.n), hash_kind=1)
+        words, roots = ctx.get_state()
+        colour, state = scc.split_word(words)
+        assert st["steps"] == o["steps"]
+        assert np.array_equal(colour, o["colour"])
+        assert np.array_equal(roots, o["root"])
+    # labels never leave their graph
+    assert np.all(np.searchsorted(off, lab, side="right") == np.searchsorted(off, np.arange(g.n), side="right"))
+    ctx.set_batch(None)
+
+
+def test_batch_rejects_crossing_arcs(ctx):
+    g = graph_from_arcs(6, [(0, 1), (1, 0), (2, 4), (4, 3)])
+    ctx.upload(g)
+    with pytest.raises(scc.SccError):
+        ctx.set_batch([0, 3, 6])   # arc 2 -> 4 crosses
+    ctx.set_batch([0, 2, 6])
+    lab, _ = ctx.run("t")
+    assert list(lab) == [0, 0, 2, 3, 4, 5]
+    with pytest.raises(scc.SccError):
+        ctx.set_batch([0, 4, 3, 6])  # decreasing offsets
