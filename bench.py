@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--schedule", default="t", choices=list("qtdlh"))
+    ap.add_argument("--schedule", default="h", choices=list("qtdlh"))
     ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2])
     ap.add_argument("--workload", default="rmat", choices=["rmat", "mesh", "batch"],
                     help="rmat: R-MAT (default, scale 22); mesh: 2048x2048 grid; "

@@ -26,7 +26,7 @@ class SccOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("hybrid_policy", ctypes.c_int32),
                 ("hybrid_threshold", ctypes.c_int32), ("slot_bits", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("sequential_reach", ctypes.c_int32),
-                ("huge_row", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("huge_row", ctypes.c_int32), ("local_rounds", ctypes.c_int32)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",
@@ -91,6 +91,14 @@ SIGNATURES = {
     "scc_phase_trim2": (_st, [_vp, _i64p]),
     "scc_phase_wcc": (_st, [_vp, _i64p, _i64p]),
     "scc_phase_count_elimmed": (_st, [_vp, _i64p]),
+    "scc_phase_recolour": (_st, [_vp]),
+    "scc_phase_features": (_st, [_vp, ctypes.c_int32, _i32p, ctypes.c_int64, _fp]),
+    "scc_mlp_set_params": (_st, [_vp, _fp, ctypes.c_int32]),
+    "scc_mlp_get_params": (_st, [_vp, _fp, ctypes.c_int32]),
+    "scc_mlp_infer": (_st, [_vp, _fp, ctypes.c_int64, _fp]),
+    "scc_mlp_gradient": (_st, [_vp, _fp, _fp, ctypes.c_int64, _fp, _fp]),
+    "scc_mlp_train": (_st, [_vp, _fp, _fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_float, ctypes.c_uint64, _fp]),
     "scc_profile_set": (_st, [_vp, ctypes.c_int]),
     "scc_profile_get": (_st, [_vp, ctypes.POINTER(SccProfile), ctypes.c_int]),
     "scc_load_file": (_st, [ctypes.c_char_p, _gp]),

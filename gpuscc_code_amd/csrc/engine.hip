@@ -44,6 +44,16 @@ struct scc_ctx {
     uint16_t *d_score = nullptr;  // learned-rule scores (per vertex), may be null
     int32_t *d_goff = nullptr;    // batch: graph offsets [num_graphs + 1]
     int32_t num_graphs = 0;
+    // learned pivot scorer
+    float4 *d_stat = nullptr;     // static features per vertex (computed once per upload)
+    uint32_t *d_cnt = nullptr;    // live vertices per colour slot
+    float *d_feat = nullptr;      // features of every vertex (phase-level feature dumps)
+    float *d_wf = nullptr, *d_m = nullptr, *d_v = nullptr;  // FP32 master weights, Adam moments
+    uint16_t *d_wb = nullptr;     // BF16 weights used by every kernel
+    bool have_params = false;
+    bool stat_valid = false;
+    int adam_t = 0;
+    int cur_step = 0;
     bool uploaded = false, labels_valid = false;
     int guess[3] = {4, 4, 2};     // chain batch guesses: fwd, bwd, trim
     int64_t syncs = 0;
@@ -192,6 +202,7 @@ scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, 
                                               : (uint32_t)std::max<int64_t>(1024, c->g.n / 20);
     a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
                                       : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
+    a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
     const bool queues = sched >= SCC_SCHED_D;
     scc_status s = run_chain(
         c,
@@ -224,16 +235,43 @@ scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, 
     return SCC_OK;
 }
 
-// one election into the table that is not `prev`; winners -> state 19 + pivot list
-scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv) {
+// per-vertex features (and, with want_score, MLP scores) for the current words
+scc_status do_features(scc_ctx *c, bool want_score, float *feat_out) {
+    Launch l = launch_for(c);
+    if (!c->stat_valid) {
+        launch_static_features(l, c->g, c->d_stat);
+        c->stat_valid = true;
+    }
+    launch_fill_i32(l, (int32_t *)c->d_cnt, (int64_t)1 << c->slot_bits, 0);
+    launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits);
+    float step_f = 0.25f * std::log1p((float)c->cur_step);
+    launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f,
+                 want_score ? c->d_wb : nullptr, want_score ? c->d_score : nullptr, feat_out);
+    HIPCHK(c, hipGetLastError());
+    return SCC_OK;
+}
+
+// one election into the table that is not `prev`; winners -> state 19 + pivot list.
+// learned: the colour split runs first, then the scorer, then the election
+// with the scores in the keys.
+scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = false) {
     Launch l = launch_for(c);
     c->epoch++;
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     int64_t nslots = (int64_t)1 << c->slot_bits;
     if ((c->epoch & 0xFFFFu) < 2u) launch_fill_u64(l, cur, nslots, ~0ull);  // tag wrap
     launch_ctl_reset(c->stream, c->d_ctl, 4);
-    launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
-                  recolour, c->d_score);
+    if (learned) {
+        if (recolour)
+            launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits,
+                          elect_tag(c->epoch), 1, 0, nullptr);
+        CHK(do_features(c, true, nullptr));
+        launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
+                      0, 1, c->d_score);
+    } else {
+        launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
+                      recolour, 1, nullptr);
+    }
     launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
     launch_ctl_reset(c->stream, c->d_ctl, 64);  // the pivot list seeds both directions
     CHK(read_ctl(c));
@@ -259,7 +297,7 @@ scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
 }
 
 // update_wcc (wcc.cu:146-226): labels to fixpoint, fresh colours, election
-scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv) {
+scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = false) {
     Launch l = launch_for(c);
     launch_ctl_reset(c->stream, c->d_ctl, 16);
     launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
@@ -268,7 +306,7 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv) {
     CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
                   wg, 0, iters));
     launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
-    return do_elect(c, 0, npiv);
+    return do_elect(c, 0, npiv, learned);
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -292,11 +330,58 @@ void free_graph(scc_ctx *c) {
     }
     if (c->d_score) (void)hipFree(c->d_score);
     c->d_score = nullptr;
+    if (c->d_stat) (void)hipFree(c->d_stat);
+    c->d_stat = nullptr;
+    if (c->d_cnt) (void)hipFree(c->d_cnt);
+    c->d_cnt = nullptr;
+    if (c->d_feat) (void)hipFree(c->d_feat);
+    c->d_feat = nullptr;
     if (c->d_goff) (void)hipFree(c->d_goff);
     c->d_goff = nullptr;
     c->num_graphs = 0;
     c->uploaded = false;
     c->labels_valid = false;
+}
+
+}  // namespace
+
+namespace {
+
+// ---- learned pivot scorer ---------------------------------------------------
+scc_status mlp_alloc(scc_ctx *c) {
+    if (c->d_wf) return SCC_OK;
+    HIPCHK(c, hipMalloc(&c->d_wf, sizeof(float) * MLP_PARAMS));
+    HIPCHK(c, hipMalloc(&c->d_m, sizeof(float) * MLP_PARAMS));
+    HIPCHK(c, hipMalloc(&c->d_v, sizeof(float) * MLP_PARAMS));
+    HIPCHK(c, hipMalloc(&c->d_wb, sizeof(uint16_t) * MLP_PARAMS));
+    return SCC_OK;
+}
+
+struct DevBuf {  // scoped device allocation for the training entry points
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+scc_status mlp_work_alloc(scc_ctx *c, int64_t B, MlpWork &w, DevBuf *bufs) {
+    size_t b = (size_t)std::max<int64_t>(B, 1);
+    size_t sizes[8] = {b * MLP_HID * 2, b * MLP_HID * 2, b * MLP_HID * 4, b * MLP_HID * 4, b * 4,
+                       b * MLP_IN * 4, MLP_PARAMS * 4, 4};
+    for (int i = 0; i < 8; i++) HIPCHK(c, hipMalloc(&bufs[i].p, sizes[i]));
+    w.H1 = (uint16_t *)bufs[0].p;
+    w.H2 = (uint16_t *)bufs[1].p;
+    w.dH1 = (float *)bufs[2].p;
+    w.dH2 = (float *)bufs[3].p;
+    w.ds = (float *)bufs[4].p;
+    w.Xb = (float *)bufs[5].p;
+    w.grad = (float *)bufs[6].p;
+    w.loss = (float *)bufs[7].p;
+    return SCC_OK;
+}
+
+Launch mlp_launch(const scc_ctx *c, int64_t N) {
+    uint64_t need = ((uint64_t)std::max<int64_t>(N, 1) + 255) / 256;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(need, (uint64_t)c->cu_count * 4);
+    return Launch{c->stream, std::max<uint32_t>(grid, 1), 256};
 }
 
 }  // namespace
@@ -364,6 +449,8 @@ void scc_destroy(scc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     for (auto e : c->evs) (void)hipEventDestroy(e);
+    for (void *p : {(void *)c->d_wf, (void *)c->d_m, (void *)c->d_v, (void *)c->d_wb})
+        if (p) (void)hipFree(p);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -428,6 +515,11 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     size_t sbytes = sizeof(unsigned long long) << sb;
     HIPCHK(c, hipMalloc(&c->d_slots[0], sbytes));
     HIPCHK(c, hipMalloc(&c->d_slots[1], sbytes));
+    // scorer buffers (used by the learned pivot rule; allocated here so runs never allocate)
+    HIPCHK(c, hipMalloc(&c->d_stat, sizeof(float4) * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, hipMalloc(&c->d_cnt, sizeof(uint32_t) << sb));
+    HIPCHK(c, hipMalloc(&c->d_score, sizeof(uint16_t) * std::max<size_t>((size_t)n, 1)));
+    c->stat_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->d_row, row, nb, hipMemcpyHostToDevice, c->stream));
     if (m > 0) HIPCHK(c, hipMemcpyAsync(c->d_col, col, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_rowT, rowT, nb, hipMemcpyHostToDevice, c->stream));
@@ -452,16 +544,16 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     if (variant < SCC_VARIANT_PLAIN || variant > SCC_VARIANT_2W) return fail(c, SCC_E_INPUT, "bad variant");
     if (pivot_rule != SCC_PIVOT_PLAIN && pivot_rule != SCC_PIVOT_LEARNED)
         return fail(c, SCC_E_INPUT, "bad pivot rule");
-    if (pivot_rule == SCC_PIVOT_LEARNED && !c->d_score)
-        return fail(c, SCC_E_STATE, "learned pivot rule needs scores (scc_mlp_score)");
+    if (pivot_rule == SCC_PIVOT_LEARNED && !c->have_params)
+        return fail(c, SCC_E_STATE, "learned pivot rule needs parameters (scc_mlp_set_params or scc_mlp_train)");
+    const bool learned = pivot_rule == SCC_PIVOT_LEARNED;
     HIPCHK(c, hipSetDevice(c->device));
     scc_stats st;
     std::memset(&st, 0, sizeof(st));
     st.first_pivot = -1;
     const int32_t n = c->g.n;
-    uint16_t *score_saved = c->d_score;
-    if (pivot_rule == SCC_PIVOT_PLAIN) c->d_score = nullptr;
     c->syncs = 0;
+    c->cur_step = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t_start = std::chrono::steady_clock::now();
     scc_status s = SCC_OK;
@@ -475,7 +567,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         st.trim_iters += it;
         st.ms_trim += ms_since(tp);
         tp = std::chrono::steady_clock::now();
-        CHK(do_elect(c, 0, &npiv));  // first pivot: lowest live id (scc.cu:235-245)
+        CHK(do_elect(c, 0, &npiv, learned));  // first pivot: lowest live id (scc.cu:235-245)
         st.ms_update += ms_since(tp);
         if (npiv > 0) {
             int32_t fp = -1;
@@ -511,7 +603,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             st.trim_iters += it;
             st.ms_trim += ms_since(tp);
             tp = std::chrono::steady_clock::now();
-            CHK(do_elect(c, 1, &npiv));
+            c->cur_step = (int)st.steps;
+            CHK(do_elect(c, 1, &npiv, learned));
             if (variant != SCC_VARIANT_PLAIN && !wcc_done) {
                 CHK(do_count(c, 0, &ne));
                 if (ne >= one_pct) {  // scc.cu:787-821
@@ -525,7 +618,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                         st.trim2_pairs += pairs;
                     }
                     int64_t wi = 0;
-                    CHK(do_wcc(c, &wi, &npiv));
+                    CHK(do_wcc(c, &wi, &npiv, learned));
                 }
             } else if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1 && ne < one_pct) {
                 CHK(do_count(c, 0, &ne));  // scc.cu:879-890
@@ -539,7 +632,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return SCC_OK;
     };
     s = body();
-    c->d_score = score_saved;
     if (s != SCC_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_total = ms_since(t_start);
@@ -598,6 +690,150 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
         }
     }
     c->num_graphs = G;
+    return SCC_OK;
+}
+
+scc_status scc_mlp_set_params(scc_ctx *c, const float *params, int32_t count) {
+    if (!c || !params) return SCC_E_INPUT;
+    if (count != MLP_PARAMS) return fail(c, SCC_E_INPUT, "the scorer has 1505 parameters");
+    HIPCHK(c, hipSetDevice(c->device));
+    CHK(mlp_alloc(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_wf, params, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->d_m, 0, sizeof(float) * MLP_PARAMS));
+    HIPCHK(c, hipMemset(c->d_v, 0, sizeof(float) * MLP_PARAMS));
+    launch_to_bf16(c->stream, c->d_wf, c->d_wb);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->have_params = true;
+    return SCC_OK;
+}
+
+scc_status scc_mlp_get_params(scc_ctx *c, float *params, int32_t count) {
+    if (!c || !params) return SCC_E_INPUT;
+    if (count != MLP_PARAMS) return fail(c, SCC_E_INPUT, "the scorer has 1505 parameters");
+    if (!c->have_params) return fail(c, SCC_E_STATE, "no parameters");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(params, c->d_wf, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost));
+    return SCC_OK;
+}
+
+scc_status scc_mlp_infer(scc_ctx *c, const float *X, int64_t N, float *logits) {
+    if (!c || N < 0 || (N > 0 && (!X || !logits))) return SCC_E_INPUT;
+    if (!c->have_params) return fail(c, SCC_E_STATE, "no parameters");
+    if (N == 0) return SCC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf dx, dl;
+    HIPCHK(c, hipMalloc(&dx.p, sizeof(float) * MLP_IN * (size_t)N));
+    HIPCHK(c, hipMalloc(&dl.p, sizeof(float) * (size_t)N));
+    HIPCHK(c, hipMemcpy(dx.p, X, sizeof(float) * MLP_IN * (size_t)N, hipMemcpyHostToDevice));
+    launch_mlp_infer(mlp_launch(c, N), (const float *)dx.p, N, c->d_wb, (float *)dl.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(logits, dl.p, sizeof(float) * (size_t)N, hipMemcpyDeviceToHost));
+    return SCC_OK;
+}
+
+scc_status scc_mlp_gradient(scc_ctx *c, const float *X, const float *y, int64_t N, float *grad,
+                            float *loss) {
+    if (!c || N <= 0 || !X || !y || !grad) return SCC_E_INPUT;
+    if (!c->have_params) return fail(c, SCC_E_STATE, "no parameters");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf dx, dy, wb[8];
+    MlpWork w;
+    CHK(mlp_work_alloc(c, N, w, wb));
+    HIPCHK(c, hipMalloc(&dx.p, sizeof(float) * MLP_IN * (size_t)N));
+    HIPCHK(c, hipMalloc(&dy.p, sizeof(float) * (size_t)N));
+    HIPCHK(c, hipMemcpy(dx.p, X, sizeof(float) * MLP_IN * (size_t)N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dy.p, y, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+    launch_fill_f32(mlp_launch(c, 1), w.loss, 1, 0.f);
+    launch_mlp_fwdbwd(mlp_launch(c, N), (const float *)dx.p, (const float *)dy.p, nullptr, N, c->d_wb, w);
+    launch_mlp_grad(c->stream, N, w);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(grad, w.grad, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost));
+    if (loss) {
+        float ls = 0;
+        HIPCHK(c, hipMemcpy(&ls, w.loss, sizeof(float), hipMemcpyDeviceToHost));
+        *loss = ls / (float)N;
+    }
+    return SCC_OK;
+}
+
+scc_status scc_mlp_train(scc_ctx *c, const float *X, const float *y, int64_t N, int32_t epochs,
+                         int32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+    if (!c || N <= 0 || !X || !y || epochs < 0 || batch <= 0 || !(lr > 0.f)) return SCC_E_INPUT;
+    if (!c->have_params) return fail(c, SCC_E_STATE, "initialise the parameters first (scc_mlp_set_params)");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t B = std::min<int64_t>(batch, N);
+    DevBuf dx, dy, di, wb[8];
+    MlpWork w;
+    CHK(mlp_work_alloc(c, B, w, wb));
+    HIPCHK(c, hipMalloc(&dx.p, sizeof(float) * MLP_IN * (size_t)N));
+    HIPCHK(c, hipMalloc(&dy.p, sizeof(float) * (size_t)N));
+    HIPCHK(c, hipMalloc(&di.p, sizeof(int32_t) * (size_t)N));
+    HIPCHK(c, hipMemcpy(dx.p, X, sizeof(float) * MLP_IN * (size_t)N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dy.p, y, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+    std::vector<int32_t> perm((size_t)N);
+    for (int64_t i = 0; i < N; i++) perm[i] = (int32_t)i;
+    uint64_t rs = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&]() {  // xorshift64*
+        rs ^= rs >> 12; rs ^= rs << 25; rs ^= rs >> 27;
+        return rs * 0x2545F4914F6CDD1Dull;
+    };
+    for (int32_t ep = 0; ep < epochs; ep++) {
+        for (int64_t i = N - 1; i > 0; i--) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+        HIPCHK(c, hipMemcpy(di.p, perm.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice));
+        launch_fill_f32(mlp_launch(c, 1), w.loss, 1, 0.f);
+        int64_t nb = 0;
+        for (int64_t b0 = 0; b0 + B <= N; b0 += B, nb++) {
+            launch_mlp_fwdbwd(mlp_launch(c, B), (const float *)dx.p, (const float *)dy.p,
+                              (const int32_t *)di.p + b0, B, c->d_wb, w);
+            launch_mlp_grad(c->stream, B, w);
+            launch_adam(c->stream, c->d_wf, w.grad, c->d_m, c->d_v, c->d_wb, lr, 0.9f, 0.999f, 1e-8f,
+                        ++c->adam_t);
+        }
+        HIPCHK(c, hipGetLastError());
+        float ls = 0;
+        HIPCHK(c, hipMemcpyAsync(&ls, w.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (epoch_loss) epoch_loss[ep] = nb > 0 ? ls / (float)(nb * B) : 0.f;
+    }
+    return SCC_OK;
+}
+
+scc_status scc_phase_recolour(scc_ctx *c) {
+    if (!c) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    if (c->g.n == 0) return SCC_OK;
+    Launch l = launch_for(c);
+    unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
+    launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), 1, 0,
+                  nullptr);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SCC_OK;
+}
+
+scc_status scc_phase_features(scc_ctx *c, int32_t step, const int32_t *idx, int64_t count, float *X_out) {
+    if (!c || count < 0 || (count > 0 && (!idx || !X_out))) return SCC_E_INPUT;
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    if (c->g.n == 0 || count == 0) return SCC_OK;
+    for (int64_t i = 0; i < count; i++)
+        if (idx[i] < 0 || idx[i] >= c->g.n) return fail(c, SCC_E_INPUT, "vertex index out of range");
+    if (!c->d_feat) HIPCHK(c, hipMalloc(&c->d_feat, sizeof(float) * MLP_IN * (size_t)c->g.n));
+    c->cur_step = step;
+    CHK(do_features(c, false, c->d_feat));
+    DevBuf di, dx;
+    HIPCHK(c, hipMalloc(&di.p, sizeof(int32_t) * (size_t)count));
+    HIPCHK(c, hipMalloc(&dx.p, sizeof(float) * MLP_IN * (size_t)count));
+    HIPCHK(c, hipMemcpyAsync(di.p, idx, sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice, c->stream));
+    launch_gather_rows(mlp_launch(c, count * MLP_IN), c->d_feat, (const int32_t *)di.p, count, (float *)dx.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(X_out, dx.p, sizeof(float) * MLP_IN * (size_t)count, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SCC_OK;
 }
 

@@ -8,6 +8,17 @@ namespace gpuscc {
 
 struct Ctl;
 
+// pivot scorer: 12 features -> 32 -> 32 -> 1
+constexpr int MLP_IN = 12, MLP_HID = 32;
+constexpr int MLP_PARAMS = MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID + 1;
+
+struct MlpWork {  // per-batch training buffers
+    uint16_t *H1, *H2;          // BF16 activations [B][32]
+    float *dH1, *dH2, *ds, *Xb;  // gradients of the activations, of the logit; gathered features [B][12]
+    float *grad;                 // [MLP_PARAMS]
+    float *loss;                 // [1] sum of per-sample losses
+};
+
 struct Launch {
     hipStream_t stream;
     uint32_t grid;   // workgroups (persistent grid-stride)
@@ -32,6 +43,7 @@ struct ReachArgs {
     uint32_t dirs;
     uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
+    int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int instr;          // count work (expanded vertices, scanned arcs, visits)
 };
 
@@ -54,7 +66,7 @@ void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t 
                        int iter);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
-                   unsigned long long tag, int recolour, const uint16_t *score);
+                   unsigned long long tag, int recolour, int elect, const uint16_t *score);
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
@@ -69,6 +81,23 @@ void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, in
 void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n);
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl);
+
+// kernels_mlp.hip -- learned pivot scorer
+void launch_static_features(const Launch &l, const Graph &g, float4 *out);
+void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits);
+// features of every vertex into feat_out (n x 12, may be null) and, with params, scores
+void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat,
+                  const uint32_t *cnt, int slot_bits, float step_f, const uint16_t *params,
+                  uint16_t *score, float *feat_out);
+void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit);
+void launch_mlp_fwdbwd(const Launch &l, const float *X, const float *y, const int32_t *idx, int64_t N,
+                       const uint16_t *params, MlpWork &w);
+void launch_mlp_grad(hipStream_t s, int64_t N, MlpWork &w);
+void launch_adam(hipStream_t s, float *wf, const float *g, float *m, float *v, uint16_t *wb, float lr,
+                 float beta1, float beta2, float eps, int t);
+void launch_to_bf16(hipStream_t s, const float *w, uint16_t *wb);
+void launch_fill_f32(const Launch &l, float *p, int64_t count, float v);
+void launch_gather_rows(const Launch &l, const float *feat, const int32_t *idx, int64_t count, float *out);
 
 // kernels_reach.hip -- one level of a reachability chain; sched 0..4 = q t d l h
 void launch_reach(const Launch &l, int sched, const ReachArgs &a);

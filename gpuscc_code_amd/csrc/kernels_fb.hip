@@ -144,7 +144,7 @@ __global__ void k_trim_sweep(const int32_t *__restrict__ row, const int32_t *__r
 // ---------------------------------------------------------------------------
 __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          const unsigned long long *__restrict__ prev, unsigned long long *cur,
-                         int slot_bits, unsigned long long tag, int recolour,
+                         int slot_bits, unsigned long long tag, int recolour, int elect,
                          const uint16_t *__restrict__ score) {
     int64_t beg, end;
     my_range(n, beg, end);
@@ -172,7 +172,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                         word[v] = make_word(c, 0);
                     }
                 }
-                if (!elim) {
+                if (!elim && elect) {
                     cand = true;
                     sl = slot_of(c, slot_bits);
                     uint32_t sc = score ? (uint32_t)(0xFFFFu - score[v]) : 0u;
@@ -473,8 +473,9 @@ void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t 
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
-                   unsigned long long tag, int recolour, const uint16_t *score) {
-    hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, score);
+                   unsigned long long tag, int recolour, int elect, const uint16_t *score) {
+    hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, elect,
+                       score);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl) {

@@ -342,11 +342,87 @@ __global__ void k_reach_topo(ReachArgs a) {
 
 // ---------------------------------------------------------------------------
 // d: frontier queues, one lane per item, serial row scan (reach_d.cu:36-64);
-// the item space is the forward queue followed by the backward queue
+// the item space is the forward queue followed by the backward queue, dealt to
+// workgroups in contiguous chunks.  After its chunk, a workgroup keeps
+// expanding the vertices it discovered itself, in LDS, for up to
+// local_rounds rounds: reachability is monotone, so reaching a vertex before
+// its BFS level changes only the number of levels.  What is left (and any
+// staging overflow) goes to the next level's queue.
 // ---------------------------------------------------------------------------
+struct LocalQ {
+    int32_t v[2 * SQ_CAP];
+};
+
+// expands item v of direction d serially; new visits are staged
+__device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, bool &found,
+                                     unsigned long long &cx, unsigned long long &ce,
+                                     unsigned long long &cv) {
+    uint32_t c = colour_of(a.word[v]), bit = dbit(d);
+    const int32_t *row = drow(a, d), *col = dcol(a, d);
+    int32_t e0 = row[v], e1 = row[v + 1];
+    if ((uint32_t)(e1 - e0) >= a.huge_thr) {
+        push_huge(a, d, v);
+        found = true;
+        return;
+    }
+    cx++;
+    ce += (unsigned long long)(e1 - e0);
+    for (int32_t e = e0; e < e1; e++) {
+        int32_t u = col[e];
+        if (try_claim(a.word, u, c, bit)) {
+            found = true;
+            cv++;
+            st_push(st, d, u, a);
+        }
+    }
+}
+
+constexpr int32_t LOCAL_MAX_DEG = 64;       // longer rows leave the local rounds for the next level
+constexpr uint32_t LOCAL_ITEMS_PER_WG = 8;  // local rounds only when a level's frontier is this thin
+constexpr int LOCAL_FIRST_LEVEL = 4;        // ... and the reach is this many levels deep
+
+// Workgroup-local rounds on the visits staged by this workgroup: each round
+// moves the staged vertices into lq and expands them one per lane; rows longer
+// than LOCAL_MAX_DEG go to the next level's queue instead.  All work-items call this.
+__device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, Stage &st, LocalQ &lq,
+                                    bool &found, unsigned long long &cx, unsigned long long &ce,
+                                    unsigned long long &cv) {
+    // a thick frontier keeps every workgroup busy without them; a thin one
+    // (high-diameter graphs) would leave most workgroups idle per level.  The
+    // first levels of a reach are thin on every graph, so they never qualify.
+    if (a.iter < LOCAL_FIRST_LEVEL || level_items > LOCAL_ITEMS_PER_WG * gridDim.x) return;
+    for (int r = 0; r < a.local_rounds; r++) {
+        __syncthreads();
+        uint32_t c0 = st.cnt[0] < SQ_CAP ? st.cnt[0] : SQ_CAP;
+        uint32_t c1 = st.cnt[1] < SQ_CAP ? st.cnt[1] : SQ_CAP;
+        if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
+        for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x) lq.v[k] = st.buf[0][k];
+        for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x) lq.v[SQ_CAP + k] = st.buf[1][k];
+        __syncthreads();
+        if (threadIdx.x < 2) st.cnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t k0 = 0; k0 < c0 + c1; k0 += blockDim.x) {
+            uint32_t k = k0 + threadIdx.x;
+            if (k < c0 + c1) {
+                int d = k < c0 ? 0 : 1;
+                int32_t v = d == 0 ? lq.v[k] : lq.v[SQ_CAP + (k - c0)];
+                const int32_t *row = drow(a, d);
+                if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
+                    uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
+                    a.outq[d][g] = v;
+                } else {
+                    expand_serial(a, st, d, v, found, cx, ce, cv);
+                }
+            }
+            st_maybe_flush(st, a);
+        }
+    }
+}
+
 __global__ void k_reach_queue(ReachArgs a) {
     __shared__ Items it;
     __shared__ Stage st;
+    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
     if (!level_begin(a, true)) return;
     st_init(st);
@@ -358,34 +434,19 @@ __global__ void k_reach_queue(ReachArgs a) {
     uint32_t qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
     uint32_t qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
     uint32_t tot = qs0 + qs1;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += stride) {
+    uint32_t chunk = (tot + gridDim.x - 1) / gridDim.x;
+    uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < tot ? b0 + chunk : tot;
+    for (uint32_t base = b0; base < b1; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
-        if (i < tot) {
+        if (i < b1) {
             int d = i < qs0 ? 0 : 1;
             int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
-            uint32_t c = colour_of(a.word[v]), bit = dbit(d);
-            const int32_t *row = drow(a, d), *col = dcol(a, d);
-            int32_t e0 = row[v], e1 = row[v + 1];
-            if ((uint32_t)(e1 - e0) >= a.huge_thr) {
-                push_huge(a, d, v);
-                found = true;
-                e1 = e0;
-            } else {
-                cx++;
-                ce += (unsigned long long)(e1 - e0);
-            }
-            for (int32_t e = e0; e < e1; e++) {
-                int32_t u = col[e];
-                if (try_claim(a.word, u, c, bit)) {
-                    found = true;
-                    cv++;
-                    st_push(st, d, u, a);
-                }
-            }
+            expand_serial(a, st, d, v, found, cx, ce, cv);
         }
-        st_flush(st, a);
+        st_maybe_flush(st, a);
     }
+    local_rounds(a, tot, st, lq, found, cx, ce, cv);
+    st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
     if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }
@@ -394,12 +455,13 @@ __global__ void k_reach_queue(ReachArgs a) {
 __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t tot, Items &it,
                                       Stage &st, bool &found, unsigned long long &cx,
                                       unsigned long long &ce, unsigned long long &cv) {
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += stride) {
+    uint32_t chunk = (tot + gridDim.x - 1) / gridDim.x;
+    uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < tot ? b0 + chunk : tot;
+    for (uint32_t base = b0; base < b1; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
         uint32_t deg = 0, cd = 0;
         int32_t start = 0;
-        if (i < tot) {
+        if (i < b1) {
             int d = i < qs0 ? 0 : 1;
             int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
             const int32_t *row = drow(a, d);
@@ -422,7 +484,7 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
         it.cd[threadIdx.x] = cd;
         __syncthreads();
         if (total > 0) expand_items<true>(a, it, blockDim.x, total, &st, found, cv);
-        st_flush(st, a);
+        st_maybe_flush(st, a);
     }
 }
 
@@ -432,6 +494,7 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
 __global__ void k_reach_lb(ReachArgs a) {
     __shared__ Items it;
     __shared__ Stage st;
+    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
     if (!level_begin(a, true)) return;
     st_init(st);
@@ -443,6 +506,8 @@ __global__ void k_reach_lb(ReachArgs a) {
     huge_phase<true>(a, it, &st, found, cx, ce, cv);
     st_flush(st, a);
     queue_balanced(a, qs0, qs0 + qs1, it, st, found, cx, ce, cv);
+    local_rounds(a, qs0 + qs1, st, lq, found, cx, ce, cv);
+    st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
     if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }
@@ -457,6 +522,7 @@ __global__ void k_reach_lb(ReachArgs a) {
 __global__ void k_reach_hybrid(ReachArgs a) {
     __shared__ Items it;
     __shared__ Stage st;
+    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
     if (!level_begin(a, true)) return;
     st_init(st);
@@ -509,8 +575,9 @@ __global__ void k_reach_hybrid(ReachArgs a) {
             }
             st_maybe_flush(st, a);
         }
-        st_flush(st, a);
     }
+    local_rounds(a, (pull0 || pull1) ? 0xFFFFFFFFu : qs0 + qs1, st, lq, found, cx, ce, cv);
+    st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
     if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }

@@ -22,6 +22,7 @@ VARIANTS = {"scc": 0, "scc_w": 1, "scc_2_w": 2}
 PIVOT_PLAIN, PIVOT_LEARNED = 0, 1
 
 ST_B, ST_F, ST_E, ST_T, ST_P, ST_BX, ST_FX = 1, 2, 4, 8, 16, 32, 64
+MLP_PARAMS, MLP_FEATURES, MLP_HIDDEN = 1505, 12, 32
 COLOUR_MASK = 0x00FFFFFF
 
 
@@ -35,6 +36,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     grid_blocks: int = 0
     sequential_reach: int = 0
     huge_row: int = 0
+    local_rounds: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -45,6 +47,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.grid_blocks = self.grid_blocks
         o.sequential_reach = self.sequential_reach
         o.huge_row = self.huge_row
+        o.local_rounds = self.local_rounds
         return o
 
 
@@ -222,6 +225,49 @@ class Context:
 
     def count_elimmed(self):
         return self._i64call(lib().scc_phase_count_elimmed)
+
+    def recolour(self):
+        self._chk(lib().scc_phase_recolour(self._h))
+
+    def features(self, idx, step=0):
+        idx = np.ascontiguousarray(idx, np.int32)
+        X = np.empty((max(len(idx), 1), MLP_FEATURES), np.float32)
+        self._chk(lib().scc_phase_features(self._h, int(step), _p(idx, _lib._i32p), len(idx),
+                                           _p(X, _lib._fp)))
+        return X[:len(idx)]
+
+    # learned pivot scorer
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._chk(lib().scc_mlp_set_params(self._h, _p(p, _lib._fp), len(p)))
+
+    def mlp_get(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        self._chk(lib().scc_mlp_get_params(self._h, _p(p, _lib._fp), MLP_PARAMS))
+        return p
+
+    def mlp_infer(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty(max(len(X), 1), np.float32)
+        self._chk(lib().scc_mlp_infer(self._h, _p(X, _lib._fp), len(X), _p(out, _lib._fp)))
+        return out[:len(X)]
+
+    def mlp_gradient(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._chk(lib().scc_mlp_gradient(self._h, _p(X, _lib._fp), _p(y, _lib._fp), len(X),
+                                         _p(g, _lib._fp), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, X, y, epochs=10, batch=8192, lr=3e-3, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        losses = np.empty(max(epochs, 1), np.float32)
+        self._chk(lib().scc_mlp_train(self._h, _p(X, _lib._fp), _p(y, _lib._fp), len(X), epochs,
+                                      batch, lr, seed, _p(losses, _lib._fp)))
+        return losses[:epochs]
 
     # measurement: flags 1 = device work counters, 2 = HIP-event kernel timing
     def profile(self, flags):

@@ -67,7 +67,9 @@ typedef struct {
                                   0: both directions advance in the same level kernels */
     int32_t huge_row;          /* rows of at least this many arcs are split over all workgroups
                                   (deferred one level); default 1024 for q/d, 4096 for t/l/h */
-    int32_t reserved[1];
+    int32_t local_rounds;      /* d, l, h: after its share of a level, a workgroup keeps expanding its own
+                                  new visits in LDS for up to this many rounds (0 = default 64,
+                                  -1 = off) */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -158,6 +160,36 @@ scc_status scc_phase_trim2(scc_ctx *ctx, int64_t *pairs);
 scc_status scc_phase_wcc(scc_ctx *ctx, int64_t *iterations, int64_t *pivots);
 /* countElimmed (scc.cu:42-51). */
 scc_status scc_phase_count_elimmed(scc_ctx *ctx, int64_t *count);
+/* The colour split of update without the election (scc_routines.cu:199-210). */
+scc_status scc_phase_recolour(scc_ctx *ctx);
+/* Scorer features (12 floats per vertex, layout in kernels_mlp.hip) of the
+ * vertices idx[0..count) for the current words; step is the FB step index. */
+scc_status scc_phase_features(scc_ctx *ctx, int32_t step, const int32_t *idx, int64_t count,
+                              float *X_out);
+
+/* ---- learned pivot scorer --------------------------------------------------
+ * No reference counterpart (the reference's pivot is whichever vertex wins an
+ * atomicCAS race, scc_routines.cu:214-221).  An MLP 12 -> 32 -> 32 -> 1 with
+ * BF16 weights and activations and FP32 accumulation scores every live
+ * vertex before each election; with SCC_PIVOT_LEARNED the highest score of a
+ * colour slot wins (ties: lowest id).  The partition does not depend on the
+ * scores.  Parameters: 1505 floats, W1[32][12] b1[32] W2[32][32] b2[32]
+ * w3[32] b3, held in FP32 (master) and BF16 (used by every kernel). */
+#define SCC_MLP_PARAMS 1505
+#define SCC_MLP_FEATURES 12
+scc_status scc_mlp_set_params(scc_ctx *ctx, const float *params, int32_t count);
+scc_status scc_mlp_get_params(scc_ctx *ctx, float *params, int32_t count);
+/* logits of N feature rows (host X[N][12] -> host logits[N]) */
+scc_status scc_mlp_infer(scc_ctx *ctx, const float *X, int64_t N, float *logits);
+/* gradient of the mean binary cross-entropy of sigmoid(logit) against y over
+ * the N rows (host in, host grad[1505] out) */
+scc_status scc_mlp_gradient(scc_ctx *ctx, const float *X, const float *y, int64_t N, float *grad,
+                            float *loss);
+/* Adam training on the GPU: `epochs` passes over the N rows in shuffled
+ * mini-batches of `batch`; epoch_loss (may be NULL) receives the mean loss of
+ * each epoch.  Starts from the current parameters. */
+scc_status scc_mlp_train(scc_ctx *ctx, const float *X, const float *y, int64_t N, int32_t epochs,
+                         int32_t batch, float lr, uint64_t seed, float *epoch_loss);
 
 /* ---- measurement ----------------------------------------------------------
  * Work counters and kernel timing of the reachability levels, for the roofline
