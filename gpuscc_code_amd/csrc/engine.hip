@@ -35,6 +35,8 @@ struct scc_ctx {
     int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
     int32_t *d_seedB = nullptr;  // backward seeds of a phase-level reach
     int32_t *d_huge[4] = {nullptr, nullptr, nullptr, nullptr};  // deferred huge rows [dir * 2 + parity]
+    uint32_t *d_stamp = nullptr;  // trim levels: last level that checked each vertex
+    uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
     int prev_tab = 0;        // table holding the last election's winners
@@ -169,16 +171,38 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
 }
 
 scc_status do_trim(scc_ctx *c, int64_t *iters) {
-    Launch l = launch_for(c);
+    Launch l = fixed_launch(c);
+    ReachArgs a{};
+    a.row = c->g.row;
+    a.col = c->g.col;
+    a.rowT = c->g.rowT;
+    a.colT = c->g.colT;
+    a.word = c->d_word;
+    a.n = c->g.n;
+    a.ctl = c->d_ctl;
+    a.root = c->d_root;
+    a.stamp = c->d_stamp;
+    a.dirs = 1;
+    launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
     return run_chain(
         c,
         [&](int it, hipEvent_t e0, hipEvent_t e1) {
+            if (++c->trim_tag == 0) {  // tag wrap: forget every stamp
+                launch_fill_i32(launch_for(c), (int32_t *)c->d_stamp, c->g.n, 0);
+                c->trim_tag = 1;
+            }
+            a.iter = it;
+            a.tag = c->trim_tag;
+            a.inq[0] = c->d_q[(it - 1) & 1];
+            a.outq[0] = c->d_q[it & 1];
+            a.inq[1] = a.inq[0];
+            a.outq[1] = c->d_q[2 + (it & 1)];
             Launch lt = l;
             lt.ev_start = e0;
             lt.ev_stop = e1;
-            launch_trim_sweep(lt, c->g, c->d_word, c->d_root, c->d_ctl, it);
+            launch_trim_level(lt, a);
         },
-        c->guess[2], 0, iters, 1);
+        c->guess[2], 2, iters, 1);
 }
 
 // One reachability chain for the directions in `dirs` (1 = F, 2 = B, 3 = both).
@@ -317,7 +341,7 @@ void free_graph(scc_ctx *c) {
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
                         &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
                         &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
-                        &c->d_huge[3]};
+                        &c->d_huge[3], (int32_t **)&c->d_stamp};
     for (auto b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -505,6 +529,9 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_tmp, vb));
     for (auto &q : c->d_q) HIPCHK(c, hipMalloc(&q, vb));
     for (auto &q : c->d_huge) HIPCHK(c, hipMalloc(&q, vb));
+    HIPCHK(c, hipMalloc(&c->d_stamp, vb));
+    HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
+    c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
     HIPCHK(c, hipMalloc(&c->d_piv, vb));
     // election table: about n slots (at least 2^10, at most 2^22), or the option

@@ -45,6 +45,10 @@ struct ReachArgs {
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int instr;          // count work (expanded vertices, scanned arcs, visits)
+    // trim levels
+    int32_t *root;      // trimmed vertices become their own SCC
+    uint32_t *stamp;    // per-vertex tag of the last trim level that checked it
+    uint32_t tag;       // this level's tag (unique per launch)
 };
 
 struct Graph {  // device CSR + CSR^T
@@ -62,8 +66,6 @@ void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);
-void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl,
-                       int iter);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score);
@@ -98,6 +100,11 @@ void launch_adam(hipStream_t s, float *wf, const float *g, float *m, float *v, u
 void launch_to_bf16(hipStream_t s, const float *w, uint16_t *wb);
 void launch_fill_f32(const Launch &l, float *p, int64_t count, float v);
 void launch_gather_rows(const Launch &l, const float *feat, const int32_t *idx, int64_t count, float *out);
+
+// kernels_reach.hip -- data-driven trim: level 0 sweeps every live vertex and
+// queues what it trims (outq[0]); level i >= 1 re-checks only the neighbours
+// of the vertices level i-1 trimmed (inq[0])
+void launch_trim_level(const Launch &l, const ReachArgs &a);
 
 // kernels_reach.hip -- one level of a reachability chain; sched 0..4 = q t d l h
 void launch_reach(const Launch &l, int sched, const ReachArgs &a);

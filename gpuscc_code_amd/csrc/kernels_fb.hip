@@ -98,44 +98,6 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
 }
 
 // ---------------------------------------------------------------------------
-// trim within colours, one sweep: trim_kernel (scc_routines.cu:336-394).
-// Live v with no live same-colour out-neighbour other than itself, or no such
-// in-neighbour, becomes state 28 (P|T|E) and its own SCC.
-// ---------------------------------------------------------------------------
-__device__ inline bool has_live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                                   const uint32_t *word, int32_t v, uint32_t c) {
-    int32_t e0 = row[v], e1 = row[v + 1];
-    for (int32_t e = e0; e < e1; e++) {
-        int32_t u = col[e];
-        if (u == v) continue;
-        uint32_t wu = word[u];
-        if (!(wu & ST_E) && colour_of(wu) == c) return true;
-    }
-    return false;
-}
-
-__global__ void k_trim_sweep(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                             const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
-                             uint32_t *word, int32_t *root, int32_t n, Ctl *ctl, int iter) {
-    if (chain_exit(ctl, iter)) return;
-    int64_t beg, end;
-    my_range(n, beg, end);
-    bool found = false;
-    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
-        int32_t v = (int32_t)vv;
-        uint32_t w = word[v];
-        if (w & ST_E) continue;
-        uint32_t c = colour_of(w);
-        if (!has_live_nb(row, col, word, v, c) || !has_live_nb(rowT, colT, word, v, c)) {
-            word[v] = make_word(c, 28);
-            root[v] = v;
-            found = true;
-        }
-    }
-    chain_progress(ctl, iter, found);
-}
-
-// ---------------------------------------------------------------------------
 // colour update + election: update_kernel (scc_routines.cu:187-224).
 // recolour: live v with F and B joins the SCC of the pivot elected for its
 // colour last time (E set, root = that pivot); other live v take colour
@@ -461,15 +423,6 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
 }
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what);
-}
-void launch_trim_sweep(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl,
-                       int iter) {
-    if (l.ev_start)
-        hipExtLaunchKernelGGL(k_trim_sweep, dim3(l.grid), dim3(l.block), 0, l.stream, l.ev_start,
-                              l.ev_stop, 0, g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl, iter);
-    else
-        hipLaunchKernelGGL(k_trim_sweep, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n, ctl,
-                           iter);
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,

@@ -582,6 +582,103 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
 }
 
+// ---------------------------------------------------------------------------
+// data-driven trim within colours (trim_kernel, scc_routines.cu:336-394, run to
+// its fixpoint by trim(), :616-643): the fixpoint is unique, so instead of
+// re-sweeping every live vertex per iteration, level 0 sweeps once and every
+// later level re-checks only the live same-colour neighbours of the vertices
+// trimmed in the level before (each at most once per level, by stamp).
+// ---------------------------------------------------------------------------
+__device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                               const uint32_t *word, int32_t v, uint32_t c) {
+    for (int32_t e = row[v]; e < row[v + 1]; e++) {
+        int32_t u = col[e];
+        if (u == v) continue;
+        uint32_t wu = word[u];
+        if (!(wu & ST_E) && colour_of(wu) == c) return true;
+    }
+    return false;
+}
+
+// trims v (colour c) when it has no live same-colour in- or out-neighbour
+__device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c) {
+    if (live_nb(a.row, a.col, a.word, v, c) && live_nb(a.rowT, a.colT, a.word, v, c)) return false;
+    a.word[v] = make_word(c, 28);  // P|T|E (scc_routines.cu:381)
+    a.root[v] = v;
+    return true;
+}
+
+__global__ void k_trim_level(ReachArgs a) {
+    __shared__ Items it;
+    __shared__ Stage st;
+    if (!level_begin(a, true)) return;
+    st_init(st);
+    __syncthreads();
+    bool found = false;
+    if (a.iter == 0) {
+        int64_t beg, end;
+        my_range(a.n, beg, end);
+        for (int64_t base = beg; base < end; base += blockDim.x) {
+            int64_t vv = base + threadIdx.x;
+            if (vv < end) {
+                int32_t v = (int32_t)vv;
+                uint32_t w = a.word[v];
+                if (!(w & ST_E) && trim_check(a, v, colour_of(w))) {
+                    found = true;
+                    st_push(st, 0, v, a);
+                }
+            }
+            st_maybe_flush(st, a);
+        }
+    } else {
+        uint32_t qs = a.ctl->qsize[0][a.iter & 3];
+        uint32_t chunk = (qs + gridDim.x - 1) / gridDim.x;
+        uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < qs ? b0 + chunk : qs;
+        for (uint32_t base = b0; base < b1; base += blockDim.x) {
+            // each trimmed vertex contributes its out-row and its in-row
+            uint32_t i = base + threadIdx.x;
+            uint32_t deg[2] = {0, 0}, c = 0;
+            int32_t start[2] = {0, 0};
+            if (i < b1) {
+                int32_t x = a.inq[0][i];
+                c = colour_of(a.word[x]);
+                start[0] = a.row[x];
+                deg[0] = (uint32_t)(a.row[x + 1] - start[0]);
+                start[1] = a.rowT[x];
+                deg[1] = (uint32_t)(a.rowT[x + 1] - start[1]);
+            }
+            uint32_t total;
+            uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total);
+            it.off[2 * threadIdx.x] = off;
+            it.off[2 * threadIdx.x + 1] = off + deg[0];
+            it.start[2 * threadIdx.x] = start[0];
+            it.start[2 * threadIdx.x + 1] = start[1];
+            it.cd[2 * threadIdx.x] = c;
+            it.cd[2 * threadIdx.x + 1] = c | 0x80000000u;
+            __syncthreads();
+            for (uint32_t pb = 0; pb < total; pb += blockDim.x) {  // block-uniform
+                uint32_t e = pb + threadIdx.x;
+                if (e < total) {
+                    uint32_t k = find_item(it.off, 2 * blockDim.x, e);
+                    uint32_t cd = it.cd[k];
+                    const int32_t *col = (cd >> 31) ? a.colT : a.col;
+                    int32_t u = col[it.start[k] + (int32_t)(e - it.off[k])];
+                    uint32_t cc = cd & 0x7FFFFFFFu;
+                    uint32_t wu = a.word[u];
+                    if (!(wu & ST_E) && colour_of(wu) == cc &&
+                        atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
+                        found = true;
+                        st_push(st, 0, u, a);
+                    }
+                }
+                st_maybe_flush(st, a);
+            }
+        }
+    }
+    st_flush(st, a);
+    chain_progress(a.ctl, a.iter, found);
+}
+
 __global__ void k_clear_bits(uint32_t *word, int32_t n, uint32_t bits) {
     int64_t beg, end;
     my_range(n, beg, end);
@@ -613,6 +710,8 @@ void launch_reach(const Launch &l, int sched, const ReachArgs &a) {
     default: launch_reach_kernel(k_reach_hybrid, l, a); break;
     }
 }
+
+void launch_trim_level(const Launch &l, const ReachArgs &a) { launch_reach_kernel(k_trim_level, l, a); }
 
 void launch_clear_bits(const Launch &l, uint32_t *word, int32_t n, uint32_t bits) {
     hipLaunchKernelGGL(k_clear_bits, dim3(l.grid), dim3(l.block), 0, l.stream, word, n, bits);
