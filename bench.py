@@ -49,6 +49,17 @@ def algorithmic_bytes(sched, n, prof):
     return S * 4 * n + X * 12 + E * 8 + V * 8
 
 
+def traffic_for(args):
+    """HBM bytes per launch of the level kernel measured by rocprofv3 counters
+    (profiles/), when they were measured for this exact configuration."""
+    if not args.traffic or not os.path.exists(args.traffic):
+        return None
+    t = json.load(open(args.traffic))
+    if (args.workload, args.schedule, args.scale, args.edge_factor, args.seed) != ("rmat", "h", 22, 8, 2):
+        return None
+    return t["bytes_per_launch"]
+
+
 def cpu_tarjan_baseline(g, reps=3):
     import oracle
     try:
@@ -87,6 +98,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sequential", action="store_true",
                     help="forward reach to its fixpoint, then backward (the reference's order)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run the calibration copy kernel over MIB MiB (for traffic counters)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_reach_hybrid_rmat22_traffic.json"),
+                    metavar="JSON",
+                    help="measured HBM bytes per launch of the level kernel (from rocprofv3 counters)")
     ap.add_argument("--pull-threshold", type=int, default=0,
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
     args = ap.parse_args()
@@ -120,6 +136,9 @@ def main():
     if args.workload == "batch":
         ctx.set_batch(g.graph_offsets)
         num_graphs = len(g.graph_offsets) - 1
+
+    if args.calibrate_copy:
+        ctx.profile_copy(args.calibrate_copy << 20)
 
     # untimed counting run: algorithmic work of the reachability levels
     ctx.profile(1)
@@ -157,7 +176,8 @@ def main():
     roofline = {
         "bound": "hbm", "kernel": KERNEL_OF[args.schedule],
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_for(args),
         "bytes_per_launch": int(per_run_bytes / max(launches, 1)),
         "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 2),
         "launches_per_step": launches, "kernel_ms_per_step": round(kernel_ms, 3),

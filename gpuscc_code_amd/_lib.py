@@ -101,6 +101,7 @@ SIGNATURES = {
                             ctypes.c_float, ctypes.c_uint64, _fp]),
     "scc_profile_set": (_st, [_vp, ctypes.c_int]),
     "scc_profile_get": (_st, [_vp, ctypes.POINTER(SccProfile), ctypes.c_int]),
+    "scc_profile_copy": (_st, [_vp, ctypes.c_int64]),
     "scc_load_file": (_st, [ctypes.c_char_p, _gp]),
     "scc_csr_from_arcs": (_st, [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, ctypes.c_int, _gp]),
     "scc_gen_rmat": (_st, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

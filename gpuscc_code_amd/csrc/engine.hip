@@ -864,6 +864,20 @@ scc_status scc_phase_features(scc_ctx *c, int32_t step, const int32_t *idx, int6
     return SCC_OK;
 }
 
+scc_status scc_profile_copy(scc_ctx *c, int64_t bytes) {
+    if (!c || bytes <= 0 || (bytes % 16) != 0) return SCC_E_INPUT;
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf src, dst;
+    HIPCHK(c, hipMalloc(&src.p, (size_t)bytes));
+    HIPCHK(c, hipMalloc(&dst.p, (size_t)bytes));
+    HIPCHK(c, hipMemsetAsync(src.p, 1, (size_t)bytes, c->stream));
+    Launch l{c->stream, (uint32_t)c->cu_count * 8u, 256};
+    launch_stream_copy(l, src.p, dst.p, bytes);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SCC_OK;
+}
+
 scc_status scc_labels_device(scc_ctx *c, const int32_t **dptr) {
     if (!c || !dptr) return SCC_E_INPUT;
     if (!c->labels_valid) return fail(c, SCC_E_STATE, "no labels yet");

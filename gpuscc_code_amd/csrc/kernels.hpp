@@ -64,6 +64,7 @@ void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, cons
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
+void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t bytes);
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,

@@ -50,6 +50,13 @@ __global__ void k_batch_check(const int32_t *row, const int32_t *col, int32_t n,
     if (threadIdx.x == 0 && bad) atomicAdd(&ctl->count, bad);
 }
 
+// calibration copy of known traffic: 16-byte loads and stores
+__global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int64_t count, unsigned long long val) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -414,6 +421,9 @@ void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, cons
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
+}
+void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t bytes) {
+    hipLaunchKernelGGL(k_stream_copy, GRID(l), (const uint4 *)src, (uint4 *)dst, bytes / 16);
 }
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v) {
     hipLaunchKernelGGL(k_fill_u64, GRID(l), p, count, v);

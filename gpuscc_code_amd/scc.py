@@ -273,6 +273,10 @@ class Context:
     def profile(self, flags):
         self._chk(lib().scc_profile_set(self._h, int(flags)))
 
+    def profile_copy(self, nbytes):
+        """One run of the calibration copy kernel (k_stream_copy) over nbytes."""
+        self._chk(lib().scc_profile_copy(self._h, int(nbytes)))
+
     def profile_read(self, reset=True):
         p = _lib.SccProfile()
         self._chk(lib().scc_profile_get(self._h, ctypes.byref(p), 1 if reset else 0))

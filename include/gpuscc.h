@@ -211,6 +211,10 @@ typedef struct {
 } scc_profile_data;
 scc_status scc_profile_set(scc_ctx *ctx, int flags);
 scc_status scc_profile_get(scc_ctx *ctx, scc_profile_data *out, int reset);
+/* Runs the calibration kernel k_stream_copy once: reads `bytes` bytes of
+ * device memory and writes the same number (for scaling hardware traffic
+ * counters against a kernel of known traffic). */
+scc_status scc_profile_copy(scc_ctx *ctx, int64_t bytes);
 
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
