@@ -103,6 +103,10 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_reach_hybrid_rmat22_traffic.json"),
                     metavar="JSON",
                     help="measured HBM bytes per launch of the level kernel (from rocprofv3 counters)")
+    ap.add_argument("--grid-blocks", type=int, default=0,
+                    help="workgroups of the persistent kernels (0 = 4 per CU)")
+    ap.add_argument("--pull-arc-div", type=int, default=0,
+                    help="hybrid: frontier arcs >= m / DIV also pull (0 = 14, -1 = off)")
     ap.add_argument("--pull-threshold", type=int, default=0,
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
     args = ap.parse_args()
@@ -130,7 +134,9 @@ def main():
               "planted cycles Geometric(0.05) over half the vertices, seed 4, packed CSR")
     log(f"generated {args.workload}: n={g.n} m={g.m} in {time.time() - t0:.1f}s")
     ctx = scc.Context(0, scc.alg_opt(sequential_reach=1 if args.sequential else 0,
-                                     hybrid_threshold=args.pull_threshold))
+                                     hybrid_threshold=args.pull_threshold,
+                                     hybrid_arc_div=args.pull_arc_div,
+                                     grid_blocks=args.grid_blocks))
     ctx.upload(g)
     num_graphs = 1
     if args.workload == "batch":

@@ -26,7 +26,8 @@ class SccOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("hybrid_policy", ctypes.c_int32),
                 ("hybrid_threshold", ctypes.c_int32), ("slot_bits", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("sequential_reach", ctypes.c_int32),
-                ("huge_row", ctypes.c_int32), ("local_rounds", ctypes.c_int32)]
+                ("huge_row", ctypes.c_int32), ("local_rounds", ctypes.c_int32),
+                ("hybrid_arc_div", ctypes.c_int32)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",
@@ -72,6 +73,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u16p = ctypes.POINTER(ctypes.c_uint16)
 _fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
 _gp = ctypes.POINTER(SccHostGraph)
 _st = ctypes.c_int
 SIGNATURES = {
@@ -102,6 +104,7 @@ SIGNATURES = {
     "scc_profile_set": (_st, [_vp, ctypes.c_int]),
     "scc_profile_get": (_st, [_vp, ctypes.POINTER(SccProfile), ctypes.c_int]),
     "scc_profile_copy": (_st, [_vp, ctypes.c_int64]),
+    "scc_profile_levels": (_st, [_vp, _dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
     "scc_load_file": (_st, [ctypes.c_char_p, _gp]),
     "scc_csr_from_arcs": (_st, [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, ctypes.c_int, _gp]),
     "scc_gen_rmat": (_st, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

@@ -36,6 +36,7 @@ struct scc_ctx {
     int32_t *d_seedB = nullptr;  // backward seeds of a phase-level reach
     int32_t *d_huge[4] = {nullptr, nullptr, nullptr, nullptr};  // deferred huge rows [dir * 2 + parity]
     uint32_t *d_stamp = nullptr;  // trim levels: last level that checked each vertex
+    uint8_t *d_dcls = nullptr;    // degree classes: out-degrees [0, n), in-degrees [n, 2n)
     uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
@@ -63,6 +64,7 @@ struct scc_ctx {
     int prof_flags = 0;
     scc_profile_data prof{};
     std::vector<hipEvent_t> evs;    // flag 2: one start/stop pair per level launch
+    std::vector<double> level_log;  // flag 4: SCC_LEVEL_FIELDS per level
 };
 
 namespace {
@@ -136,9 +138,17 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
                      int kind = 2) {
     launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
     int64_t launched = 0;
-    int L = std::max(1, std::min(guess, 64));
+    const bool logged = (c->prof_flags & 4) && kind < 2;
+    int L = logged ? 1 : std::max(1, std::min(guess, 64));
     const int64_t limit = (int64_t)c->g.n + 4;
     const bool timed = (c->prof_flags & 2) && kind < 2;
+    unsigned long long last[3] = {0, 0, 0};
+    if (logged) {  // the counters are cumulative over this chain
+        CHK(read_ctl(c));
+        last[0] = c->h_ctl->st_expanded;
+        last[1] = c->h_ctl->st_edges;
+        last[2] = c->h_ctl->st_visited;
+    }
     for (;;) {
         if (timed) CHK(ensure_events(c, 2 * (size_t)L));
         for (int i = 0; i < L; i++) {
@@ -159,18 +169,35 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
         if (kind == 0) c->prof.reach_launches += L;
         if (kind == 1) c->prof.trim_launches += L;
         uint32_t ran = c->h_ctl->ran;
+        if (logged && (int64_t)ran >= launched) {
+            const Ctl &h = *c->h_ctl;
+            const int i = (int)((launched - 1) & 3);
+            float t = 0;
+            HIPCHK(c, hipEventElapsedTime(&t, c->evs[0], c->evs[1]));
+            double row[SCC_LEVEL_FIELDS] = {(double)kind, (double)(launched - 1), (double)t,
+                                            (double)h.qsize[0][i], (double)h.qsize[1][i],
+                                            (double)h.qarcs[0][i], (double)h.qarcs[1][i],
+                                            (double)(kind == 0 ? h.pull_mask : 0),
+                                            (double)(h.st_expanded - last[0]), (double)(h.st_edges - last[1]),
+                                            (double)(h.st_visited - last[2])};
+            last[0] = h.st_expanded;
+            last[1] = h.st_edges;
+            last[2] = h.st_visited;
+            c->level_log.insert(c->level_log.end(), row, row + SCC_LEVEL_FIELDS);
+        }
         if ((int64_t)ran < launched || c->h_ctl->changed[(launched - 1) & 3] == 0) {
             *levels = ran;
             break;
         }
         if (launched > limit) return fail(c, SCC_E_STATE, "level chain did not converge");
-        L = std::min(L * 2, 1024);
+        L = logged ? 1 : std::min(L * 2, 1024);
     }
     guess = (int)std::max<int64_t>(1, *levels);
     return SCC_OK;
 }
 
-scc_status do_trim(scc_ctx *c, int64_t *iters) {
+// fresh: the words are as k_init left them (all live, arcs never leave a colour)
+scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false) {
     Launch l = fixed_launch(c);
     ReachArgs a{};
     a.row = c->g.row;
@@ -182,6 +209,7 @@ scc_status do_trim(scc_ctx *c, int64_t *iters) {
     a.ctl = c->d_ctl;
     a.root = c->d_root;
     a.stamp = c->d_stamp;
+    a.fresh = fresh ? 1 : 0;
     a.dirs = 1;
     launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
     return run_chain(
@@ -227,6 +255,10 @@ scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, 
     a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
                                       : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
     a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
+    const int div = c->opts.hybrid_arc_div > 0 ? c->opts.hybrid_arc_div : 14;
+    a.pull_arcs = c->opts.hybrid_arc_div < 0 ? ~0ull : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
+    a.dcls[0] = c->d_dcls;
+    a.dcls[1] = c->d_dcls + c->g.n;
     const bool queues = sched >= SCC_SCHED_D;
     scc_status s = run_chain(
         c,
@@ -341,7 +373,7 @@ void free_graph(scc_ctx *c) {
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
                         &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
                         &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
-                        &c->d_huge[3], (int32_t **)&c->d_stamp};
+                        &c->d_huge[3], (int32_t **)&c->d_stamp, (int32_t **)&c->d_dcls};
     for (auto b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -530,6 +562,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     for (auto &q : c->d_q) HIPCHK(c, hipMalloc(&q, vb));
     for (auto &q : c->d_huge) HIPCHK(c, hipMalloc(&q, vb));
     HIPCHK(c, hipMalloc(&c->d_stamp, vb));
+    HIPCHK(c, hipMalloc(&c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -555,6 +588,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     Launch l = launch_for(c);
     launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
+    launch_degree_class(l, c->d_row, n, c->d_dcls);
+    launch_degree_class(l, c->d_rowT, n, c->d_dcls + n);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());
@@ -590,7 +625,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();
-        CHK(do_trim(c, &it));  // trim1 (scc.cu:232)
+        CHK(do_trim(c, &it, true));  // trim1 (scc.cu:232)
         st.trim_iters += it;
         st.ms_trim += ms_since(tp);
         tp = std::chrono::steady_clock::now();
@@ -679,7 +714,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
 }
 
 scc_status scc_profile_set(scc_ctx *c, int flags) {
-    if (!c || flags < 0 || flags > 3) return SCC_E_INPUT;
+    if (!c || flags < 0 || flags > 7) return SCC_E_INPUT;
+    if (flags & 4) flags |= 3;
     c->prof_flags = flags;
     c->instr = flags & 1;
     return SCC_OK;
@@ -689,6 +725,16 @@ scc_status scc_profile_get(scc_ctx *c, scc_profile_data *out, int reset) {
     if (!c || !out) return SCC_E_INPUT;
     *out = c->prof;
     if (reset) std::memset(&c->prof, 0, sizeof(c->prof));
+    return SCC_OK;
+}
+
+scc_status scc_profile_levels(scc_ctx *c, double *out, int64_t cap, int64_t *rows, int reset) {
+    if (!c || !rows || cap < 0 || (cap > 0 && !out)) return SCC_E_INPUT;
+    const int64_t have = (int64_t)c->level_log.size() / SCC_LEVEL_FIELDS;
+    *rows = have;
+    if (cap > 0 && have > 0)
+        std::memcpy(out, c->level_log.data(), sizeof(double) * SCC_LEVEL_FIELDS * (size_t)std::min(have, cap));
+    if (reset) c->level_log.clear();
     return SCC_OK;
 }
 

@@ -43,7 +43,9 @@ struct Ctl {
     uint32_t npiv;         // pivots elected by the last election = pivot list length
     uint32_t maxc;         // max colour (reduction scratch)
     uint32_t pull_levels;  // hybrid: direction-levels that ran bottom-up (pull)
+    uint32_t pull_mask;    // hybrid: directions the last level pulled (bit 0 F, bit 1 B)
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
+    unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
 };
 

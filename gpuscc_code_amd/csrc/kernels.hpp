@@ -42,6 +42,8 @@ struct ReachArgs {
     int iter;
     uint32_t dirs;
     uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
+    unsigned long long pull_arcs;  // hybrid: ... or one with at least this many (estimated) arcs
+    const uint8_t *dcls[2];  // degree class per vertex and direction (bit length of the degree)
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int instr;          // count work (expanded vertices, scanned arcs, visits)
@@ -49,6 +51,7 @@ struct ReachArgs {
     int32_t *root;      // trimmed vertices become their own SCC
     uint32_t *stamp;    // per-vertex tag of the last trim level that checked it
     uint32_t tag;       // this level's tag (unique per launch)
+    int fresh;          // every vertex live, every arc inside one colour: level 0 tests degrees only
 };
 
 struct Graph {  // device CSR + CSR^T
@@ -66,6 +69,8 @@ void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
 void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t bytes);
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
+// out[v] = bit length of row[v + 1] - row[v]
+void launch_degree_class(const Launch &l, const int32_t *row, int32_t n, uint8_t *out);
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,

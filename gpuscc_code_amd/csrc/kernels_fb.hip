@@ -69,6 +69,14 @@ __global__ void k_fill_i32(int32_t *p, int64_t count, int32_t val) {
         p[i] = val;
 }
 
+__global__ void k_degree_class(const int32_t *__restrict__ row, int32_t n, uint8_t *out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t deg = (uint32_t)(row[v + 1] - row[v]);
+        out[v] = (uint8_t)(deg ? 32 - __clz(deg) : 0);
+    }
+}
+
 // chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in graphs)
 __global__ void k_ctl_reset(Ctl *ctl, int what) {
     if (threadIdx.x != 0) return;
@@ -84,6 +92,7 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
         for (int d = 0; d < 2; d++) {
             ctl->qsize[d][0] = ctl->seeds[d];
             for (int i = 1; i < 4; i++) ctl->qsize[d][i] = 0;
+            for (int i = 0; i < 4; i++) ctl->qarcs[d][i] = 0;
         }
     }
     if (what & 64) {  // the pivot list seeds both directions
@@ -430,6 +439,9 @@ void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsi
 }
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
     hipLaunchKernelGGL(k_fill_i32, GRID(l), p, count, v);
+}
+void launch_degree_class(const Launch &l, const int32_t *row, int32_t n, uint8_t *out) {
+    hipLaunchKernelGGL(k_degree_class, GRID(l), row, n, out);
 }
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what);

@@ -46,13 +46,21 @@ struct Stage {
     int32_t buf[2][SQ_CAP];
     uint32_t cnt[2];
     uint32_t base[2];
+    unsigned long long arcs[2];  // estimated arcs of the vertices pushed since the last flush
 };
 
 __device__ inline void st_init(Stage &s) {
-    if (threadIdx.x < 2) s.cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 2) {
+        s.cnt[threadIdx.x] = 0;
+        s.arcs[threadIdx.x] = 0;
+    }
 }
 
+// arcs of a vertex of degree class k (bit length k): about 3/4 of 2^k
+__device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << k) >> 2; }
+
 __device__ inline void st_push(Stage &s, int d, int32_t v, const ReachArgs &a) {
+    if (a.dcls[d]) atomicAdd(&s.arcs[d], class_arcs(a.dcls[d][v]));
     uint32_t i = atomicAdd(&s.cnt[d], 1u);
     if (i < SQ_CAP) {
         s.buf[d][i] = v;
@@ -69,11 +77,16 @@ __device__ inline void st_flush(Stage &s, const ReachArgs &a) {
     uint32_t c1 = s.cnt[1] < SQ_CAP ? s.cnt[1] : SQ_CAP;
     if (threadIdx.x == 0 && c0) s.base[0] = atomicAdd(&a.ctl->qsize[0][(a.iter + 1) & 3], c0);
     if (threadIdx.x == 1 && c1) s.base[1] = atomicAdd(&a.ctl->qsize[1][(a.iter + 1) & 3], c1);
+    if (threadIdx.x < 2 && s.arcs[threadIdx.x])
+        atomicAdd(&a.ctl->qarcs[threadIdx.x][(a.iter + 1) & 3], s.arcs[threadIdx.x]);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < c0; i += blockDim.x) a.outq[0][s.base[0] + i] = s.buf[0][i];
     for (uint32_t i = threadIdx.x; i < c1; i += blockDim.x) a.outq[1][s.base[1] + i] = s.buf[1][i];
     __syncthreads();
-    if (threadIdx.x < 2) s.cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 2) {
+        s.cnt[threadIdx.x] = 0;
+        s.arcs[threadIdx.x] = 0;
+    }
     __syncthreads();
 }
 
@@ -133,11 +146,13 @@ __device__ inline uint32_t find_item(const uint32_t *off, uint32_t cnt, uint32_t
 // arcs t, t + blockDim, ... of their concatenation (the CTA gather of
 // reach_t.cu:195-265 and reach_l.cu:190-245, without their 256-arc windows)
 // ---------------------------------------------------------------------------
+constexpr uint32_t ITEM_CAP = 512;  // two items per work-item of a 256-wide workgroup
+
 struct Items {
-    uint32_t off[1024];
-    int32_t start[1024];
-    uint32_t cd[1024];  // colour | direction << 31
-    int32_t lim[1024];  // end of the row (huge rows)
+    uint32_t off[ITEM_CAP];
+    int32_t start[ITEM_CAP];
+    uint32_t cd[ITEM_CAP];  // colour | direction << 31
+    int32_t lim[ITEM_CAP];  // end of the row (huge rows)
     uint32_t shw[16];
 };
 
@@ -167,7 +182,10 @@ __device__ inline bool level_begin(const ReachArgs &a, bool queues) {
     if (chain_exit(a.ctl, a.iter)) return false;
     if (blockIdx.x == 0 && threadIdx.x < 2) {
         a.ctl->hsize[threadIdx.x][(a.iter + 2) & 3] = 0;
-        if (queues) a.ctl->qsize[threadIdx.x][(a.iter + 2) & 3] = 0;
+        if (queues) {
+            a.ctl->qsize[threadIdx.x][(a.iter + 2) & 3] = 0;
+            a.ctl->qarcs[threadIdx.x][(a.iter + 2) & 3] = 0;
+        }
     }
     return true;
 }
@@ -196,11 +214,12 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
         const int32_t *row = drow(a, d), *col = dcol(a, d);
         uint32_t bit = dbit(d);
         uint32_t acc = 0;
-        for (uint32_t base = 0; base < hs; base += blockDim.x) {
+        const uint32_t step = blockDim.x < ITEM_CAP ? blockDim.x : ITEM_CAP;  // rows per pass
+        for (uint32_t base = 0; base < hs; base += step) {
             uint32_t i = base + threadIdx.x;
             uint32_t nch = 0, c = 0;
             int32_t e0 = 0, e1 = 0;
-            if (i < hs) {
+            if (threadIdx.x < step && i < hs) {
                 int32_t v = a.hin[d][i];
                 e0 = row[v];
                 e1 = row[v + 1];
@@ -213,14 +232,16 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
             }
             uint32_t total;
             uint32_t off = block_excl_scan(nch, it.shw, total);
-            it.off[threadIdx.x] = off;
-            it.start[threadIdx.x] = e0;
-            it.lim[threadIdx.x] = e1;
-            it.cd[threadIdx.x] = c;
+            if (threadIdx.x < step) {
+                it.off[threadIdx.x] = off;
+                it.start[threadIdx.x] = e0;
+                it.lim[threadIdx.x] = e1;
+                it.cd[threadIdx.x] = c;
+            }
             __syncthreads();
             for (uint32_t k = acc + (blockIdx.x + G - acc % G) % G; k < acc + total; k += G) {
                 uint32_t local = k - acc;
-                uint32_t j = find_item(it.off, blockDim.x, local);
+                uint32_t j = find_item(it.off, step, local);
                 int32_t cs = it.start[j] + (int32_t)((local - it.off[j]) * HCHUNK);
                 int32_t cend = cs + (int32_t)HCHUNK < it.lim[j] ? cs + (int32_t)HCHUNK : it.lim[j];
                 uint32_t cc = it.cd[j];
@@ -353,6 +374,13 @@ struct LocalQ {
     int32_t v[2 * SQ_CAP];
 };
 
+// The local rounds start after the last use of the items of a level, so the
+// two share LDS (fewer bytes per workgroup, more workgroups per CU).
+union ItemsLocal {
+    Items it;
+    LocalQ lq;
+};
+
 // expands item v of direction d serially; new visits are staged
 __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, bool &found,
                                      unsigned long long &cx, unsigned long long &ce,
@@ -398,6 +426,13 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
         if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
         for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x) lq.v[k] = st.buf[0][k];
         for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x) lq.v[SQ_CAP + k] = st.buf[1][k];
+        // the moved vertices leave the next level's frontier (rows sent on below rejoin it)
+        if (a.dcls[0]) {
+            for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x)
+                atomicAdd(&st.arcs[0], 0ull - class_arcs(a.dcls[0][lq.v[k]]));
+            for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x)
+                atomicAdd(&st.arcs[1], 0ull - class_arcs(a.dcls[1][lq.v[SQ_CAP + k]]));
+        }
         __syncthreads();
         if (threadIdx.x < 2) st.cnt[threadIdx.x] = 0;
         __syncthreads();
@@ -408,6 +443,7 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                 int32_t v = d == 0 ? lq.v[k] : lq.v[SQ_CAP + (k - c0)];
                 const int32_t *row = drow(a, d);
                 if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
+                    if (a.dcls[d]) atomicAdd(&st.arcs[d], class_arcs(a.dcls[d][v]));
                     uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
                     a.outq[d][g] = v;
                 } else {
@@ -420,10 +456,11 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
 }
 
 __global__ void k_reach_queue(ReachArgs a) {
-    __shared__ Items it;
+    __shared__ ItemsLocal ul;
     __shared__ Stage st;
-    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
+    Items &it = ul.it;
+    LocalQ &lq = ul.lq;
     if (!level_begin(a, true)) return;
     st_init(st);
     __syncthreads();
@@ -492,10 +529,11 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
 // l: frontier queues with balanced expansion (reach_l.cu:151-253)
 // ---------------------------------------------------------------------------
 __global__ void k_reach_lb(ReachArgs a) {
-    __shared__ Items it;
+    __shared__ ItemsLocal ul;
     __shared__ Stage st;
-    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
+    Items &it = ul.it;
+    LocalQ &lq = ul.lq;
     if (!level_begin(a, true)) return;
     st_init(st);
     __syncthreads();
@@ -513,25 +551,110 @@ __global__ void k_reach_lb(ReachArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// h: a direction whose frontier holds fewer than pull_thr vertices expands its
-// queue (balanced push); a larger one runs bottom-up: every live vertex of
+// h: a direction whose frontier holds fewer than pull_thr vertices and fewer
+// than pull_arcs arcs (estimated from degree classes: few vertices may hold
+// hubs with most of the graph's arcs) expands its queue (balanced push); a
+// larger one runs bottom-up: every live vertex of
 // the direction that is not reached scans its reverse row for a reached
 // same-colour neighbour and stops at the first.  Pulled visits are queued too,
 // so the next level may push again.
 // ---------------------------------------------------------------------------
+// the directions level a.iter pulls (read once the previous level is complete)
+__device__ inline void pull_dirs(const ReachArgs &a, uint32_t &qs0, uint32_t &qs1, bool &pull0,
+                                 bool &pull1) {
+    qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
+    qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
+    pull0 = qs0 > 0 && (qs0 >= a.pull_thr || a.ctl->qarcs[0][a.iter & 3] >= a.pull_arcs);
+    pull1 = qs1 > 0 && (qs1 >= a.pull_thr || a.ctl->qarcs[1][a.iter & 3] >= a.pull_arcs);
+}
+
+// ---------------------------------------------------------------------------
+// Row scans with early exit, several per work-item in lockstep.  Task t scans
+// col[t][e[t] .. e1[t]) for a neighbour u != self[t] whose word w has
+// (w & mask[t]) == val[t] and colour c[t]; bit t of the result says it found
+// one.  One lane's scans are independent, so their gathers overlap: a lane
+// waits for one round trip per arc position instead of one per arc.  At most
+// max_steps arcs per task: tasks still open stay set in act, e[t] at their
+// next arc.
+// ---------------------------------------------------------------------------
+constexpr int SCAN_ILP = 4;  // vertices per work-item per pass of the sweeps below
+
+template <int T>
+__device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const (&col)[T], int32_t (&e)[T],
+                                     const int32_t (&e1)[T], const uint32_t (&mask)[T],
+                                     const uint32_t (&val)[T], const uint32_t (&c)[T],
+                                     const int32_t (&self)[T], uint32_t &act, int max_steps,
+                                     unsigned long long &ce) {
+    uint32_t hit = 0;
+    for (int s = 0; act && s < max_steps; s++) {
+        uint32_t wu[T];
+        int32_t u[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) u[t] = ((act >> t) & 1u) ? col[t][e[t]] : -1;
+#pragma unroll
+        for (int t = 0; t < T; t++) wu[t] = u[t] >= 0 ? word[u[t]] : 0u;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            if (!((act >> t) & 1u)) continue;
+            ce++;
+            if (u[t] != self[t] && (wu[t] & mask[t]) == val[t] && colour_of(wu[t]) == c[t]) {
+                hit |= 1u << t;
+                act &= ~(1u << t);
+            } else if (++e[t] >= e1[t]) {
+                act &= ~(1u << t);
+            }
+        }
+    }
+    return hit;
+}
+
+constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
+
+// Queues the vertices one pull round claimed: bit k of claimed[d] is vertex
+// base + k * blockDim.x + threadIdx.x.  One prefix sum and one atomicAdd per
+// direction and workgroup; the only barriers of the round.  All work-items call this.
+__device__ inline void pull_commit(const ReachArgs &a, int64_t base, const uint32_t (&claimed)[2],
+                                   const unsigned long long (&arcs)[2], uint32_t *shw, uint32_t *sbase,
+                                   unsigned long long *sh) {
+    const uint32_t c0 = __popc(claimed[0]), c1 = __popc(claimed[1]);
+    uint32_t total;  // packed counts: a workgroup claims at most 16 * 1024 < 2^16 per direction
+    const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total);
+    unsigned long long ar0 = block_sum(arcs[0], sh);
+    unsigned long long ar1 = block_sum(arcs[1], sh);
+    if (threadIdx.x == 0) {
+        const int slot = (a.iter + 1) & 3;
+        sbase[0] = (total & 0xFFFFu) ? atomicAdd(&a.ctl->qsize[0][slot], total & 0xFFFFu) : 0u;
+        sbase[1] = (total >> 16) ? atomicAdd(&a.ctl->qsize[1][slot], total >> 16) : 0u;
+        if (ar0) atomicAdd(&a.ctl->qarcs[0][slot], ar0);
+        if (ar1) atomicAdd(&a.ctl->qarcs[1][slot], ar1);
+    }
+    __syncthreads();
+    uint32_t o[2] = {sbase[0] + (pre & 0xFFFFu), sbase[1] + (pre >> 16)};
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        for (uint32_t m = claimed[d]; m; m &= m - 1) {
+            int k = __ffs(m) - 1;
+            a.outq[d][o[d]++] = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+        }
+    }
+    __syncthreads();  // sbase is rewritten by the next round
+}
+
 __global__ void k_reach_hybrid(ReachArgs a) {
-    __shared__ Items it;
+    __shared__ ItemsLocal ul;
     __shared__ Stage st;
-    __shared__ LocalQ lq;
     __shared__ unsigned long long sh[16];
+    Items &it = ul.it;
+    LocalQ &lq = ul.lq;
     if (!level_begin(a, true)) return;
     st_init(st);
     __syncthreads();
-    uint32_t qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
-    uint32_t qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
-    bool pull0 = qs0 > 0 && qs0 >= a.pull_thr, pull1 = qs1 > 0 && qs1 >= a.pull_thr;
+    uint32_t qs0, qs1;
+    bool pull0, pull1;
+    pull_dirs(a, qs0, qs1, pull0, pull1);
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->pull_mask = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
     huge_phase<true>(a, it, &st, found, cx, ce, cv);
     st_flush(st, a);
     // push part: queue items of the directions that do not pull
@@ -540,40 +663,73 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     if (pull0 || pull1) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             a.ctl->pull_levels += (pull0 ? 1u : 0u) + (pull1 ? 1u : 0u);
+        // Rounds of PULL_V vertices per work-item: all their words are loaded at
+        // once, then only the candidates (live, not reached in a pulling
+        // direction) are scanned, SCAN_ILP at a time in lockstep.  No barrier
+        // until the round's claims are queued.
         int64_t beg, end;
         my_range(a.n, beg, end);
-        for (int64_t base = beg; base < end; base += blockDim.x) {
-            int64_t vv = base + threadIdx.x;
-            if (vv < end) {
-                int32_t v = (int32_t)vv;
-                uint32_t w = a.word[v];
-                if (!(w & ST_E)) {
-                    uint32_t c = colour_of(w);
-                    for (int d = 0; d < 2; d++) {
-                        if (!(d == 0 ? pull0 : pull1)) continue;
-                        uint32_t bit = dbit(d);
-                        if (w & bit) continue;
-                        // reverse direction: a forward visit needs a reached in-neighbour
-                        const int32_t *row = drow(a, 1 - d), *col = dcol(a, 1 - d);
-                        int32_t e0 = row[v], e1 = row[v + 1];
+        constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, direction d
+        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * PULL_V) {  // block-uniform
+            uint32_t cand = 0;
+#pragma unroll
+            for (int k = 0; k < PULL_V; k++) {
+                int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+                uint32_t w = vv < end ? a.word[vv] : ST_E;
+                if (!(w & ST_E) && ((pull0 && !(w & ST_F)) || (pull1 && !(w & ST_B)))) cand |= 1u << k;
+            }
+            uint32_t claimed[2] = {0u, 0u};
+            unsigned long long arcs[2] = {0ull, 0ull};
+            while (cand) {  // per work-item
+                int32_t v[SCAN_ILP];
+                int kk[SCAN_ILP];
+#pragma unroll
+                for (int j = 0; j < SCAN_ILP; j++) {
+                    kk[j] = cand ? __ffs(cand) - 1 : -1;
+                    cand &= cand ? cand - 1 : 0u;
+                    v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
+                }
+                uint32_t w[SCAN_ILP];
+#pragma unroll
+                for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
+                const int32_t *col[T];
+                int32_t e[T], e1[T], self[T];
+                uint32_t mask[T], val[T], c[T], act = 0;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const int d = t / SCAN_ILP, j = t % SCAN_ILP;
+                    // a visit in direction d needs a reached neighbour over the reverse arcs
+                    const int32_t *row = drow(a, 1 - d);
+                    col[t] = dcol(a, 1 - d);
+                    mask[t] = dbit(d) | ST_E;
+                    val[t] = dbit(d);
+                    c[t] = colour_of(w[j]);
+                    self[t] = -1;
+                    e[t] = e1[t] = 0;
+                    if ((d == 0 ? pull0 : pull1) && !(w[j] & (ST_E | dbit(d)))) {
+                        e[t] = row[v[j]];
+                        e1[t] = row[v[j] + 1];
                         cx++;
-                        for (int32_t e = e0; e < e1; e++) {
-                            ce++;
-                            uint32_t wu = a.word[col[e]];
-                            if ((wu & (bit | ST_E)) == bit && colour_of(wu) == c) {
-                                uint32_t old = atomicOr(&a.word[v], bit);
-                                if (!(old & bit)) {
-                                    found = true;
-                                    cv++;
-                                    st_push(st, d, v, a);
-                                }
-                                break;
-                            }
-                        }
+                        if (e[t] < e1[t]) act |= 1u << t;
+                    }
+                }
+                uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    if (!((hit >> t) & 1u)) continue;
+                    const int d = t / SCAN_ILP, j = t % SCAN_ILP;
+                    uint32_t old = atomicOr(&a.word[v[j]], dbit(d));
+                    if (!(old & dbit(d))) {
+                        claimed[d] |= 1u << kk[j];
+                        arcs[d] += class_arcs(a.dcls[d][v[j]]);
                     }
                 }
             }
-            st_maybe_flush(st, a);
+            if (claimed[0] | claimed[1]) {
+                found = true;
+                cv += (unsigned long long)(__popc(claimed[0]) + __popc(claimed[1]));
+            }
+            pull_commit(a, base, claimed, arcs, it.shw, st.base, sh);
         }
     }
     local_rounds(a, (pull0 || pull1) ? 0xFFFFFFFFu : qs0 + qs1, st, lq, found, cx, ce, cv);
@@ -615,20 +771,74 @@ __global__ void k_trim_level(ReachArgs a) {
     st_init(st);
     __syncthreads();
     bool found = false;
-    if (a.iter == 0) {
+    if (a.iter == 0 && a.fresh) {
+        // every neighbour is live and of the same colour: a vertex keeps a live
+        // same-colour neighbour in a direction iff its row there holds a vertex
+        // other than itself (rows are duplicate-free: two entries suffice)
         int64_t beg, end;
         my_range(a.n, beg, end);
-        for (int64_t base = beg; base < end; base += blockDim.x) {
+        for (int64_t base = beg; base < end; base += blockDim.x) {  // block-uniform
             int64_t vv = base + threadIdx.x;
             if (vv < end) {
                 int32_t v = (int32_t)vv;
-                uint32_t w = a.word[v];
-                if (!(w & ST_E) && trim_check(a, v, colour_of(w))) {
+                int32_t o0 = a.row[v], o1 = a.row[v + 1], i0 = a.rowT[v], i1 = a.rowT[v + 1];
+                bool out = o1 - o0 > 1 || (o1 - o0 == 1 && a.col[o0] != v);
+                bool in = i1 - i0 > 1 || (i1 - i0 == 1 && a.colT[i0] != v);
+                if (!(out && in)) {
+                    a.word[v] = make_word(colour_of(a.word[v]), 28);  // P|T|E (scc_routines.cu:381)
+                    a.root[v] = v;
                     found = true;
                     st_push(st, 0, v, a);
                 }
             }
             st_maybe_flush(st, a);
+        }
+    } else if (a.iter == 0) {
+        int64_t beg, end;
+        my_range(a.n, beg, end);
+        constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: vertex j, out-row (d = 0) or in-row
+        unsigned long long ce = 0;
+        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * SCAN_ILP) {  // block-uniform
+            int32_t v[SCAN_ILP];
+            uint32_t w[SCAN_ILP];
+#pragma unroll
+            for (int j = 0; j < SCAN_ILP; j++) {
+                int64_t vv = base + (int64_t)j * blockDim.x + threadIdx.x;
+                v[j] = vv < end ? (int32_t)vv : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
+            const int32_t *col[T];
+            int32_t e[T], e1[T], self[T];
+            uint32_t mask[T], val[T], c[T], act = 0;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const int d = t / SCAN_ILP, j = t % SCAN_ILP;
+                const int32_t *row = d == 0 ? a.row : a.rowT;
+                col[t] = d == 0 ? a.col : a.colT;
+                mask[t] = ST_E;
+                val[t] = 0;
+                c[t] = colour_of(w[j]);
+                self[t] = v[j];
+                e[t] = e1[t] = 0;
+                if (!(w[j] & ST_E)) {
+                    e[t] = row[v[j]];
+                    e1[t] = row[v[j] + 1];
+                    if (e[t] < e1[t]) act |= 1u << t;
+                }
+            }
+            // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
+            uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
+#pragma unroll
+            for (int j = 0; j < SCAN_ILP; j++) {
+                if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
+                    a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
+                    a.root[v[j]] = v[j];
+                    found = true;
+                    st_push(st, 0, v[j], a);
+                }
+                st_maybe_flush(st, a);
+            }
         }
     } else {
         uint32_t qs = a.ctl->qsize[0][a.iter & 3];
@@ -666,6 +876,7 @@ __global__ void k_trim_level(ReachArgs a) {
                     uint32_t cc = cd & 0x7FFFFFFFu;
                     uint32_t wu = a.word[u];
                     if (!(wu & ST_E) && colour_of(wu) == cc &&
+                        a.stamp[u] != a.tag &&  // plain load first: hubs are reached by many arcs
                         atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
                         found = true;
                         st_push(st, 0, u, a);

@@ -23,6 +23,8 @@ PIVOT_PLAIN, PIVOT_LEARNED = 0, 1
 
 ST_B, ST_F, ST_E, ST_T, ST_P, ST_BX, ST_FX = 1, 2, 4, 8, 16, 32, 64
 MLP_PARAMS, MLP_FEATURES, MLP_HIDDEN = 1505, 12, 32
+LEVEL_FIELDS = ("kind", "level", "ms", "queue_f", "queue_b", "arcs_f", "arcs_b", "pulled",
+                "expanded", "edges", "visited")  # one row of scc_profile_levels
 COLOUR_MASK = 0x00FFFFFF
 
 
@@ -37,6 +39,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     sequential_reach: int = 0
     huge_row: int = 0
     local_rounds: int = 0
+    hybrid_arc_div: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -48,6 +51,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.sequential_reach = self.sequential_reach
         o.huge_row = self.huge_row
         o.local_rounds = self.local_rounds
+        o.hybrid_arc_div = self.hybrid_arc_div
         return o
 
 
@@ -269,7 +273,8 @@ class Context:
                                       batch, lr, seed, _p(losses, _lib._fp)))
         return losses[:epochs]
 
-    # measurement: flags 1 = device work counters, 2 = HIP-event kernel timing
+    # measurement: flags 1 = device work counters, 2 = HIP-event kernel timing,
+    # 4 = per-level log (levels launched one at a time)
     def profile(self, flags):
         self._chk(lib().scc_profile_set(self._h, int(flags)))
 
@@ -281,6 +286,15 @@ class Context:
         p = _lib.SccProfile()
         self._chk(lib().scc_profile_get(self._h, ctypes.byref(p), 1 if reset else 0))
         return p.as_dict()
+
+    def profile_levels(self, reset=True):
+        """The level log of profile flag 4: array (rows, len(LEVEL_FIELDS))."""
+        rows = ctypes.c_int64()
+        self._chk(lib().scc_profile_levels(self._h, None, 0, ctypes.byref(rows), 0))
+        out = np.zeros((rows.value, len(LEVEL_FIELDS)), np.float64)
+        self._chk(lib().scc_profile_levels(self._h, _p(out, _lib._dp), rows.value, ctypes.byref(rows),
+                                           1 if reset else 0))
+        return out
 
 
 def computeSCC(g: Graph, opt: alg_opt = None, variant=0, ctx: Context = None):  # noqa: N802

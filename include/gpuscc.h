@@ -70,6 +70,8 @@ typedef struct {
     int32_t local_rounds;      /* d, l, h: after its share of a level, a workgroup keeps expanding its own
                                   new visits in LDS for up to this many rounds (0 = default 64,
                                   -1 = off) */
+    int32_t hybrid_arc_div;    /* h: a direction also pulls when the arcs of its frontier (estimated
+                                  from degree classes) reach m / this (0 = default 14, -1 = off) */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -196,7 +198,8 @@ scc_status scc_mlp_train(scc_ctx *ctx, const float *X, const float *y, int64_t N
  * report (the reference times phases with gettimeofday + cudaDeviceSynchronize,
  * timing.h:14).  flags: 1 = count expanded vertices / scanned edges / new
  * visits with device counters; 2 = time every level kernel with HIP events on
- * the context's stream. */
+ * the context's stream; 4 = also log one row per reach or trim level (implies
+ * 1 and 2; levels are then launched one at a time, so wall times grow). */
 typedef struct {
     int64_t reach_launches;   /* level kernels launched, including ones that exited at once */
     int64_t reach_levels;     /* level kernels that did work */
@@ -215,6 +218,13 @@ scc_status scc_profile_get(scc_ctx *ctx, scc_profile_data *out, int reset);
  * device memory and writes the same number (for scaling hardware traffic
  * counters against a kernel of known traffic). */
 scc_status scc_profile_copy(scc_ctx *ctx, int64_t bytes);
+/* The level log of flag 4, SCC_LEVEL_FIELDS doubles per row: kind (0 reach,
+ * 1 trim), level, kernel ms, input queue F, input queue B, estimated arcs of
+ * queue F, of queue B, pulled directions (bit 0 F, bit 1 B), expanded vertices,
+ * scanned arcs, new visits (the last three for reach levels).  Copies up to
+ * cap rows, sets *rows to the rows logged; reset clears the log. */
+#define SCC_LEVEL_FIELDS 11
+scc_status scc_profile_levels(scc_ctx *ctx, double *out, int64_t cap, int64_t *rows, int reset);
 
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
