@@ -309,8 +309,9 @@ scc_status do_features(scc_ctx *c, bool want_score, float *feat_out) {
 
 // one election into the table that is not `prev`; winners -> state 19 + pivot list.
 // learned: the colour split runs first, then the scorer, then the election
-// with the scores in the keys.
-scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = false) {
+// with the scores in the keys.  sync = false leaves *npiv alone: the count is
+// then read from the control block with the next chain's results.
+scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = false, bool sync = true) {
     Launch l = launch_for(c);
     c->epoch++;
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
@@ -320,21 +321,23 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
     if (learned) {
         if (recolour)
             launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits,
-                          elect_tag(c->epoch), 1, 0, nullptr);
+                          elect_tag(c->epoch), 1, 0, nullptr, c->d_ctl);
         CHK(do_features(c, true, nullptr));
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
-                      0, 1, c->d_score);
+                      0, 1, c->d_score, c->d_ctl);
     } else {
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
-                      recolour, 1, nullptr);
+                      recolour, 1, nullptr, c->d_ctl);
     }
     launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
     launch_ctl_reset(c->stream, c->d_ctl, 64);  // the pivot list seeds both directions
-    CHK(read_ctl(c));
     c->prev_tab ^= 1;
+    if (!sync) return SCC_OK;
+    CHK(read_ctl(c));
     *npiv = c->h_ctl->npiv;
     return SCC_OK;
 }
+
 
 scc_status do_count(scc_ctx *c, int what, int64_t *out) {
     launch_ctl_reset(c->stream, c->d_ctl, 8);
@@ -353,7 +356,7 @@ scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
 }
 
 // update_wcc (wcc.cu:146-226): labels to fixpoint, fresh colours, election
-scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = false) {
+scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = false, bool sync = true) {
     Launch l = launch_for(c);
     launch_ctl_reset(c->stream, c->d_ctl, 16);
     launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
@@ -362,7 +365,7 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = fals
     CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
                   wg, 0, iters));
     launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
-    return do_elect(c, 0, npiv, learned);
+    return do_elect(c, 0, npiv, learned, sync);
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -621,25 +624,27 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     scc_status s = SCC_OK;
     auto body = [&]() -> scc_status {
         Launch l = launch_for(c);
-        launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs);
+        launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs,
+                    c->d_ctl);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();
         CHK(do_trim(c, &it, true));  // trim1 (scc.cu:232)
         st.trim_iters += it;
         st.ms_trim += ms_since(tp);
-        tp = std::chrono::steady_clock::now();
-        CHK(do_elect(c, 0, &npiv, learned));  // first pivot: lowest live id (scc.cu:235-245)
-        st.ms_update += ms_since(tp);
-        if (npiv > 0) {
-            int32_t fp = -1;
-            HIPCHK(c, hipMemcpy(&fp, c->d_piv, sizeof(int32_t), hipMemcpyDeviceToHost));
-            st.first_pivot = fp;
+        // The device counts live vertices: after trim-1 every live vertex still
+        // has its initial colour, so live vertices mean at least one pivot and
+        // the first election needs no wait of its own (its count arrives with
+        // the first reach's).
+        bool unsynced = c->h_ctl->live > 0;
+        if (unsynced) {
+            tp = std::chrono::steady_clock::now();
+            CHK(do_elect(c, 0, &npiv, learned, false));  // first pivot: lowest live id (scc.cu:235-245)
+            st.ms_update += ms_since(tp);
         }
-        st.pivots_total += npiv;
         bool wcc_done = false;
         const int64_t one_pct = n / 100;
-        while (npiv > 0) {
+        while (unsynced || npiv > 0) {
             st.steps++;
             int sched = schedule;
             if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1)
@@ -660,6 +665,13 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 st.bwd_depths += d;
                 st.ms_fwd += ms_since(tp);
             }
+            if (unsynced) {  // the first election's results, read with the reach's
+                npiv = c->h_ctl->npiv;
+                st.first_pivot = (int64_t)c->h_ctl->first_piv;
+                if (npiv == 0) return fail(c, SCC_E_STATE, "live vertices left but no pivot elected");
+                unsynced = false;
+            }
+            st.pivots_total += npiv;
             tp = std::chrono::steady_clock::now();
             CHK(do_trim(c, &it));
             st.trim_iters += it;
@@ -686,7 +698,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 CHK(do_count(c, 0, &ne));  // scc.cu:879-890
             }
             st.ms_update += ms_since(tp);
-            st.pivots_total += npiv;
         }
         // canonical labels + #SCC (scc.cu:912-919)
         launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
@@ -883,7 +894,7 @@ scc_status scc_phase_recolour(scc_ctx *c) {
     Launch l = launch_for(c);
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), 1, 0,
-                  nullptr);
+                  nullptr, c->d_ctl);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SCC_OK;

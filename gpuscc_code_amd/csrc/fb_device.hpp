@@ -44,6 +44,9 @@ struct Ctl {
     uint32_t maxc;         // max colour (reduction scratch)
     uint32_t pull_levels;  // hybrid: direction-levels that ran bottom-up (pull)
     uint32_t pull_mask;    // hybrid: directions the last level pulled (bit 0 F, bit 1 B)
+    uint32_t live;         // vertices not eliminated (set by k_init; trims, pair trims and
+                           // FB eliminations subtract)
+    uint32_t first_piv;    // lowest pivot of the last election
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
@@ -78,6 +81,28 @@ __device__ inline bool chain_exit(Ctl *ctl, int iter) {
 
 __device__ inline void chain_progress(Ctl *ctl, int iter, bool found) {
     if (__any(found) && (threadIdx.x & (WAVE - 1)) == 0) ctl->changed[iter & 3] = 1;
+}
+
+// exclusive prefix sum over the workgroup (blockDim.x <= 1024, wave64)
+__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &total) {
+    int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    uint32_t inc = x;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, WAVE);
+        if (lane >= o) inc += y;
+    }
+    __syncthreads();
+    if (lane == WAVE - 1) shw[wid] = inc;
+    __syncthreads();
+    int nw = (blockDim.x + WAVE - 1) / WAVE;
+    uint32_t pre = 0, tot = 0;
+    for (int i = 0; i < nw; i++) {
+        uint32_t s = shw[i];
+        if (i < wid) pre += s;
+        tot += s;
+    }
+    total = tot;
+    return pre + inc - x;
 }
 
 // Block-wide sum / max through LDS (blockDim.x <= 1024).

@@ -61,9 +61,9 @@ struct Graph {  // device CSR + CSR^T
 };
 
 // kernels_fb.hip
-// colour 1 everywhere, or graph id + 1 when off (G + 1 offsets) is given
+// colour 1 everywhere, or graph id + 1 when off (G + 1 offsets) is given; ctl->live = n
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G);
+                 int32_t G, Ctl *ctl);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
@@ -74,7 +74,8 @@ void launch_degree_class(const Launch &l, const int32_t *row, int32_t n, uint8_t
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
-                   unsigned long long tag, int recolour, int elect, const uint16_t *score);
+                   unsigned long long tag, int recolour, int elect, const uint16_t *score,
+                   Ctl *ctl);  // ctl (may be null): FB eliminations leave ctl->live
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);

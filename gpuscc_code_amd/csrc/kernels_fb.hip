@@ -24,7 +24,8 @@ __device__ inline uint32_t graph_of(const int32_t *off, int32_t G, int32_t v) {
 
 // init: colour 1 (thrust::fill, scc.cu:214) or, for a batch, graph id + 1;
 // state 0; root = self
-__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G) {
+__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G, Ctl *ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->live = (uint32_t)n;
     int64_t beg, end;
     my_range(n, beg, end);
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
@@ -103,7 +104,10 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
         ctl->seeds[0] = 0;
         ctl->seeds[1] = 0;
     }
-    if (what & 4) ctl->npiv = 0;
+    if (what & 4) {
+        ctl->npiv = 0;
+        ctl->first_piv = 0xFFFFFFFFu;
+    }
     if (what & 8) ctl->count = 0;
     if (what & 16) ctl->maxc = 0;
     if (what & 32) {
@@ -120,94 +124,126 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
 // 3c+[F]+2[B] and state 0.  Every live v then offers its key to the slot of
 // its colour with a 64-bit atomicMin (lowest key wins, deterministic).
 // ---------------------------------------------------------------------------
+constexpr int UPD_V = 8;  // vertices per work-item per round: their loads are issued together
+
 __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          const unsigned long long *__restrict__ prev, unsigned long long *cur,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
-                         const uint16_t *__restrict__ score) {
+                         const uint16_t *__restrict__ score, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
     int64_t beg, end;
     my_range(n, beg, end);
-    int lane = threadIdx.x & (WAVE - 1);
-    for (int64_t base = beg; base < end; base += blockDim.x) {
-        int64_t vv = base + threadIdx.x;
-        bool cand = false;
-        uint32_t sl = 0;
-        unsigned long long key = ~0ull;
-        if (vv < end) {
-            int32_t v = (int32_t)vv;
-            uint32_t w = word[v];
-            if (!(w & ST_E)) {
-                uint32_t c = colour_of(w);
-                bool elim = false;
-                if (recolour) {
-                    uint32_t k = ((w & ST_F) ? 1u : 0u) + ((w & ST_B) ? 2u : 0u);
-                    if (k == 3) {
-                        word[v] = w | ST_E;
-                        uint32_t pv = (uint32_t)prev[slot_of(c, slot_bits)];
-                        root[v] = pv < (uint32_t)n ? (int32_t)pv : v;
-                        elim = true;
-                    } else {
-                        c = (3u * c + k) & COLOUR_MASK;
-                        word[v] = make_word(c, 0);
-                    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    unsigned long long gone = 0;  // F and B: eliminated into the pivot's SCC
+    for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * UPD_V) {  // block-uniform
+        uint32_t w[UPD_V];
+#pragma unroll
+        for (int k = 0; k < UPD_V; k++) {
+            int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+            w[k] = vv < end ? word[vv] : ST_E;
+        }
+        uint32_t cmask = 0, sl[UPD_V];
+        unsigned long long key[UPD_V];
+#pragma unroll
+        for (int k = 0; k < UPD_V; k++) {
+            sl[k] = 0;
+            key[k] = ~0ull;
+            if (w[k] & ST_E) continue;
+            int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+            uint32_t c = colour_of(w[k]);
+            if (recolour) {
+                uint32_t f = ((w[k] & ST_F) ? 1u : 0u) + ((w[k] & ST_B) ? 2u : 0u);
+                if (f == 3) {
+                    word[v] = w[k] | ST_E;
+                    uint32_t pv = (uint32_t)prev[slot_of(c, slot_bits)];
+                    root[v] = pv < (uint32_t)n ? (int32_t)pv : v;
+                    gone++;
+                    continue;
                 }
-                if (!elim && elect) {
-                    cand = true;
-                    sl = slot_of(c, slot_bits);
-                    uint32_t sc = score ? (uint32_t)(0xFFFFu - score[v]) : 0u;
-                    key = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
-                }
+                c = (3u * c + f) & COLOUR_MASK;
+                word[v] = make_word(c, 0);
+            }
+            if (elect) {
+                cmask |= 1u << k;
+                sl[k] = slot_of(c, slot_bits);
+                uint32_t sc = score ? (uint32_t)(0xFFFFu - score[v]) : 0u;
+                key[k] = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
             }
         }
-        // wave-level combine when every candidate lane targets the same slot
-        // (the first election sends all live vertices to one slot)
-        unsigned long long mask = __ballot(cand);
-        if (mask == 0) continue;
-        int first = __ffsll((long long)mask) - 1;
-        uint32_t sl0 = __shfl(sl, first, WAVE);
-        bool same = __all(!cand || sl == sl0);
-        if (same) {
-            unsigned long long k = key;
+        // One atomicMin per wave and round when every candidate of the wave
+        // targets one slot (the first election sends all live vertices there);
+        // otherwise one per candidate.
+        uint32_t sl_lane = cmask ? sl[__ffs(cmask) - 1] : 0u;
+        bool lane_same = true;
+        unsigned long long lane_min = ~0ull;
+#pragma unroll
+        for (int k = 0; k < UPD_V; k++) {
+            if (!((cmask >> k) & 1u)) continue;
+            lane_same = lane_same && sl[k] == sl_lane;
+            lane_min = key[k] < lane_min ? key[k] : lane_min;
+        }
+        unsigned long long bm = __ballot(cmask != 0);
+        if (bm == 0) continue;  // wave-uniform (the loop's trip count is block-uniform)
+        int first = __ffsll((long long)bm) - 1;
+        uint32_t sl0 = __shfl(sl_lane, first, WAVE);
+        if (__all(cmask == 0 || (lane_same && sl_lane == sl0))) {
+            unsigned long long m = lane_min;
             for (int o = WAVE / 2; o > 0; o >>= 1) {
-                unsigned long long x = __shfl_xor(k, o, WAVE);
-                k = x < k ? x : k;
+                unsigned long long x = __shfl_xor(m, o, WAVE);
+                m = x < m ? x : m;
             }
-            if (lane == first && k < cur[sl0]) atomicMin(&cur[sl0], k);
-        } else if (cand && key < cur[sl]) {
-            atomicMin(&cur[sl], key);
+            if (lane == first && m < cur[sl0]) atomicMin(&cur[sl0], m);
+        } else {
+#pragma unroll
+            for (int k = 0; k < UPD_V; k++)
+                if (((cmask >> k) & 1u) && key[k] < cur[sl[k]]) atomicMin(&cur[sl[k]], key[k]);
         }
+    }
+    if (recolour && ctl) {
+        gone = block_sum(gone, sh);
+        if (threadIdx.x == 0 && gone) atomicSub(&ctl->live, (uint32_t)gone);
     }
 }
 
 // Winners become pivots (state 19 = P|F|B, scc_routines.cu:219) and are
 // appended to the pivot list through an LDS-staged block queue.
+constexpr int CONF_V = 16;  // vertices per work-item per round
+
 __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *__restrict__ cur,
                           int slot_bits, int32_t *piv, Ctl *ctl) {
-    __shared__ int32_t sq[1024];
-    __shared__ uint32_t scount, sbase;
+    __shared__ uint32_t shw[16];
+    __shared__ uint32_t sbase;
     int64_t beg, end;
     my_range(n, beg, end);
-    for (int64_t base = beg; base < end; base += blockDim.x) {
-        if (threadIdx.x == 0) scount = 0;
-        __syncthreads();
-        int64_t vv = base + threadIdx.x;
-        if (vv < end) {
-            int32_t v = (int32_t)vv;
-            uint32_t w = word[v];
-            if (!(w & ST_E)) {
-                uint32_t c = colour_of(w);
-                if ((uint32_t)cur[slot_of(c, slot_bits)] == (uint32_t)v) {
-                    word[v] = make_word(c, 19);
-                    sq[atomicAdd(&scount, 1u)] = v;
-                }
+    for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * CONF_V) {  // block-uniform
+        uint32_t w[CONF_V];
+#pragma unroll
+        for (int k = 0; k < CONF_V; k++) {
+            int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+            w[k] = vv < end ? word[vv] : ST_E;
+        }
+        uint32_t won = 0;
+#pragma unroll
+        for (int k = 0; k < CONF_V; k++) {
+            if (w[k] & ST_E) continue;
+            uint32_t v = (uint32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+            uint32_t c = colour_of(w[k]);
+            if ((uint32_t)cur[slot_of(c, slot_bits)] == v) {
+                word[v] = make_word(c, 19);
+                won |= 1u << k;
+                atomicMin(&ctl->first_piv, v);
             }
         }
+        // the round's winners join the pivot list: one prefix sum, one atomicAdd
+        uint32_t total;
+        uint32_t pre = block_excl_scan((uint32_t)__popc(won), shw, total);
+        if (total == 0) continue;  // block-uniform
+        if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, total);
         __syncthreads();
-        if (scount > 0) {
-            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, scount);
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < scount; i += blockDim.x) piv[sbase + i] = sq[i];
-        }
-        __syncthreads();
+        uint32_t o = sbase + pre;
+        for (uint32_t m = won; m; m &= m - 1)
+            piv[o++] = (int32_t)(base + (int64_t)(__ffs(m) - 1) * blockDim.x + threadIdx.x);
+        __syncthreads();  // sbase is rewritten by the next round
     }
 }
 
@@ -221,6 +257,7 @@ __global__ void k_count_state(const uint32_t *word, int32_t n, int what, Ctl *ct
     int64_t beg, end;
     my_range(n, beg, end);
     unsigned long long c = 0;
+#pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
         uint32_t s = word[v] & 0xFFu;
         if (what == 0) c += ((s & 12u) == 4u);
@@ -236,6 +273,7 @@ __global__ void k_max_colour(const uint32_t *word, int32_t n, Ctl *ctl) {
     int64_t beg, end;
     my_range(n, beg, end);
     uint32_t m = 0;
+#pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
         uint32_t c = colour_of(word[v]);
         m = c > m ? c : m;
@@ -311,7 +349,10 @@ __global__ void k_trim2_commit(uint32_t *word, int32_t n, Ctl *ctl) {
         pairs += p ? 1 : 0;
     }
     pairs = block_sum(pairs, sh);
-    if (threadIdx.x == 0 && pairs) atomicAdd(&ctl->count, pairs);
+    if (threadIdx.x == 0 && pairs) {
+        atomicAdd(&ctl->count, pairs);
+        atomicSub(&ctl->live, (uint32_t)(2 * pairs));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -375,6 +416,7 @@ __global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, co
 __global__ void k_label_seed(const int32_t *root, int32_t *mn, int32_t n) {
     int64_t beg, end;
     my_range(n, beg, end);
+#pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x)
         if (root[v] == (int32_t)v) mn[v] = (int32_t)v;
 }
@@ -382,6 +424,7 @@ __global__ void k_label_seed(const int32_t *root, int32_t *mn, int32_t n) {
 __global__ void k_label_min(const int32_t *root, int32_t *mn, int32_t n) {
     int64_t beg, end;
     my_range(n, beg, end);
+#pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
         int32_t r = root[v];
         if ((int32_t)v < mn[r]) atomicMin(&mn[r], (int32_t)v);
@@ -391,6 +434,7 @@ __global__ void k_label_min(const int32_t *root, int32_t *mn, int32_t n) {
 __global__ void k_label_final(const int32_t *root, const int32_t *mn, int32_t *label, int32_t n) {
     int64_t beg, end;
     my_range(n, beg, end);
+#pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) label[v] = mn[root[v]];
 }
 
@@ -425,8 +469,8 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G) {
-    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G);
+                 int32_t G, Ctl *ctl) {
+    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, ctl);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
@@ -448,9 +492,9 @@ void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
-                   unsigned long long tag, int recolour, int elect, const uint16_t *score) {
+                   unsigned long long tag, int recolour, int elect, const uint16_t *score, Ctl *ctl) {
     hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, elect,
-                       score);
+                       score, ctl);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl) {

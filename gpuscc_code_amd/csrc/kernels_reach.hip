@@ -108,28 +108,6 @@ __device__ inline void instr_flush(Ctl *ctl, unsigned long long x, unsigned long
     if (threadIdx.x == 0 && v) atomicAdd(&ctl->st_visited, v);
 }
 
-// exclusive prefix sum over the workgroup (blockDim.x <= 1024, wave64)
-__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &total) {
-    int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    uint32_t inc = x;
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(inc, o, WAVE);
-        if (lane >= o) inc += y;
-    }
-    __syncthreads();
-    if (lane == WAVE - 1) shw[wid] = inc;
-    __syncthreads();
-    int nw = (blockDim.x + WAVE - 1) / WAVE;
-    uint32_t pre = 0, tot = 0;
-    for (int i = 0; i < nw; i++) {
-        uint32_t s = shw[i];
-        if (i < wid) pre += s;
-        tot += s;
-    }
-    total = tot;
-    return pre + inc - x;
-}
-
 // largest k < cnt with off[k] <= e (off[0] == 0 <= e)
 __device__ inline uint32_t find_item(const uint32_t *off, uint32_t cnt, uint32_t e) {
     uint32_t lo = 0, hi = cnt;
@@ -610,12 +588,13 @@ __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const 
 
 constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
 
-// Queues the vertices one pull round claimed: bit k of claimed[d] is vertex
-// base + k * blockDim.x + threadIdx.x.  One prefix sum and one atomicAdd per
-// direction and workgroup; the only barriers of the round.  All work-items call this.
-__device__ inline void pull_commit(const ReachArgs &a, int64_t base, const uint32_t (&claimed)[2],
+// Queues the vertices one sweep round claimed (pull visits, trims): bit k of
+// claimed[d] is vertex base + k * blockDim.x + threadIdx.x.  One prefix sum
+// and one atomicAdd per direction and workgroup; the only barriers of the
+// round.  All work-items call this.
+__device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint32_t (&claimed)[2],
                                    const unsigned long long (&arcs)[2], uint32_t *shw, uint32_t *sbase,
-                                   unsigned long long *sh) {
+                                   unsigned long long *sh, bool trims = false) {
     const uint32_t c0 = __popc(claimed[0]), c1 = __popc(claimed[1]);
     uint32_t total;  // packed counts: a workgroup claims at most 16 * 1024 < 2^16 per direction
     const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total);
@@ -627,6 +606,7 @@ __device__ inline void pull_commit(const ReachArgs &a, int64_t base, const uint3
         sbase[1] = (total >> 16) ? atomicAdd(&a.ctl->qsize[1][slot], total >> 16) : 0u;
         if (ar0) atomicAdd(&a.ctl->qarcs[0][slot], ar0);
         if (ar1) atomicAdd(&a.ctl->qarcs[1][slot], ar1);
+        if (trims && (total & 0xFFFFu)) atomicSub(&a.ctl->live, total & 0xFFFFu);
     }
     __syncthreads();
     uint32_t o[2] = {sbase[0] + (pre & 0xFFFFu), sbase[1] + (pre >> 16)};
@@ -729,7 +709,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                 found = true;
                 cv += (unsigned long long)(__popc(claimed[0]) + __popc(claimed[1]));
             }
-            pull_commit(a, base, claimed, arcs, it.shw, st.base, sh);
+            commit_round(a, base, claimed, arcs, it.shw, st.base, sh);
         }
     }
     local_rounds(a, (pull0 || pull1) ? 0xFFFFFFFFu : qs0 + qs1, st, lq, found, cx, ce, cv);
@@ -741,9 +721,10 @@ __global__ void k_reach_hybrid(ReachArgs a) {
 // ---------------------------------------------------------------------------
 // data-driven trim within colours (trim_kernel, scc_routines.cu:336-394, run to
 // its fixpoint by trim(), :616-643): the fixpoint is unique, so instead of
-// re-sweeping every live vertex per iteration, level 0 sweeps once and every
-// later level re-checks only the live same-colour neighbours of the vertices
-// trimmed in the level before (each at most once per level, by stamp).
+// re-sweeping every live vertex per iteration, level 0 sweeps once and a later
+// level re-checks only the live same-colour neighbours of the vertices trimmed
+// in the level before (each at most once per level, by stamp) -- unless that
+// level trimmed so many that sweeping again is cheaper.
 // ---------------------------------------------------------------------------
 __device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                                const uint32_t *word, int32_t v, uint32_t c) {
@@ -764,84 +745,136 @@ __device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c) {
     return true;
 }
 
+constexpr int TRIM_V = 16;           // vertices per work-item per round of a trim sweep
+constexpr int32_t TRIM_SWEEP_DIV = 16;  // a level after one that trimmed >= n/16 vertices sweeps
+
+// Fresh words (every vertex live, arcs inside colours): v keeps a live
+// same-colour neighbour in a direction iff its row there holds a vertex other
+// than itself (rows are duplicate-free, so two entries suffice).  Returns the
+// round's trimmed vertices as a mask (bit k: base + k * blockDim.x + threadIdx.x).
+__device__ inline uint32_t trim_round_fresh(const ReachArgs &a, int64_t base, int64_t end) {
+    int32_t o0[TRIM_V], od[TRIM_V], i0[TRIM_V], id[TRIM_V];
+#pragma unroll
+    for (int k = 0; k < TRIM_V; k++) {
+        int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+        o0[k] = i0[k] = 0;
+        od[k] = id[k] = 2;  // out of range: never trimmed
+        if (vv < end) {
+            o0[k] = a.row[vv];
+            od[k] = a.row[vv + 1] - o0[k];
+            i0[k] = a.rowT[vv];
+            id[k] = a.rowT[vv + 1] - i0[k];
+        }
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < TRIM_V; k++) {
+        int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+        int32_t uo = od[k] == 1 ? a.col[o0[k]] : -1, ui = id[k] == 1 ? a.colT[i0[k]] : -1;
+        bool out = od[k] > 1 || (od[k] == 1 && uo != v);
+        bool in = id[k] > 1 || (id[k] == 1 && ui != v);
+        if (!(out && in)) m |= 1u << k;
+    }
+    uint32_t w[TRIM_V];
+#pragma unroll
+    for (int k = 0; k < TRIM_V; k++)
+        w[k] = ((m >> k) & 1u) ? a.word[base + (int64_t)k * blockDim.x + threadIdx.x] : 0u;
+#pragma unroll
+    for (int k = 0; k < TRIM_V; k++) {
+        if (!((m >> k) & 1u)) continue;
+        int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+        a.word[v] = make_word(colour_of(w[k]), 28);  // P|T|E (scc_routines.cu:381)
+        a.root[v] = v;
+    }
+    return m;
+}
+
+// General words: the live vertices of the round scan their out- and in-rows
+// for a live same-colour neighbour (other than themselves), SCAN_ILP vertices
+// at a time in lockstep; those missing one in either direction are trimmed.
+__device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int64_t end,
+                                           unsigned long long &ce) {
+    uint32_t cand = 0;
+#pragma unroll
+    for (int k = 0; k < TRIM_V; k++) {
+        int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+        uint32_t w = vv < end ? a.word[vv] : ST_E;
+        if (!(w & ST_E)) cand |= 1u << k;
+    }
+    constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, out-row (d = 0) or in-row
+    uint32_t trimmed = 0;
+    while (cand) {  // per work-item
+        int32_t v[SCAN_ILP];
+        int kk[SCAN_ILP];
+#pragma unroll
+        for (int j = 0; j < SCAN_ILP; j++) {
+            kk[j] = cand ? __ffs(cand) - 1 : -1;
+            cand &= cand ? cand - 1 : 0u;
+            v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
+        }
+        uint32_t w[SCAN_ILP];
+#pragma unroll
+        for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
+        const int32_t *col[T];
+        int32_t e[T], e1[T], self[T];
+        uint32_t mask[T], val[T], c[T], act = 0;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int d = t / SCAN_ILP, j = t % SCAN_ILP;
+            const int32_t *row = d == 0 ? a.row : a.rowT;
+            col[t] = d == 0 ? a.col : a.colT;
+            mask[t] = ST_E;
+            val[t] = 0;
+            c[t] = colour_of(w[j]);
+            self[t] = v[j];
+            e[t] = e1[t] = 0;
+            if (!(w[j] & ST_E)) {
+                e[t] = row[v[j]];
+                e1[t] = row[v[j] + 1];
+                if (e[t] < e1[t]) act |= 1u << t;
+            }
+        }
+        // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
+        uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
+#pragma unroll
+        for (int j = 0; j < SCAN_ILP; j++) {
+            if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
+                a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
+                a.root[v[j]] = v[j];
+                trimmed |= 1u << kk[j];
+            }
+        }
+    }
+    return trimmed;
+}
+
 __global__ void k_trim_level(ReachArgs a) {
     __shared__ Items it;
     __shared__ Stage st;
+    __shared__ unsigned long long sh[16];
     if (!level_begin(a, true)) return;
     st_init(st);
     __syncthreads();
     bool found = false;
-    if (a.iter == 0 && a.fresh) {
-        // every neighbour is live and of the same colour: a vertex keeps a live
-        // same-colour neighbour in a direction iff its row there holds a vertex
-        // other than itself (rows are duplicate-free: two entries suffice)
+    const uint32_t qs = a.iter == 0 ? 0u : a.ctl->qsize[0][a.iter & 3];
+    if (a.iter == 0 || qs >= (uint32_t)(a.n / TRIM_SWEEP_DIV)) {
+        // sweep: rounds of TRIM_V vertices per work-item, no barrier until the
+        // round's trims are queued
         int64_t beg, end;
         my_range(a.n, beg, end);
-        for (int64_t base = beg; base < end; base += blockDim.x) {  // block-uniform
-            int64_t vv = base + threadIdx.x;
-            if (vv < end) {
-                int32_t v = (int32_t)vv;
-                int32_t o0 = a.row[v], o1 = a.row[v + 1], i0 = a.rowT[v], i1 = a.rowT[v + 1];
-                bool out = o1 - o0 > 1 || (o1 - o0 == 1 && a.col[o0] != v);
-                bool in = i1 - i0 > 1 || (i1 - i0 == 1 && a.colT[i0] != v);
-                if (!(out && in)) {
-                    a.word[v] = make_word(colour_of(a.word[v]), 28);  // P|T|E (scc_routines.cu:381)
-                    a.root[v] = v;
-                    found = true;
-                    st_push(st, 0, v, a);
-                }
-            }
-            st_maybe_flush(st, a);
-        }
-    } else if (a.iter == 0) {
-        int64_t beg, end;
-        my_range(a.n, beg, end);
-        constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: vertex j, out-row (d = 0) or in-row
         unsigned long long ce = 0;
-        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * SCAN_ILP) {  // block-uniform
-            int32_t v[SCAN_ILP];
-            uint32_t w[SCAN_ILP];
-#pragma unroll
-            for (int j = 0; j < SCAN_ILP; j++) {
-                int64_t vv = base + (int64_t)j * blockDim.x + threadIdx.x;
-                v[j] = vv < end ? (int32_t)vv : -1;
-            }
-#pragma unroll
-            for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
-            const int32_t *col[T];
-            int32_t e[T], e1[T], self[T];
-            uint32_t mask[T], val[T], c[T], act = 0;
-#pragma unroll
-            for (int t = 0; t < T; t++) {
-                const int d = t / SCAN_ILP, j = t % SCAN_ILP;
-                const int32_t *row = d == 0 ? a.row : a.rowT;
-                col[t] = d == 0 ? a.col : a.colT;
-                mask[t] = ST_E;
-                val[t] = 0;
-                c[t] = colour_of(w[j]);
-                self[t] = v[j];
-                e[t] = e1[t] = 0;
-                if (!(w[j] & ST_E)) {
-                    e[t] = row[v[j]];
-                    e1[t] = row[v[j] + 1];
-                    if (e[t] < e1[t]) act |= 1u << t;
-                }
-            }
-            // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
-            uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
-#pragma unroll
-            for (int j = 0; j < SCAN_ILP; j++) {
-                if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
-                    a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
-                    a.root[v[j]] = v[j];
-                    found = true;
-                    st_push(st, 0, v[j], a);
-                }
-                st_maybe_flush(st, a);
-            }
+        const unsigned long long none[2] = {0ull, 0ull};
+        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * TRIM_V) {  // block-uniform
+            uint32_t m = (a.iter == 0 && a.fresh) ? trim_round_fresh(a, base, end)
+                                                  : trim_round_scan(a, base, end, ce);
+            if (m) found = true;
+            const uint32_t claimed[2] = {m, 0u};
+            commit_round(a, base, claimed, none, it.shw, st.base, sh, true);
         }
     } else {
-        uint32_t qs = a.ctl->qsize[0][a.iter & 3];
+        // data-driven: re-check the live same-colour neighbours of the vertices
+        // the previous level trimmed, each at most once (stamp = this level's tag)
+        unsigned long long ntrim = 0;
         uint32_t chunk = (qs + gridDim.x - 1) / gridDim.x;
         uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < qs ? b0 + chunk : qs;
         for (uint32_t base = b0; base < b1; base += blockDim.x) {
@@ -879,12 +912,15 @@ __global__ void k_trim_level(ReachArgs a) {
                         a.stamp[u] != a.tag &&  // plain load first: hubs are reached by many arcs
                         atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
                         found = true;
+                        ntrim++;
                         st_push(st, 0, u, a);
                     }
                 }
                 st_maybe_flush(st, a);
             }
         }
+        ntrim = block_sum(ntrim, sh);
+        if (threadIdx.x == 0 && ntrim) atomicSub(&a.ctl->live, (uint32_t)ntrim);
     }
     st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
