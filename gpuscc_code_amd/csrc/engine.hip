@@ -58,7 +58,7 @@ struct scc_ctx {
     int adam_t = 0;
     int cur_step = 0;
     bool uploaded = false, labels_valid = false;
-    int guess[3] = {4, 4, 2};     // chain batch guesses: fwd, bwd, trim
+    int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
     int64_t syncs = 0;
     int instr = 0;                  // profile flag 1: device work counters
     int prof_flags = 0;
@@ -230,7 +230,7 @@ scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false) {
             lt.ev_stop = e1;
             launch_trim_level(lt, a);
         },
-        c->guess[2], 2, iters, 1);
+        c->guess[fresh ? 3 : 2], 2, iters, 1);
 }
 
 // One reachability chain for the directions in `dirs` (1 = F, 2 = B, 3 = both).
@@ -685,7 +685,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     if (ne == n) { st.pivots_total += npiv; st.ms_update += ms_since(tp); break; }
                     wcc_done = true;
                     st.wcc_step = st.steps;
-                    launch_unset_pivots(l, c->d_word, n);
+                    launch_unset_pivots(l, c->d_word, n, c->d_ctl);
                     if (variant == SCC_VARIANT_2W) {
                         int64_t pairs = 0;
                         CHK(do_trim2(c, &pairs));
@@ -699,9 +699,10 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             }
             st.ms_update += ms_since(tp);
         }
-        // canonical labels + #SCC (scc.cu:912-919)
+        // canonical labels + #SCC (scc.cu:912-919): the device counts P bits as
+        // they are set, and the last wait above read the count
         launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
-        CHK(do_count(c, 1, &st.num_scc));
+        st.num_scc = (int64_t)c->h_ctl->pcount;
         return SCC_OK;
     };
     s = body();

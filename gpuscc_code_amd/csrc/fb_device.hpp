@@ -47,6 +47,7 @@ struct Ctl {
     uint32_t live;         // vertices not eliminated (set by k_init; trims, pair trims and
                            // FB eliminations subtract)
     uint32_t first_piv;    // lowest pivot of the last election
+    uint32_t pcount;       // vertices holding P (class representatives: #SCC at the end)
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters

@@ -80,7 +80,7 @@ void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned l
                     int slot_bits, int32_t *piv, Ctl *ctl);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl);
-void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n);
+void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl);
 void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl);
 void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n);
 void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,

@@ -25,7 +25,10 @@ __device__ inline uint32_t graph_of(const int32_t *off, int32_t G, int32_t v) {
 // init: colour 1 (thrust::fill, scc.cu:214) or, for a batch, graph id + 1;
 // state 0; root = self
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G, Ctl *ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->live = (uint32_t)n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->live = (uint32_t)n;
+        ctl->pcount = 0;
+    }
     int64_t beg, end;
     my_range(n, beg, end);
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
@@ -238,7 +241,10 @@ __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *_
         uint32_t total;
         uint32_t pre = block_excl_scan((uint32_t)__popc(won), shw, total);
         if (total == 0) continue;  // block-uniform
-        if (threadIdx.x == 0) sbase = atomicAdd(&ctl->npiv, total);
+        if (threadIdx.x == 0) {
+            sbase = atomicAdd(&ctl->npiv, total);
+            atomicAdd(&ctl->pcount, total);  // live vertices hold no P before they win
+        }
         __syncthreads();
         uint32_t o = sbase + pre;
         for (uint32_t m = won; m; m &= m - 1)
@@ -283,13 +289,20 @@ __global__ void k_max_colour(const uint32_t *word, int32_t n, Ctl *ctl) {
 }
 
 // state 19 -> 0: unsetPivots_kernel (scc_routines.cu:10-19)
-__global__ void k_unset_pivots(uint32_t *word, int32_t n) {
+__global__ void k_unset_pivots(uint32_t *word, int32_t n, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
     int64_t beg, end;
     my_range(n, beg, end);
+    unsigned long long unset = 0;
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
         uint32_t w = word[v];
-        if ((w & 0xFFu) == 19u) word[v] = w & ~0xFFu;
+        if ((w & 0xFFu) == 19u) {
+            word[v] = w & ~0xFFu;
+            unset++;
+        }
     }
+    unset = block_sum(unset, sh);
+    if (threadIdx.x == 0 && unset) atomicSub(&ctl->pcount, (uint32_t)unset);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,6 +365,7 @@ __global__ void k_trim2_commit(uint32_t *word, int32_t n, Ctl *ctl) {
     if (threadIdx.x == 0 && pairs) {
         atomicAdd(&ctl->count, pairs);
         atomicSub(&ctl->live, (uint32_t)(2 * pairs));
+        atomicAdd(&ctl->pcount, (uint32_t)pairs);
     }
 }
 
@@ -506,8 +520,8 @@ void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int wh
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl) {
     hipLaunchKernelGGL(k_max_colour, GRID(l), word, n, ctl);
 }
-void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n) {
-    hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n);
+void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl) {
+    hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n, ctl);
 }
 void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl) {
     hipLaunchKernelGGL(k_trim2_mark, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n);

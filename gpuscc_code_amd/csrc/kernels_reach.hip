@@ -594,12 +594,13 @@ constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
 // round.  All work-items call this.
 __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint32_t (&claimed)[2],
                                    const unsigned long long (&arcs)[2], uint32_t *shw, uint32_t *sbase,
-                                   unsigned long long *sh, bool trims = false) {
+                                   unsigned long long *sh, bool trims = false, uint32_t newp = 0) {
     const uint32_t c0 = __popc(claimed[0]), c1 = __popc(claimed[1]);
     uint32_t total;  // packed counts: a workgroup claims at most 16 * 1024 < 2^16 per direction
     const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total);
     unsigned long long ar0 = block_sum(arcs[0], sh);
     unsigned long long ar1 = block_sum(arcs[1], sh);
+    unsigned long long np = trims ? block_sum((unsigned long long)newp, sh) : 0ull;
     if (threadIdx.x == 0) {
         const int slot = (a.iter + 1) & 3;
         sbase[0] = (total & 0xFFFFu) ? atomicAdd(&a.ctl->qsize[0][slot], total & 0xFFFFu) : 0u;
@@ -607,6 +608,7 @@ __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint
         if (ar0) atomicAdd(&a.ctl->qarcs[0][slot], ar0);
         if (ar1) atomicAdd(&a.ctl->qarcs[1][slot], ar1);
         if (trims && (total & 0xFFFFu)) atomicSub(&a.ctl->live, total & 0xFFFFu);
+        if (np) atomicAdd(&a.ctl->pcount, (uint32_t)np);
     }
     __syncthreads();
     uint32_t o[2] = {sbase[0] + (pre & 0xFFFFu), sbase[1] + (pre >> 16)};
@@ -793,7 +795,7 @@ __device__ inline uint32_t trim_round_fresh(const ReachArgs &a, int64_t base, in
 // for a live same-colour neighbour (other than themselves), SCAN_ILP vertices
 // at a time in lockstep; those missing one in either direction are trimmed.
 __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int64_t end,
-                                           unsigned long long &ce) {
+                                           unsigned long long &ce, uint32_t &newp) {
     uint32_t cand = 0;
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++) {
@@ -842,6 +844,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
                 a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
                 a.root[v[j]] = v[j];
                 trimmed |= 1u << kk[j];
+                newp += (w[j] & ST_P) ? 0u : 1u;  // a trimmed pivot holds P already
             }
         }
     }
@@ -865,16 +868,22 @@ __global__ void k_trim_level(ReachArgs a) {
         unsigned long long ce = 0;
         const unsigned long long none[2] = {0ull, 0ull};
         for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * TRIM_V) {  // block-uniform
-            uint32_t m = (a.iter == 0 && a.fresh) ? trim_round_fresh(a, base, end)
-                                                  : trim_round_scan(a, base, end, ce);
+            uint32_t newp = 0;
+            uint32_t m;
+            if (a.iter == 0 && a.fresh) {
+                m = trim_round_fresh(a, base, end);
+                newp = (uint32_t)__popc(m);  // fresh words hold no P
+            } else {
+                m = trim_round_scan(a, base, end, ce, newp);
+            }
             if (m) found = true;
             const uint32_t claimed[2] = {m, 0u};
-            commit_round(a, base, claimed, none, it.shw, st.base, sh, true);
+            commit_round(a, base, claimed, none, it.shw, st.base, sh, true, newp);
         }
     } else {
         // data-driven: re-check the live same-colour neighbours of the vertices
         // the previous level trimmed, each at most once (stamp = this level's tag)
-        unsigned long long ntrim = 0;
+        unsigned long long ntrim = 0, newp = 0;
         uint32_t chunk = (qs + gridDim.x - 1) / gridDim.x;
         uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < qs ? b0 + chunk : qs;
         for (uint32_t base = b0; base < b1; base += blockDim.x) {
@@ -913,6 +922,7 @@ __global__ void k_trim_level(ReachArgs a) {
                         atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
                         found = true;
                         ntrim++;
+                        newp += (wu & ST_P) ? 0u : 1u;  // a trimmed pivot holds P already
                         st_push(st, 0, u, a);
                     }
                 }
@@ -920,7 +930,9 @@ __global__ void k_trim_level(ReachArgs a) {
             }
         }
         ntrim = block_sum(ntrim, sh);
+        newp = block_sum(newp, sh);
         if (threadIdx.x == 0 && ntrim) atomicSub(&a.ctl->live, (uint32_t)ntrim);
+        if (threadIdx.x == 0 && newp) atomicAdd(&a.ctl->pcount, (uint32_t)newp);
     }
     st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
