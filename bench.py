@@ -60,6 +60,16 @@ def traffic_for(args):
     return t["bytes_per_launch"]
 
 
+def write_levels(path, workload, rows):
+    """Per-level table of one decomposition (profile flag 4 of the library)."""
+    with open(path, "w") as f:
+        f.write(f"# {workload}\n")
+        f.write("\t".join(scc.LEVEL_FIELDS) + "\n")
+        for r in rows:
+            f.write("\t".join(f"{x:.1f}" if i == 2 else f"{int(x)}"
+                              for i, x in enumerate([r[0], r[1], r[2] * 1e3] + list(r[3:]))) + "\n")
+
+
 def cpu_tarjan_baseline(g, reps=3):
     import oracle
     try:
@@ -103,12 +113,17 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_reach_hybrid_rmat22_traffic.json"),
                     metavar="JSON",
                     help="measured HBM bytes per launch of the level kernel (from rocprofv3 counters)")
+    ap.add_argument("--levels", default=None, metavar="FILE",
+                    help="also write the per-level log of one decomposition (kernel time, frontier, "
+                         "push/pull, work) to FILE")
     ap.add_argument("--grid-blocks", type=int, default=0,
                     help="workgroups of the persistent kernels (0 = 4 per CU)")
     ap.add_argument("--pull-arc-div", type=int, default=0,
                     help="hybrid: frontier arcs >= m / DIV also pull (0 = 14, -1 = off)")
     ap.add_argument("--pull-threshold", type=int, default=0,
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
+    ap.add_argument("--no-tuned", action="store_true",
+                    help="library defaults instead of the autotuned options (gpuscc_code_amd/tuned.json)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -117,7 +132,7 @@ def main():
 
     import numpy as np
 
-    from gpuscc_code_amd import scc
+    from gpuscc_code_amd import autotune, scc
 
     t0 = time.time()
     if args.workload == "rmat":
@@ -142,6 +157,16 @@ def main():
     if args.workload == "batch":
         ctx.set_batch(g.graph_offsets)
         num_graphs = len(g.graph_offsets) - 1
+    # autotuned options for this GPU and graph class, unless any knob is given by hand
+    tuned_key, opts = None, {}
+    manual = args.sequential or args.grid_blocks or args.pull_arc_div or args.pull_threshold
+    if not args.no_tuned and not manual:
+        arch, _ = ctx.device_info()
+        opt, tuned_key = autotune.opts_for(g, arch, args.schedule, args.variant,
+                                           batch=args.workload == "batch")
+        if tuned_key:
+            ctx.set_opts(opt)
+            opts = autotune.load_table()[arch][tuned_key]["opts"]
 
     if args.calibrate_copy:
         ctx.profile_copy(args.calibrate_copy << 20)
@@ -173,6 +198,12 @@ def main():
 
     ms_per_step = t_total * 1e3 / args.steps
     mteps = g.m / (ms_per_step * 1e3)
+
+    if args.levels:  # one more decomposition, levels launched one at a time
+        ctx.profile(4)
+        ctx.run(args.schedule, args.variant, labels=False)
+        write_levels(args.levels, wl, ctx.profile_levels())
+        ctx.profile(0)
 
     # correctness of the measured configuration
     labels, st_check = ctx.run(args.schedule, args.variant)
@@ -206,7 +237,8 @@ def main():
         "config": {"workload": wl, "graphs": num_graphs,
                    "n": g.n, "m": g.m, "schedule": args.schedule,
                    "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
-                   "reach": "sequential" if args.sequential else "fused"},
+                   "reach": "sequential" if args.sequential else "fused",
+                   "tuned": tuned_key, "opts": opts},
         "bit_exact": bit_exact, "num_scc": st_check["num_scc"], "biggest_scc": tst["biggest"],
         "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],
         "trim_iters": last["trim_iters"], "host_syncs": last["host_syncs"],

@@ -79,6 +79,8 @@ _st = ctypes.c_int
 SIGNATURES = {
     "scc_create": (_st, [ctypes.c_int, ctypes.POINTER(SccOpts), ctypes.POINTER(_vp)]),
     "scc_destroy": (None, [_vp]),
+    "scc_set_opts": (_st, [_vp, ctypes.POINTER(SccOpts)]),
+    "scc_device_info": (_st, [_vp, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "scc_status_string": (ctypes.c_char_p, [_st]),
     "scc_last_error": (ctypes.c_char_p, [_vp]),
     "scc_upload": (_st, [_vp, ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, _i32p, _i32p]),

@@ -27,6 +27,7 @@ struct scc_ctx {
     scc_opts opts{};
     std::string err;
     int cu_count = 256;
+    char arch[64] = "unknown";
     Graph g{};
     int32_t *d_row = nullptr, *d_col = nullptr, *d_rowT = nullptr, *d_colT = nullptr;
     uint32_t *d_word = nullptr;
@@ -482,8 +483,15 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
     }
     if (hipSetDevice(device) != hipSuccess) { delete c; return SCC_E_HIP; }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        c->cu_count = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        if (prop.multiProcessorCount > 0) c->cu_count = prop.multiProcessorCount;
+        size_t k = 0;  // "gfx950:sramecc+:xnack-" -> "gfx950"
+        while (k + 1 < sizeof(c->arch) && prop.gcnArchName[k] && prop.gcnArchName[k] != ':') {
+            c->arch[k] = prop.gcnArchName[k];
+            k++;
+        }
+        c->arch[k] = 0;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SCC_E_HIP;
@@ -499,6 +507,28 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
         return SCC_E_HIP;
     }
     *out = c;
+    return SCC_OK;
+}
+
+scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
+    if (!c || !opts) return SCC_E_INPUT;
+    if (opts->block_size != 0 &&
+        (opts->block_size < 64 || opts->block_size > 1024 || (opts->block_size & 63) != 0))
+        return fail(c, SCC_E_INPUT, "block_size must be a multiple of 64 in [64, 1024]");
+    if (opts->grid_blocks < 0) return fail(c, SCC_E_INPUT, "grid_blocks < 0");
+    if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
+        return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
+    c->opts = *opts;
+    return SCC_OK;
+}
+
+scc_status scc_device_info(const scc_ctx *c, char *arch, int32_t len, int32_t *cu_count) {
+    if (!c || (len > 0 && !arch)) return SCC_E_INPUT;
+    if (len > 0) {
+        std::strncpy(arch, c->arch, (size_t)len - 1);
+        arch[len - 1] = 0;
+    }
+    if (cu_count) *cu_count = c->cu_count;
     return SCC_OK;
 }
 

@@ -161,6 +161,18 @@ class Context:
     def _chk(self, st):
         check(st, self._h)
 
+    def set_opts(self, opt: alg_opt):
+        """New run-time options for the next runs (slot_bits stays fixed once uploaded)."""
+        o = opt.to_c()
+        self._chk(lib().scc_set_opts(self._h, ctypes.byref(o)))
+
+    def device_info(self):
+        """(gfx target, compute units) of the context's device."""
+        buf = ctypes.create_string_buffer(64)
+        cus = ctypes.c_int32()
+        self._chk(lib().scc_device_info(self._h, buf, 64, ctypes.byref(cus)))
+        return buf.value.decode(), cus.value
+
     def upload(self, g: Graph, transpose=True):
         row = np.ascontiguousarray(g.row, np.int32)
         col = np.ascontiguousarray(g.col, np.int32)

@@ -111,6 +111,13 @@ typedef struct {
 /* ---- context ------------------------------------------------------------ */
 /* Replaces the device setup of computeSCC (scc.cu:199-215). */
 scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out);
+/* Changes the run-time options of a context (workgroup size, grid, hybrid
+ * thresholds, huge rows, local rounds, reach order) between runs; slot_bits
+ * is fixed once a graph is uploaded (0 keeps it).  No reference counterpart:
+ * alg_opt is fixed per process there (scc.cu:76-109); the autotuner needs it. */
+scc_status scc_set_opts(scc_ctx *ctx, const scc_opts *opts);
+/* Device of the context: gfx target name (e.g. "gfx950") and compute units. */
+scc_status scc_device_info(const scc_ctx *ctx, char *arch, int32_t len, int32_t *cu_count);
 void scc_destroy(scc_ctx *ctx);
 const char *scc_status_string(scc_status s);
 const char *scc_last_error(const scc_ctx *ctx);
