@@ -62,6 +62,7 @@ def traffic_for(args):
 
 def write_levels(path, workload, rows):
     """Per-level table of one decomposition (profile flag 4 of the library)."""
+    from gpuscc_code_amd import scc
     with open(path, "w") as f:
         f.write(f"# {workload}\n")
         f.write("\t".join(scc.LEVEL_FIELDS) + "\n")

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fb_device.hpp"
@@ -369,6 +370,31 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = fals
     return do_elect(c, 0, npiv, learned, sync);
 }
 
+// Hub-first rows: every row of the CSR and of the CSR^T lists its neighbours by
+// decreasing total degree (ties by id).  The order inside a row changes no
+// result; the scans that stop at the first live or reached neighbour (trim
+// sweeps, pull levels) meet the hubs first, and the hubs' words stay in cache.
+void hub_first_rows(int32_t n, int64_t m, const int32_t *row, const int32_t *col, const int32_t *rowT,
+                    const int32_t *colT, std::vector<int32_t> &oc, std::vector<int32_t> &ocT) {
+    std::vector<int32_t> deg((size_t)n);
+    for (int32_t v = 0; v < n; v++) deg[v] = (row[v + 1] - row[v]) + (rowT[v + 1] - rowT[v]);
+    oc.assign(col, col + m);
+    ocT.assign(colT, colT + m);
+    auto before = [&](int32_t a, int32_t b) { return deg[a] != deg[b] ? deg[a] > deg[b] : a < b; };
+    const int32_t T = (int32_t)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int32_t t = 0; t < T; t++) {
+        pool.emplace_back([&, t] {
+            int32_t lo = (int32_t)((int64_t)n * t / T), hi = (int32_t)((int64_t)n * (t + 1) / T);
+            for (int32_t v = lo; v < hi; v++) {
+                std::sort(oc.begin() + row[v], oc.begin() + row[v + 1], before);
+                std::sort(ocT.begin() + rowT[v], ocT.begin() + rowT[v + 1], before);
+            }
+        });
+    }
+    for (auto &th : pool) th.join();
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -579,9 +605,17 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
         graph_free(&tg);
         rowT = hrT.data();
         colT = hcT.data();
-    } else if (rowT[0] != 0 || rowT[n] != m) {
-        return fail(c, SCC_E_INPUT, "rowT[0] != 0 or rowT[n] != m");
+    } else {
+        if (rowT[0] != 0 || rowT[n] != m) return fail(c, SCC_E_INPUT, "rowT[0] != 0 or rowT[n] != m");
+        for (int32_t v = 0; v < n; v++)
+            if (rowT[v + 1] < rowT[v]) return fail(c, SCC_E_INPUT, "rowT offsets decrease");
+        for (int64_t i = 0; i < m; i++)
+            if (colT[i] < 0 || colT[i] >= n) return fail(c, SCC_E_INPUT, "colT id out of range");
     }
+    std::vector<int32_t> hub_col, hub_colT;
+    hub_first_rows(n, m, row, col, rowT, colT, hub_col, hub_colT);
+    col = hub_col.data();
+    colT = hub_colT.data();
     size_t nb = sizeof(int32_t) * ((size_t)n + 1), mb = sizeof(int32_t) * std::max<size_t>((size_t)m, 1);
     size_t vb = sizeof(int32_t) * std::max<size_t>((size_t)n, 1);
     HIPCHK(c, hipMalloc(&c->d_row, nb));
