@@ -188,6 +188,7 @@ def evaluate(params, graphs, schedules=("h",), variants=(0,), reps=3):
         ctx.mlp_set(params)
         for name, g in graphs:
             ctx.upload(g)
+            ctx.set_batch(g.graph_offsets if name.startswith("batch") else None)
             for sched in schedules:
                 for var in variants:
                     res = {}
@@ -222,6 +223,8 @@ def main():
     e = sub.add_parser("evaluate")
     e.add_argument("--params", default=PARAMS_PATH)
     e.add_argument("--s24", action="store_true", help="include R-MAT scale 24, edge factor 16, seed 5")
+    e.add_argument("--batch", action="store_true", help="include the batch of 4,096 graphs (seed 4)")
+    e.add_argument("--variants", nargs="+", type=int, default=[0, 1])
     args = ap.parse_args()
     if args.cmd == "train":
         params, info = train(n_graphs=args.graphs, epochs=args.epochs, seed=args.seed)
@@ -233,7 +236,9 @@ def main():
                   ("rmat22", scc.gen_rmat(22, 8, seed=2))]
         if args.s24:
             graphs.append(("rmat24ef16", scc.gen_rmat(24, 16, seed=5)))
-        rows = evaluate(params, graphs, schedules=("h", "d"), variants=(0, 1))
+        if args.batch:
+            graphs.append(("batch4096", scc.gen_batch(4096, 64, 65536, 1.5, 6.0, 0.05, 0.5, seed=4)))
+        rows = evaluate(params, graphs, schedules=("h", "d"), variants=tuple(args.variants))
         print(json.dumps(rows))
 
 
