@@ -669,7 +669,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                    scc_stats *stats) {
     if (!c) return SCC_E_INPUT;
     if (!c->uploaded) return fail(c, SCC_E_STATE, "scc_run before scc_upload");
-    if (schedule < SCC_SCHED_Q || schedule > SCC_SCHED_H) return fail(c, SCC_E_INPUT, "bad schedule");
+    if (schedule < SCC_SCHED_Q || schedule > SCC_SCHED_B) return fail(c, SCC_E_INPUT, "bad schedule");
     if (variant < SCC_VARIANT_PLAIN || variant > SCC_VARIANT_2W) return fail(c, SCC_E_INPUT, "bad variant");
     if (pivot_rule != SCC_PIVOT_PLAIN && pivot_rule != SCC_PIVOT_LEARNED)
         return fail(c, SCC_E_INPUT, "bad pivot rule");
@@ -677,6 +677,15 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return fail(c, SCC_E_STATE, "learned pivot rule needs parameters (scc_mlp_set_params or scc_mlp_train)");
     const bool learned = pivot_rule == SCC_PIVOT_LEARNED;
     HIPCHK(c, hipSetDevice(c->device));
+    struct Restore {  // BEST changes the hybrid thresholds for this run only
+        scc_ctx *c;
+        scc_opts o;
+        ~Restore() { c->opts = o; }
+    } restore{c, c->opts};
+    if (schedule == SCC_SCHED_B) {  // alpha = 15, beta = 18 (reach_b.cu:540,645) unless set
+        if (c->opts.hybrid_arc_div == 0) c->opts.hybrid_arc_div = 15;
+        if (c->opts.hybrid_threshold == 0) c->opts.hybrid_threshold = (int32_t)std::max<int64_t>(1024, c->g.n / 18);
+    }
     scc_stats st;
     std::memset(&st, 0, sizeof(st));
     st.first_pivot = -1;
@@ -708,11 +717,13 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         }
         bool wcc_done = false;
         const int64_t one_pct = n / 100;
+        // schedules that switch once n/100 vertices are FB-eliminated: the
+        // reference's HYBRID (t, then q; scc.cu:493-563) and COMBINE (l, then q)
+        const bool by_elim = (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1) || schedule == SCC_SCHED_C;
         while (unsynced || npiv > 0) {
             st.steps++;
-            int sched = schedule;
-            if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1)
-                sched = ne < one_pct ? SCC_SCHED_T : SCC_SCHED_Q;  // scc.cu:493-563
+            int sched = schedule == SCC_SCHED_B ? SCC_SCHED_H : schedule;
+            if (by_elim) sched = ne < one_pct ? (schedule == SCC_SCHED_C ? SCC_SCHED_L : SCC_SCHED_T) : SCC_SCHED_Q;
             int64_t d = 0;
             tp = std::chrono::steady_clock::now();
             if (c->opts.sequential_reach) {  // scc.cu:285-704 order: forward, then backward
@@ -758,7 +769,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     int64_t wi = 0;
                     CHK(do_wcc(c, &wi, &npiv, learned));
                 }
-            } else if (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1 && ne < one_pct) {
+            } else if (by_elim && ne < one_pct) {
                 CHK(do_count(c, 0, &ne));  // scc.cu:879-890
             }
             st.ms_update += ms_since(tp);

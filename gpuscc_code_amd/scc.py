@@ -16,8 +16,8 @@ from . import _lib
 from ._lib import SccError, SccHostGraph, SccOpts, SccStats, check, lib
 
 # reach_strategy (scc.h:7-16) restricted to the schedules this library provides
-QUADRATIC, TOPO, DATA, LINEAR, HYBRID = 0, 1, 2, 3, 4
-SCHEDULES = {"q": QUADRATIC, "t": TOPO, "d": DATA, "l": LINEAR, "h": HYBRID}
+QUADRATIC, TOPO, DATA, LINEAR, HYBRID, COMBINE, BEST = 0, 1, 2, 3, 4, 5, 6
+SCHEDULES = {"q": QUADRATIC, "t": TOPO, "d": DATA, "l": LINEAR, "h": HYBRID, "c": COMBINE, "b": BEST}
 VARIANTS = {"scc": 0, "scc_w": 1, "scc_2_w": 2}
 PIVOT_PLAIN, PIVOT_LEARNED = 0, 1
 
@@ -333,5 +333,5 @@ def split_word(words):
 
 
 __all__ = ["alg_opt", "Graph", "Context", "computeSCC", "load", "csr_from_arcs", "gen_rmat",
-           "gen_mesh", "gen_batch", "SccError", "QUADRATIC", "TOPO", "DATA", "LINEAR", "HYBRID",
+           "gen_mesh", "gen_batch", "SccError", "QUADRATIC", "TOPO", "DATA", "LINEAR", "HYBRID", "COMBINE", "BEST",
            "SCHEDULES", "VARIANTS", "make_word", "split_word"]

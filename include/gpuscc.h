@@ -40,7 +40,10 @@ typedef enum {
     SCC_SCHED_T = 1,  /* topology sweep with workgroup edge balancing (reach_t.cu) */
     SCC_SCHED_D = 2,  /* data-driven frontier queues (reach_d.cu) */
     SCC_SCHED_L = 3,  /* frontier queues with load-balanced edge expansion (reach_l.cu) */
-    SCC_SCHED_H = 4   /* hybrid: switches schedule by frontier size (scc.cu:493-563) */
+    SCC_SCHED_H = 4,  /* hybrid: switches schedule by frontier size (scc.cu:493-563) */
+    SCC_SCHED_C = 5,  /* COMBINE: l until n/100 vertices are FB-eliminated, then q (scc.cu:565-638) */
+    SCC_SCHED_B = 6   /* BEST: direction-optimising reach (reach_b.cu:374-725): the hybrid level
+                         kernel pulling at frontier arcs >= m/15 or frontier vertices >= n/18 */
 } scc_schedule;
 
 /* The three binaries of Makefile:12-19. */

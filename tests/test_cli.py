@@ -30,8 +30,8 @@ def test_load_and_tarjan_lines(binary):
 def test_usage_and_bad_strategy():
     r = run("scc")
     assert r.returncode != 0 and "usage" in r.stdout
-    r = run("scc", "-a", "c", "-f", DEMO)
-    assert r.returncode != 0
+    r = run("scc", "-a", "x", "-f", DEMO)
+    assert r.returncode != 0 and "not provided" in r.stderr
 
 
 def test_missing_file_is_an_error():
@@ -41,7 +41,7 @@ def test_missing_file_is_an_error():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("binary", ["scc", "scc_w", "scc_2_w"])
-@pytest.mark.parametrize("alg", ["q", "t", "d", "l", "h"])
+@pytest.mark.parametrize("alg", ["q", "t", "d", "l", "h", "c", "b"])
 def test_gpu_run_lines(binary, alg):
     r = run(binary, "-t", "-a", alg, "-j", "-f", DEMO)
     assert r.returncode == 0, r.stderr
@@ -50,3 +50,31 @@ def test_gpu_run_lines(binary, alg):
     assert out.count("    ->count of SCC: 6") == 2  # Tarjan's and the GPU's
     assert "    ->steps:" in out and "    ->time:" in out
     assert '"num_scc": 6' in out
+
+
+@pytest.mark.gpu
+def test_gpu_all_strategies():
+    # -a a runs QUADRATIC..BEST in turn (scc.cu:128-139)
+    r = run("scc", "-a", "a", "-j", "-f", DEMO)
+    assert r.returncode == 0, r.stderr
+    for k in range(7):
+        assert f"computing SCC in parallel - alg: {k}..." in r.stdout
+    assert r.stdout.count("    ->count of SCC: 6") == 7
+    for name in ("QUADRATIC", "TOPO", "DATA", "LINEAR", "HYBRID", "COMBINE", "BEST"):
+        assert f"running {name} strategy" in r.stdout
+
+
+@pytest.mark.gpu
+def test_gpu_verify_files(tmp_path):
+    # VERIFY outputs (scc.cu:920-995): the demo graph's SCCs, 1-based
+    r = subprocess.run([os.path.join(BIN, "scc"), "-V", "-a", "h", "-f", DEMO], capture_output=True,
+                       text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    sccs = sorted(tuple(int(x) for x in line.split()) for line in (tmp_path / "parallel_scc.txt").read_text().splitlines())
+    assert sccs == sorted([(1,), (2, 5, 6), (3,), (4,), (7, 9, 10, 11, 12), (8, 13)])
+    assert sorted(int(x) for x in (tmp_path / "trivial.txt").read_text().split()) == [1, 3, 4]
+    dist = dict(tuple(map(int, line.split())) for line in (tmp_path / "scc_dist.txt").read_text().splitlines())
+    assert dist == {1: 3, 2: 1, 3: 1, 5: 1}
+    detail = (tmp_path / "state_range_root.txt").read_text().splitlines()
+    assert detail[0].startswith("ID") and len(detail) == 14
+    assert "number=6" in r.stdout

@@ -72,6 +72,18 @@ def test_rmat14_config(ctx, sched, variant):
     check_against_oracle(ctx, g, sched, variant)
 
 
+@pytest.mark.parametrize("sched", ["c", "b"])
+def test_combine_and_best(ctx, sched):
+    # COMBINE (l, then q after n/100 FB-eliminated; scc.cu:565-638) and BEST
+    # (direction-optimising thresholds): whole runs, every variant, full state
+    check_against_oracle(ctx, demo_graph(), sched, 0)
+    for g in (scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1), scc.gen_mesh(128, 0.6, seed=3),
+              scc.gen_batch(48, 64, 2048, seed=4)):
+        for variant in (0, 1, 2):
+            check_against_oracle(ctx, g, sched, variant)
+    ctx.set_batch(None)
+
+
 @pytest.mark.parametrize("sched", SCHEDS)
 def test_mesh_small(ctx, sched):
     g = scc.gen_mesh(128, 0.6, seed=3)
