@@ -67,6 +67,9 @@ struct scc_ctx {
     scc_profile_data prof{};
     std::vector<hipEvent_t> evs;    // flag 2: one start/stop pair per level launch
     std::vector<double> level_log;  // flag 4: SCC_LEVEL_FIELDS per level
+    // speculative steps: reach and trim chain lengths of step k in the last run
+    std::vector<int> spec_reach, spec_trim;
+    hipEvent_t spec_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -136,10 +139,10 @@ scc_status ensure_events(scc_ctx *c, size_t count) {
 // timing on (profile flag 2) the events bracket the kernel itself.
 // kind 0 = reachability, 1 = trim, 2 = other (for the profile counters).
 template <class F>
-scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int64_t *levels,
-                     int kind = 2) {
-    launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
-    int64_t launched = 0;
+scc_status run_chain(scc_ctx *c, F &&launch_level, int &guess, int reset_mask, int64_t *levels,
+                     int kind = 2, int64_t resume = 0) {
+    if (!resume) launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
+    int64_t launched = resume;
     const bool logged = (c->prof_flags & 4) && kind < 2;
     int L = logged ? 1 : std::max(1, std::min(guess, 64));
     const int64_t limit = (int64_t)c->g.n + 4;
@@ -198,98 +201,115 @@ scc_status run_chain(scc_ctx *c, F launch_level, int &guess, int reset_mask, int
     return SCC_OK;
 }
 
-// fresh: the words are as k_init left them (all live, arcs never leave a colour)
-scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false) {
-    Launch l = fixed_launch(c);
+// Level launchers of the two chains (one object per chain run).
+struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs never leave a colour)
+    scc_ctx *c;
+    Launch l;
     ReachArgs a{};
-    a.row = c->g.row;
-    a.col = c->g.col;
-    a.rowT = c->g.rowT;
-    a.colT = c->g.colT;
-    a.word = c->d_word;
-    a.n = c->g.n;
-    a.ctl = c->d_ctl;
-    a.root = c->d_root;
-    a.stamp = c->d_stamp;
-    a.fresh = fresh ? 1 : 0;
-    a.dirs = 1;
-    launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
-    return run_chain(
-        c,
-        [&](int it, hipEvent_t e0, hipEvent_t e1) {
-            if (++c->trim_tag == 0) {  // tag wrap: forget every stamp
-                launch_fill_i32(launch_for(c), (int32_t *)c->d_stamp, c->g.n, 0);
-                c->trim_tag = 1;
-            }
-            a.iter = it;
-            a.tag = c->trim_tag;
-            a.inq[0] = c->d_q[(it - 1) & 1];
-            a.outq[0] = c->d_q[it & 1];
-            a.inq[1] = a.inq[0];
-            a.outq[1] = c->d_q[2 + (it & 1)];
-            Launch lt = l;
-            lt.ev_start = e0;
-            lt.ev_stop = e1;
-            launch_trim_level(lt, a);
-        },
-        c->guess[fresh ? 3 : 2], 2, iters, 1);
-}
+    TrimChain(scc_ctx *cc, bool fresh) : c(cc), l(fixed_launch(cc)) {
+        a.row = c->g.row;
+        a.col = c->g.col;
+        a.rowT = c->g.rowT;
+        a.colT = c->g.colT;
+        a.word = c->d_word;
+        a.n = c->g.n;
+        a.ctl = c->d_ctl;
+        a.root = c->d_root;
+        a.stamp = c->d_stamp;
+        a.fresh = fresh ? 1 : 0;
+        a.dirs = 1;
+    }
+    void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
+        if (++c->trim_tag == 0) {  // tag wrap: forget every stamp
+            launch_fill_i32(launch_for(c), (int32_t *)c->d_stamp, c->g.n, 0);
+            c->trim_tag = 1;
+        }
+        a.iter = it;
+        a.tag = c->trim_tag;
+        a.inq[0] = c->d_q[(it - 1) & 1];
+        a.outq[0] = c->d_q[it & 1];
+        a.inq[1] = a.inq[0];
+        a.outq[1] = c->d_q[2 + (it & 1)];
+        Launch lt = l;
+        lt.ev_start = e0;
+        lt.ev_stop = e1;
+        launch_trim_level(lt, a);
+    }
+};
 
 // One reachability chain for the directions in `dirs` (1 = F, 2 = B, 3 = both).
 // Queue schedules start from the seed lists: d_piv (forward, ctl->seeds[0])
 // and bseed (backward, ctl->seeds[1]).
-scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, int64_t *depth) {
-    Launch l = sched == SCC_SCHED_Q ? launch_for(c) : fixed_launch(c);
-    int &guess = c->guess[dirs == 2 ? 1 : 0];
-    if (c->instr) launch_ctl_reset(c->stream, c->d_ctl, 32);
+struct ReachChain {
+    scc_ctx *c;
+    int sched;
+    const int32_t *bseed;
+    Launch l;
     ReachArgs a{};
-    a.row = c->g.row;
-    a.col = c->g.col;
-    a.rowT = c->g.rowT;
-    a.colT = c->g.colT;
-    a.word = c->d_word;
-    a.n = c->g.n;
-    a.ctl = c->d_ctl;
-    a.dirs = dirs;
-    a.instr = c->instr;
-    a.pull_thr = c->opts.hybrid_threshold > 0 ? (uint32_t)c->opts.hybrid_threshold
-                                              : (uint32_t)std::max<int64_t>(1024, c->g.n / 20);
-    a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
-                                      : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
-    a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
-    const int div = c->opts.hybrid_arc_div > 0 ? c->opts.hybrid_arc_div : 14;
-    a.pull_arcs = c->opts.hybrid_arc_div < 0 ? ~0ull : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
-    a.dcls[0] = c->d_dcls;
-    a.dcls[1] = c->d_dcls + c->g.n;
-    const bool queues = sched >= SCC_SCHED_D;
-    scc_status s = run_chain(
-        c,
-        [&](int it, hipEvent_t e0, hipEvent_t e1) {
-            a.iter = it;
-            for (int d = 0; d < 2; d++) {
-                a.inq[d] = it == 0 ? (d == 0 ? c->d_piv : bseed) : c->d_q[d * 2 + ((it - 1) & 1)];
-                a.outq[d] = c->d_q[d * 2 + (it & 1)];
-                a.hin[d] = c->d_huge[d * 2 + ((it + 1) & 1)];
-                a.hout[d] = c->d_huge[d * 2 + (it & 1)];
-            }
-            Launch lt = l;
-            lt.ev_start = e0;
-            lt.ev_stop = e1;
-            launch_reach(lt, sched, a);
-        },
-        guess, queues ? 2 : 0, depth, 0);
-    if (s != SCC_OK) return s;
-    c->prof.reach_levels += *depth;
-    int64_t sw = 0;
-    if (sched == SCC_SCHED_Q || sched == SCC_SCHED_T) sw = *depth;
-    else if (sched == SCC_SCHED_H) sw = c->h_ctl->pull_levels;
-    c->prof.sweep_levels += sw;
-    if (queues) c->prof.queue_levels += *depth;
-    if (c->instr) {
-        c->prof.expanded += (int64_t)c->h_ctl->st_expanded;
-        c->prof.edges += (int64_t)c->h_ctl->st_edges;
-        c->prof.visited += (int64_t)c->h_ctl->st_visited;
+    ReachChain(scc_ctx *cc, uint32_t dirs, int sch, const int32_t *bs)
+        : c(cc), sched(sch), bseed(bs), l(sch == SCC_SCHED_Q ? launch_for(cc) : fixed_launch(cc)) {
+        a.row = c->g.row;
+        a.col = c->g.col;
+        a.rowT = c->g.rowT;
+        a.colT = c->g.colT;
+        a.word = c->d_word;
+        a.n = c->g.n;
+        a.ctl = c->d_ctl;
+        a.dirs = dirs;
+        a.instr = c->instr;
+        a.pull_thr = c->opts.hybrid_threshold > 0 ? (uint32_t)c->opts.hybrid_threshold
+                                                  : (uint32_t)std::max<int64_t>(1024, c->g.n / 20);
+        a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
+                                          : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
+        a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
+        const int div = c->opts.hybrid_arc_div > 0 ? c->opts.hybrid_arc_div : 14;
+        a.pull_arcs = c->opts.hybrid_arc_div < 0 ? ~0ull
+                                                 : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
+        a.dcls[0] = c->d_dcls;
+        a.dcls[1] = c->d_dcls + c->g.n;
     }
+    int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
+    void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
+        a.iter = it;
+        for (int d = 0; d < 2; d++) {
+            a.inq[d] = it == 0 ? (d == 0 ? c->d_piv : bseed) : c->d_q[d * 2 + ((it - 1) & 1)];
+            a.outq[d] = c->d_q[d * 2 + (it & 1)];
+            a.hin[d] = c->d_huge[d * 2 + ((it + 1) & 1)];
+            a.hout[d] = c->d_huge[d * 2 + (it & 1)];
+        }
+        Launch lt = l;
+        lt.ev_start = e0;
+        lt.ev_stop = e1;
+        launch_reach(lt, sched, a);
+    }
+    void account(int64_t depth) {  // profile counters of a finished chain
+        c->prof.reach_levels += depth;
+        int64_t sw = 0;
+        if (sched == SCC_SCHED_Q || sched == SCC_SCHED_T) sw = depth;
+        else if (sched == SCC_SCHED_H) sw = c->h_ctl->pull_levels;
+        c->prof.sweep_levels += sw;
+        if (sched >= SCC_SCHED_D) c->prof.queue_levels += depth;
+        if (c->instr) {
+            c->prof.expanded += (int64_t)c->h_ctl->st_expanded;
+            c->prof.edges += (int64_t)c->h_ctl->st_edges;
+            c->prof.visited += (int64_t)c->h_ctl->st_visited;
+        }
+    }
+};
+
+// resume > 0: continue a chain whose first `resume` levels already ran
+scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false, int64_t resume = 0) {
+    TrimChain tc(c, fresh);
+    if (!resume) launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
+    return run_chain(c, tc, c->guess[fresh ? 3 : 2], 2, iters, 1, resume);
+}
+
+scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, int64_t *depth,
+                    int64_t resume = 0) {
+    ReachChain rc(c, dirs, sched, bseed);
+    if (c->instr && !resume) launch_ctl_reset(c->stream, c->d_ctl, 32);
+    CHK(run_chain(c, rc, c->guess[dirs == 2 ? 1 : 0], rc.reset_mask(), depth, 0, resume));
+    rc.account(*depth);
     return SCC_OK;
 }
 
@@ -545,6 +565,8 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
     c->opts = *opts;
+    c->spec_reach.clear();  // chain depths depend on the options
+    c->spec_trim.clear();
     return SCC_OK;
 }
 
@@ -564,6 +586,8 @@ void scc_destroy(scc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     for (auto e : c->evs) (void)hipEventDestroy(e);
+    for (auto e : c->spec_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)c->d_wf, (void *)c->d_m, (void *)c->d_v, (void *)c->d_wb})
         if (p) (void)hipFree(p);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
@@ -662,6 +686,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->uploaded = true;
+    c->spec_reach.clear();
+    c->spec_trim.clear();
     return SCC_OK;
 }
 
@@ -720,12 +746,91 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // schedules that switch once n/100 vertices are FB-eliminated: the
         // reference's HYBRID (t, then q; scc.cu:493-563) and COMBINE (l, then q)
         const bool by_elim = (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1) || schedule == SCC_SCHED_C;
+        // Speculative steps: reach chain, trim chain and election go out without
+        // host waits between them, each chain as long as the same step's chain
+        // was in the last run.  A guard after each chain sets ctl->abort when the
+        // chain may need more levels; every later kernel of the step then exits
+        // at once, and the host resumes that chain and finishes the step with
+        // waits.  The results are those of the waiting path.
+        const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !learned &&
+                             !by_elim && c->prof_flags == 0;
         while (unsynced || npiv > 0) {
             st.steps++;
             int sched = schedule == SCC_SCHED_B ? SCC_SCHED_H : schedule;
             if (by_elim) sched = ne < one_pct ? (schedule == SCC_SCHED_C ? SCC_SCHED_L : SCC_SCHED_T) : SCC_SCHED_Q;
             int64_t d = 0;
             tp = std::chrono::steady_clock::now();
+            const size_t k = (size_t)st.steps - 1;
+            if (spec_ok && ((c->epoch + 1) & 0xFFFFu) >= 2u) {  // no table refill inside the step
+                if (!c->spec_ev[0])
+                    for (auto &e : c->spec_ev) HIPCHK(c, hipEventCreate(&e));
+                int Lr = k < c->spec_reach.size() ? c->spec_reach[k] : c->guess[0];
+                int Lt = k < c->spec_trim.size() ? c->spec_trim[k] : c->guess[2];
+                Lr = std::max(1, std::min(Lr, 1024));
+                Lt = std::max(1, std::min(Lt, 1024));
+                HIPCHK(c, hipEventRecord(c->spec_ev[0], c->stream));
+                ReachChain rc(c, 3, sched, c->d_piv);
+                launch_ctl_reset(c->stream, c->d_ctl, rc.reset_mask() | 1);
+                for (int i = 0; i < Lr; i++) rc(i, nullptr, nullptr);
+                launch_chain_guard(c->stream, c->d_ctl, Lr - 1, 1);
+                HIPCHK(c, hipEventRecord(c->spec_ev[1], c->stream));
+                TrimChain tc(c, false);
+                launch_ctl_reset(c->stream, c->d_ctl, 128);
+                launch_ctl_reset(c->stream, c->d_ctl, 2 | 1);
+                for (int i = 0; i < Lt; i++) tc(i, nullptr, nullptr);
+                launch_chain_guard(c->stream, c->d_ctl, Lt - 1, 2);
+                HIPCHK(c, hipEventRecord(c->spec_ev[2], c->stream));
+                c->cur_step = (int)st.steps;
+                int64_t unused = 0;
+                CHK(do_elect(c, 1, &unused, false, false));
+                HIPCHK(c, hipEventRecord(c->spec_ev[3], c->stream));
+                CHK(read_ctl(c));
+                const Ctl h = *c->h_ctl;
+                const bool elected = h.abort == 0;
+                // the election before this step's reach (the step's own election,
+                // when it ran, moved those counts to the _prev fields)
+                const int64_t np = elected ? h.npiv_prev : h.npiv;
+                if (unsynced) {
+                    st.first_pivot = (int64_t)(elected ? h.first_piv_prev : h.first_piv);
+                    unsynced = false;
+                }
+                if (np == 0) {
+                    launch_ctl_reset(c->stream, c->d_ctl, 512);
+                    return fail(c, SCC_E_STATE, "live vertices left but no pivot elected");
+                }
+                st.pivots_total += np;
+                int64_t dr = h.depth[0], dt = h.depth[1];
+                if (elected) {
+                    float t[3];
+                    for (int j = 0; j < 3; j++) HIPCHK(c, hipEventElapsedTime(&t[j], c->spec_ev[j], c->spec_ev[j + 1]));
+                    st.ms_fwd += t[0];
+                    st.ms_trim += t[1];
+                    st.ms_update += t[2];
+                    npiv = h.npiv;
+                } else {  // undo the election's bookkeeping and finish the step with waits
+                    c->epoch--;
+                    c->prev_tab ^= 1;
+                    launch_ctl_reset(c->stream, c->d_ctl, 512);
+                    if (h.abort == 1) CHK(do_reach(c, 3, sched, c->d_piv, &dr, Lr));
+                    st.ms_fwd += ms_since(tp);
+                    tp = std::chrono::steady_clock::now();
+                    CHK(do_trim(c, &dt, false, h.abort == 2 ? Lt : 0));
+                    st.ms_trim += ms_since(tp);
+                    tp = std::chrono::steady_clock::now();
+                    CHK(do_elect(c, 1, &npiv, false));
+                    st.ms_update += ms_since(tp);
+                }
+                st.fwd_depths += dr;
+                st.bwd_depths += dr;
+                st.trim_iters += dt;
+                if (c->spec_reach.size() <= k) {
+                    c->spec_reach.resize(k + 1, 1);
+                    c->spec_trim.resize(k + 1, 1);
+                }
+                c->spec_reach[k] = (int)dr;
+                c->spec_trim[k] = (int)dt;
+                continue;
+            }
             if (c->opts.sequential_reach) {  // scc.cu:285-704 order: forward, then backward
                 CHK(do_reach(c, 1, sched, c->d_piv, &d));
                 st.fwd_depths += d;
@@ -829,6 +934,8 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
     if (!c) return SCC_E_INPUT;
     if (!c->uploaded) return fail(c, SCC_E_STATE, "scc_set_batch before scc_upload");
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->spec_reach.clear();
+    c->spec_trim.clear();
     if (c->d_goff) HIPCHK(c, hipFree(c->d_goff));
     c->d_goff = nullptr;
     c->num_graphs = 0;
@@ -1023,6 +1130,7 @@ scc_status scc_state_set(scc_ctx *c, const uint32_t *words, const int32_t *roots
     if (!c || !words) return SCC_E_INPUT;
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     size_t vb = sizeof(uint32_t) * (size_t)c->g.n;
+    launch_ctl_reset(c->stream, c->d_ctl, 512);  // phase-level use starts from a clean control block
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (vb == 0) return SCC_OK;
     HIPCHK(c, hipMemcpy(c->d_word, words, vb, hipMemcpyHostToDevice));

@@ -47,7 +47,11 @@ struct Ctl {
     uint32_t live;         // vertices not eliminated (set by k_init; trims, pair trims and
                            // FB eliminations subtract)
     uint32_t first_piv;    // lowest pivot of the last election
+    uint32_t npiv_prev, first_piv_prev;  // npiv, first_piv of the election before the last
     uint32_t pcount;       // vertices holding P (class representatives: #SCC at the end)
+    uint32_t abort;        // speculative step: chain stage that did not converge (0 = none);
+                           // every gated kernel exits at once while it is set
+    uint32_t depth[2];     // speculative step: levels of its reach chain and trim chain
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
@@ -73,6 +77,7 @@ __device__ inline void my_range(int64_t n, int64_t &beg, int64_t &end) {
 // Gate of a chained level kernel: block 0 clears the flag of the next level;
 // a level whose predecessor made no progress exits at once.
 __device__ inline bool chain_exit(Ctl *ctl, int iter) {
+    if (__atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return true;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->changed[(iter + 1) & 3] = 0;
     if (iter > 0 && __atomic_load_n(&ctl->changed[(iter - 1) & 3], __ATOMIC_RELAXED) == 0)
         return true;

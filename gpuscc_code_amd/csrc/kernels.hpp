@@ -71,7 +71,9 @@ void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t byt
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 // out[v] = bit length of row[v + 1] - row[v]
 void launch_degree_class(const Launch &l, const int32_t *row, int32_t n, uint8_t *out);
-void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);  // what 512: clear ctl->abort
+// after a chain of last + 1 levels launched without a wait: depth and convergence check
+void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,

@@ -28,6 +28,7 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->live = (uint32_t)n;
         ctl->pcount = 0;
+        ctl->abort = 0;
     }
     int64_t beg, end;
     my_range(n, beg, end);
@@ -81,9 +82,21 @@ __global__ void k_degree_class(const int32_t *__restrict__ row, int32_t n, uint8
     }
 }
 
+// End of a chain launched without a host wait: records its depth in
+// depth[stage - 1] and sets abort = stage when its last level still made
+// progress (the chain may need more levels).
+__global__ void k_chain_guard(Ctl *ctl, int last, int stage) {
+    if (threadIdx.x != 0 || ctl->abort) return;
+    uint32_t ran = ctl->ran;
+    ctl->depth[stage - 1] = ran;
+    if (ran == (uint32_t)(last + 1) && ctl->changed[last & 3] != 0) ctl->abort = (uint32_t)stage;
+}
+
 // chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in graphs)
 __global__ void k_ctl_reset(Ctl *ctl, int what) {
     if (threadIdx.x != 0) return;
+    if (what & 512) ctl->abort = 0;  // the host has handled the stage that stopped
+    else if (ctl->abort) return;
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
         for (int d = 0; d < 2; d++)
@@ -108,6 +121,8 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
         ctl->seeds[1] = 0;
     }
     if (what & 4) {
+        ctl->npiv_prev = ctl->npiv;
+        ctl->first_piv_prev = ctl->first_piv;
         ctl->npiv = 0;
         ctl->first_piv = 0xFFFFFFFFu;
     }
@@ -134,6 +149,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
                          const uint16_t *__restrict__ score, Ctl *ctl) {
     __shared__ unsigned long long sh[16];
+    if (ctl && __atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
     my_range(n, beg, end);
     const int lane = threadIdx.x & (WAVE - 1);
@@ -216,6 +232,7 @@ __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *_
                           int slot_bits, int32_t *piv, Ctl *ctl) {
     __shared__ uint32_t shw[16];
     __shared__ uint32_t sbase;
+    if (__atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
     my_range(n, beg, end);
     for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * CONF_V) {  // block-uniform
@@ -500,6 +517,9 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
 }
 void launch_degree_class(const Launch &l, const int32_t *row, int32_t n, uint8_t *out) {
     hipLaunchKernelGGL(k_degree_class, GRID(l), row, n, out);
+}
+void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage) {
+    hipLaunchKernelGGL(k_chain_guard, dim3(1), dim3(64), 0, s, ctl, last, stage);
 }
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what);

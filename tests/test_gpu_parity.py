@@ -381,3 +381,29 @@ def test_batch_rejects_crossing_arcs(ctx):
     assert list(lab) == [0, 0, 2, 3, 4, 5]
     with pytest.raises(scc.SccError):
         ctx.set_batch([0, 4, 3, 6])  # decreasing offsets
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_speculative_steps_match_waiting_path(ctx, sched):
+    """A step launched without waits (guards abort a chain that may need more
+    levels; the host resumes it) ends exactly where the waiting path ends: the
+    first run after an upload guesses the chain lengths from other graphs (and
+    so exercises the resume path), the second uses the lengths of the first."""
+    # level counts are left out: with the workgroup-local rounds of d, l and h
+    # they depend on timing (the reached set of every chain does not)
+    keys = ("steps", "num_scc", "pivots_total", "first_pivot", "trivial", "biggest")
+    for g in (demo_graph(), scc.gen_rmat(13, 8, seed=21), scc.gen_mesh(96, 0.6, seed=5), cycle(3000),
+              path_dag(500)):
+        ref, _ = oracle.tarjan(g.row, g.col)
+        ctx.upload(g)
+        ctx.profile(2)  # event timing: every chain waits
+        lab0, st0 = ctx.run(sched)
+        w0, r0 = ctx.get_state()
+        ctx.profile(0)
+        for _ in range(2):
+            lab, st = ctx.run(sched)
+            w, r = ctx.get_state()
+            assert np.array_equal(lab, ref) and np.array_equal(lab0, ref)
+            assert np.array_equal(w, w0) and np.array_equal(r, r0)
+            assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}
+        ctx.profile_read(reset=True)
