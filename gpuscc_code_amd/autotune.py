@@ -36,8 +36,10 @@ KNOBS = {
     "local_rounds": lambda n, cus: [0, 16, 256, -1],
     "huge_row": lambda n, cus: [0, 1024, 16384],
     "block_size": lambda n, cus: [0, 128, 512],
+    "reach_graphs": lambda n, cus: [0, 1],
 }
-ORDER = ["grid_blocks", "hybrid_arc_div", "hybrid_threshold", "local_rounds", "huge_row", "block_size"]
+ORDER = ["grid_blocks", "hybrid_arc_div", "hybrid_threshold", "local_rounds", "huge_row", "block_size",
+         "reach_graphs"]
 
 
 def log(*a):

@@ -70,6 +70,18 @@ struct scc_ctx {
     // speculative steps: reach and trim chain lengths of step k in the last run
     std::vector<int> spec_reach, spec_trim;
     hipEvent_t spec_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // reach chains captured as HIP graphs: key (schedule, levels, thresholds) -> executable
+    struct GraphKey {
+        int sched, levels;
+        uint32_t pull_thr, huge_thr;
+        unsigned long long pull_arcs;
+        int local_rounds;
+        bool operator==(const GraphKey &o) const {
+            return sched == o.sched && levels == o.levels && pull_thr == o.pull_thr && huge_thr == o.huge_thr &&
+                   pull_arcs == o.pull_arcs && local_rounds == o.local_rounds;
+        }
+    };
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> reach_graphs;
 };
 
 namespace {
@@ -201,6 +213,11 @@ scc_status run_chain(scc_ctx *c, F &&launch_level, int &guess, int reset_mask, i
     return SCC_OK;
 }
 
+void clear_graphs(scc_ctx *c) {
+    for (auto &kv : c->reach_graphs) (void)hipGraphExecDestroy(kv.second);
+    c->reach_graphs.clear();
+}
+
 // Level launchers of the two chains (one object per chain run).
 struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs never leave a colour)
     scc_ctx *c;
@@ -296,6 +313,31 @@ struct ReachChain {
         }
     }
 };
+
+// Levels 0 .. L-1 of a fused reach chain from the pivot list.  Their
+// arguments depend only on the schedule and the level (buffers are fixed per
+// upload, options per set_opts), so the L launches are captured once as a HIP
+// graph and replayed as one launch.  The chain holds no memset nodes.
+scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
+    const scc_ctx::GraphKey key{rc.sched, L, rc.a.pull_thr, rc.a.huge_thr, rc.a.pull_arcs, rc.a.local_rounds};
+    for (auto &kv : c->reach_graphs)
+        if (kv.first == key) {
+            HIPCHK(c, hipGraphLaunch(kv.second, c->stream));
+            return SCC_OK;
+        }
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < L; i++) rc(i, nullptr, nullptr);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+    if (c->reach_graphs.size() >= 64) clear_graphs(c);  // bounded cache
+    c->reach_graphs.emplace_back(key, exec);
+    HIPCHK(c, hipGraphLaunch(exec, c->stream));
+    return SCC_OK;
+}
 
 // resume > 0: continue a chain whose first `resume` levels already ran
 scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false, int64_t resume = 0) {
@@ -420,6 +462,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 }
 
 void free_graph(scc_ctx *c) {
+    clear_graphs(c);  // captured launches point at the buffers freed here
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
                         &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
                         &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
@@ -565,6 +608,7 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
     c->opts = *opts;
+    clear_graphs(c);        // captured launches hold the old options
     c->spec_reach.clear();  // chain depths depend on the options
     c->spec_trim.clear();
     return SCC_OK;
@@ -771,7 +815,11 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 HIPCHK(c, hipEventRecord(c->spec_ev[0], c->stream));
                 ReachChain rc(c, 3, sched, c->d_piv);
                 launch_ctl_reset(c->stream, c->d_ctl, rc.reset_mask() | 1);
-                for (int i = 0; i < Lr; i++) rc(i, nullptr, nullptr);
+                if (c->opts.reach_graphs > 0) {
+                    CHK(launch_reach_levels(c, rc, Lr));
+                } else {
+                    for (int i = 0; i < Lr; i++) rc(i, nullptr, nullptr);
+                }
                 launch_chain_guard(c->stream, c->d_ctl, Lr - 1, 1);
                 HIPCHK(c, hipEventRecord(c->spec_ev[1], c->stream));
                 TrimChain tc(c, false);

@@ -40,6 +40,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     huge_row: int = 0
     local_rounds: int = 0
     hybrid_arc_div: int = 0
+    reach_graphs: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -52,6 +53,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.huge_row = self.huge_row
         o.local_rounds = self.local_rounds
         o.hybrid_arc_div = self.hybrid_arc_div
+        o.reach_graphs = self.reach_graphs
         return o
 
 

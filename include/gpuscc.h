@@ -75,6 +75,8 @@ typedef struct {
                                   -1 = off) */
     int32_t hybrid_arc_div;    /* h: a direction also pulls when the arcs of its frontier (estimated
                                   from degree classes) reach m / this (0 = default 14, -1 = off) */
+    int32_t reach_graphs;      /* 1: the reach chains of speculative steps are captured once as HIP
+                                  graphs and replayed (one launch per chain); 0 = launched level by level */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
