@@ -46,7 +46,7 @@ struct Stage {
     int32_t buf[2][SQ_CAP];
     uint32_t cnt[2];
     uint32_t base[2];
-    unsigned long long arcs[2];  // estimated arcs of the vertices pushed since the last flush
+    unsigned long long arcs[2];  // estimated arcs of vertices queued past the buffers since the last flush
 };
 
 __device__ inline void st_init(Stage &s) {
@@ -59,14 +59,17 @@ __device__ inline void st_init(Stage &s) {
 // arcs of a vertex of degree class k (bit length k): about 3/4 of 2^k
 __device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << k) >> 2; }
 
+// The arc estimate of the next frontier (degree classes) is taken when the
+// staged vertices reach the queue (st_flush), not per push: a push is then one
+// LDS atomic and one LDS store, with no dependent global load.
 __device__ inline void st_push(Stage &s, int d, int32_t v, const ReachArgs &a) {
-    if (a.dcls[d]) atomicAdd(&s.arcs[d], class_arcs(a.dcls[d][v]));
     uint32_t i = atomicAdd(&s.cnt[d], 1u);
     if (i < SQ_CAP) {
         s.buf[d][i] = v;
     } else {
         uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
         a.outq[d][g] = v;
+        if (a.dcls[d]) atomicAdd(&a.ctl->qarcs[d][(a.iter + 1) & 3], class_arcs(a.dcls[d][v]));
     }
 }
 
@@ -77,13 +80,23 @@ __device__ inline void st_flush(Stage &s, const ReachArgs &a) {
     uint32_t c1 = s.cnt[1] < SQ_CAP ? s.cnt[1] : SQ_CAP;
     if (threadIdx.x == 0 && c0) s.base[0] = atomicAdd(&a.ctl->qsize[0][(a.iter + 1) & 3], c0);
     if (threadIdx.x == 1 && c1) s.base[1] = atomicAdd(&a.ctl->qsize[1][(a.iter + 1) & 3], c1);
-    if (threadIdx.x < 2 && s.arcs[threadIdx.x])
-        atomicAdd(&a.ctl->qarcs[threadIdx.x][(a.iter + 1) & 3], s.arcs[threadIdx.x]);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < c0; i += blockDim.x) a.outq[0][s.base[0] + i] = s.buf[0][i];
-    for (uint32_t i = threadIdx.x; i < c1; i += blockDim.x) a.outq[1][s.base[1] + i] = s.buf[1][i];
+    unsigned long long ar0 = 0, ar1 = 0;
+    for (uint32_t i = threadIdx.x; i < c0; i += blockDim.x) {
+        int32_t v = s.buf[0][i];
+        a.outq[0][s.base[0] + i] = v;
+        if (a.dcls[0]) ar0 += class_arcs(a.dcls[0][v]);
+    }
+    for (uint32_t i = threadIdx.x; i < c1; i += blockDim.x) {
+        int32_t v = s.buf[1][i];
+        a.outq[1][s.base[1] + i] = v;
+        if (a.dcls[1]) ar1 += class_arcs(a.dcls[1][v]);
+    }
+    if (ar0) atomicAdd(&s.arcs[0], ar0);
+    if (ar1) atomicAdd(&s.arcs[1], ar1);
     __syncthreads();
     if (threadIdx.x < 2) {
+        if (s.arcs[threadIdx.x]) atomicAdd(&a.ctl->qarcs[threadIdx.x][(a.iter + 1) & 3], s.arcs[threadIdx.x]);
         s.cnt[threadIdx.x] = 0;
         s.arcs[threadIdx.x] = 0;
     }
@@ -404,13 +417,6 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
         if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
         for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x) lq.v[k] = st.buf[0][k];
         for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x) lq.v[SQ_CAP + k] = st.buf[1][k];
-        // the moved vertices leave the next level's frontier (rows sent on below rejoin it)
-        if (a.dcls[0]) {
-            for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x)
-                atomicAdd(&st.arcs[0], 0ull - class_arcs(a.dcls[0][lq.v[k]]));
-            for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x)
-                atomicAdd(&st.arcs[1], 0ull - class_arcs(a.dcls[1][lq.v[SQ_CAP + k]]));
-        }
         __syncthreads();
         if (threadIdx.x < 2) st.cnt[threadIdx.x] = 0;
         __syncthreads();
