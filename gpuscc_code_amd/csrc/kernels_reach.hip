@@ -386,12 +386,28 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32
     }
     cx++;
     ce += (unsigned long long)(e1 - e0);
-    for (int32_t e = e0; e < e1; e++) {
-        int32_t u = col[e];
-        if (try_claim(a.word, u, c, bit)) {
+    // four arcs per round: their column ids, then their words, then the claims
+    // of the candidates, each group of loads in flight together
+    constexpr int A = 4;
+    for (int32_t e = e0; e < e1; e += A) {
+        int32_t u[A];
+        uint32_t wu[A];
+#pragma unroll
+        for (int j = 0; j < A; j++) u[j] = e + j < e1 ? col[e + j] : -1;
+#pragma unroll
+        for (int j = 0; j < A; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
+        uint32_t old[A];
+#pragma unroll
+        for (int j = 0; j < A; j++) {
+            old[j] = bit;
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+        }
+#pragma unroll
+        for (int j = 0; j < A; j++) {
+            if (old[j] & bit) continue;
             found = true;
             cv++;
-            st_push(st, d, u, a);
+            st_push(st, d, u[j], a);
         }
     }
 }
