@@ -25,7 +25,8 @@
 
 namespace gpuscc {
 
-constexpr uint32_t SQ_CAP = 1024;  // LDS staging entries per direction per workgroup
+constexpr uint32_t SQ_CAP = 2048;  // LDS staging entries per direction per workgroup
+constexpr int EXP_ILP = 4;          // arcs per work-item per pass of a balanced expansion
 
 __device__ inline uint32_t dbit(int d) { return d == 0 ? ST_F : ST_B; }
 __device__ inline uint32_t dxbit(int d) { return d == 0 ? ST_FX : ST_BX; }
@@ -103,10 +104,12 @@ __device__ inline void st_flush(Stage &s, const ReachArgs &a) {
     __syncthreads();
 }
 
-// all work-items call this; flushes when another pass could overflow a buffer
-__device__ inline void st_maybe_flush(Stage &s, const ReachArgs &a) {
+// all work-items call this; flushes when another pass (at most `per_item`
+// pushes per work-item) could overflow a buffer
+__device__ inline void st_maybe_flush(Stage &s, const ReachArgs &a, uint32_t per_item = 1) {
     __syncthreads();
-    bool full = s.cnt[0] > SQ_CAP - blockDim.x || s.cnt[1] > SQ_CAP - blockDim.x;
+    const uint32_t room = per_item * blockDim.x;
+    bool full = s.cnt[0] > SQ_CAP - room || s.cnt[1] > SQ_CAP - room;
     __syncthreads();  // every work-item has read the counts before anyone pushes again
     if (full) st_flush(s, a);
 }
@@ -150,20 +153,40 @@ struct Items {
 template <bool PUSH>
 __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nitems, uint32_t total,
                                     Stage *st, bool &found, unsigned long long &cv) {
-    for (uint32_t base = 0; base < total; base += blockDim.x) {  // block-uniform trip count
-        uint32_t e = base + threadIdx.x;
-        if (e < total) {
-            uint32_t k = find_item(it.off, nitems, e);
-            uint32_t cd = it.cd[k];
-            int d = (int)(cd >> 31);
-            int32_t u = dcol(a, d)[it.start[k] + (int32_t)(e - it.off[k])];
-            if (try_claim(a.word, u, cd & 0x7FFFFFFFu, dbit(d))) {
-                found = true;
-                cv++;
-                if (PUSH) st_push(*st, d, u, a);
+    // EXP_ILP arcs per work-item and pass: their column ids, then their words,
+    // then the claims of the candidates, each group of loads in flight together
+    for (uint32_t base = 0; base < total; base += blockDim.x * EXP_ILP) {  // block-uniform trip count
+        int32_t u[EXP_ILP];
+        uint32_t cd[EXP_ILP];
+#pragma unroll
+        for (int j = 0; j < EXP_ILP; j++) {
+            uint32_t e = base + (uint32_t)j * blockDim.x + threadIdx.x;
+            u[j] = -1;
+            cd[j] = 0;
+            if (e < total) {
+                uint32_t k = find_item(it.off, nitems, e);
+                cd[j] = it.cd[k];
+                u[j] = dcol(a, (int)(cd[j] >> 31))[it.start[k] + (int32_t)(e - it.off[k])];
             }
         }
-        if (PUSH) st_maybe_flush(*st, a);
+        uint32_t wu[EXP_ILP], old[EXP_ILP];
+#pragma unroll
+        for (int j = 0; j < EXP_ILP; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
+#pragma unroll
+        for (int j = 0; j < EXP_ILP; j++) {
+            uint32_t bit = dbit((int)(cd[j] >> 31));
+            old[j] = bit;
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == (cd[j] & 0x7FFFFFFFu))
+                old[j] = atomicOr(&a.word[u[j]], bit);
+        }
+#pragma unroll
+        for (int j = 0; j < EXP_ILP; j++) {
+            if (old[j] & dbit((int)(cd[j] >> 31))) continue;
+            found = true;
+            cv++;
+            if (PUSH) st_push(*st, (int)(cd[j] >> 31), u[j], a);
+        }
+        if (PUSH) st_maybe_flush(*st, a, EXP_ILP);
     }
 }
 
