@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = "SCC MTEPS + ms/graph, learned-pivot vs plain FB on 1 MI355X (bit-exact)"
 HBM_PEAK_GBS = 8000.0
 KERNEL_OF = {"q": "k_reach_sweep", "t": "k_reach_topo", "d": "k_reach_queue", "l": "k_reach_lb",
-             "h": "k_reach_hybrid"}
+             "h": "k_reach_hybrid", "c": "k_reach_lb+k_reach_sweep", "b": "k_reach_hybrid"}
 
 
 def log(*a):
@@ -97,7 +97,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--schedule", default="h", choices=list("qtdlh"))
+    ap.add_argument("--schedule", default="h", choices=list("qtdlhcb"),
+                    help="q t d l h as the reference's -a; c = COMBINE, b = BEST")
     ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2])
     ap.add_argument("--workload", default="rmat", choices=["rmat", "mesh", "batch"],
                     help="rmat: R-MAT (default, scale 22); mesh: 2048x2048 grid; "
