@@ -60,17 +60,42 @@ __device__ inline void st_init(Stage &s) {
 // arcs of a vertex of degree class k (bit length k): about 3/4 of 2^k
 __device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << k) >> 2; }
 
-// The arc estimate of the next frontier (degree classes) is taken when the
-// staged vertices reach the queue (st_flush), not per push: a push is then one
-// LDS atomic and one LDS store, with no dependent global load.
+// A push past the buffers goes straight to the queue (and its arc estimate
+// to the control block).
+__device__ inline void st_overflow(int d, int32_t v, const ReachArgs &a) {
+    uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
+    a.outq[d][g] = v;
+    if (a.dcls[d]) atomicAdd(&a.ctl->qarcs[d][(a.iter + 1) & 3], class_arcs(a.dcls[d][v]));
+}
+
+// One lane's push: one LDS atomic.  The arc estimate of the next frontier
+// (degree classes) is taken when the staged vertices reach the queue
+// (st_flush), not per push.  Used where few lanes push at a time (the
+// latency-bound local rounds).
 __device__ inline void st_push(Stage &s, int d, int32_t v, const ReachArgs &a) {
     uint32_t i = atomicAdd(&s.cnt[d], 1u);
-    if (i < SQ_CAP) {
-        s.buf[d][i] = v;
-    } else {
-        uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
-        a.outq[d][g] = v;
-        if (a.dcls[d]) atomicAdd(&a.ctl->qarcs[d][(a.iter + 1) & 3], class_arcs(a.dcls[d][v]));
+    if (i < SQ_CAP) s.buf[d][i] = v;
+    else st_overflow(d, v, a);
+}
+
+// Wave-aggregated push, for passes where many lanes push together: the active
+// lanes pushing direction d ballot, the lowest of them reserves their slots
+// with one LDS atomic, and each lane writes at that base plus the number of
+// pushing lanes below it.
+__device__ inline void st_push_wave(Stage &s, int d, int32_t v, const ReachArgs &a) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int dd = 0; dd < 2; dd++) {
+        const unsigned long long m = __ballot(d == dd);  // among the lanes calling now
+        if (d != dd) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&s.cnt[dd], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, WAVE);
+        const uint32_t i = base + (uint32_t)__popcll(m & below);
+        if (i < SQ_CAP) s.buf[dd][i] = v;
+        else st_overflow(dd, v, a);
     }
 }
 
@@ -184,7 +209,7 @@ __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nite
             if (old[j] & dbit((int)(cd[j] >> 31))) continue;
             found = true;
             cv++;
-            if (PUSH) st_push(*st, (int)(cd[j] >> 31), u[j], a);
+            if (PUSH) st_push_wave(*st, (int)(cd[j] >> 31), u[j], a);
         }
         if (PUSH) st_maybe_flush(*st, a, EXP_ILP);
     }
@@ -264,7 +289,7 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
                     if (e < cend && try_claim(a.word, col[e], cc, bit)) {
                         found = true;
                         cv++;
-                        if (PUSH) st_push(*st, d, col[e], a);
+                        if (PUSH) st_push_wave(*st, d, col[e], a);
                     }
                     if (PUSH) st_maybe_flush(*st, a);
                 }
@@ -396,6 +421,7 @@ union ItemsLocal {
 };
 
 // expands item v of direction d serially; new visits are staged
+template <bool WAVE_PUSH>
 __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, bool &found,
                                      unsigned long long &cx, unsigned long long &ce,
                                      unsigned long long &cv) {
@@ -430,7 +456,8 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32
             if (old[j] & bit) continue;
             found = true;
             cv++;
-            st_push(st, d, u[j], a);
+            if (WAVE_PUSH) st_push_wave(st, d, u[j], a);
+            else st_push(st, d, u[j], a);
         }
     }
 }
@@ -470,7 +497,7 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                     uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
                     a.outq[d][g] = v;
                 } else {
-                    expand_serial(a, st, d, v, found, cx, ce, cv);
+                    expand_serial<false>(a, st, d, v, found, cx, ce, cv);
                 }
             }
             st_maybe_flush(st, a);
@@ -501,7 +528,7 @@ __global__ void k_reach_queue(ReachArgs a) {
         if (i < b1) {
             int d = i < qs0 ? 0 : 1;
             int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
-            expand_serial(a, st, d, v, found, cx, ce, cv);
+            expand_serial<true>(a, st, d, v, found, cx, ce, cv);
         }
         st_maybe_flush(st, a);
     }
