@@ -65,7 +65,7 @@ def write_levels(path, workload, rows):
     from gpuscc_code_amd import scc
     with open(path, "w") as f:
         f.write(f"# {workload}\n")
-        f.write("\t".join(scc.LEVEL_FIELDS) + "\n")
+        f.write("\t".join("kernel_us" if k == "ms" else k for k in scc.LEVEL_FIELDS) + "\n")
         for r in rows:
             f.write("\t".join(f"{x:.1f}" if i == 2 else f"{int(x)}"
                               for i, x in enumerate([r[0], r[1], r[2] * 1e3] + list(r[3:]))) + "\n")
