@@ -37,9 +37,10 @@ KNOBS = {
     "huge_row": lambda n, cus: [0, 1024, 16384],
     "block_size": lambda n, cus: [0, 128, 512],
     "reach_graphs": lambda n, cus: [0, 1],
+    "hybrid_policy": lambda n, cus: [0, 2],
 }
-ORDER = ["grid_blocks", "hybrid_arc_div", "hybrid_threshold", "local_rounds", "huge_row", "block_size",
-         "reach_graphs"]
+ORDER = ["hybrid_policy", "grid_blocks", "hybrid_arc_div", "hybrid_threshold", "local_rounds", "huge_row",
+         "block_size", "reach_graphs"]
 
 
 def log(*a):

@@ -284,6 +284,7 @@ struct ReachChain {
                                                  : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
         a.dcls[0] = c->d_dcls;
         a.dcls[1] = c->d_dcls + c->g.n;
+        a.pull_cost = c->opts.hybrid_policy == 2 ? 1 : 0;
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {

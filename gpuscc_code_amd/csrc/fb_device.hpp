@@ -54,6 +54,7 @@ struct Ctl {
     uint32_t depth[2];     // speculative step: levels of its reach chain and trim chain
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
+    uint32_t creach[2][4];  // vertices reached so far by each direction (ring: level iter writes [iter & 3])
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
 };
 

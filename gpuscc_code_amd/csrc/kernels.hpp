@@ -43,6 +43,7 @@ struct ReachArgs {
     uint32_t dirs;
     uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
     unsigned long long pull_arcs;  // hybrid: ... or one with at least this many (estimated) arcs
+    int pull_cost;                 // hybrid: pull when frontier arcs >= 2 x (live - reached) instead
     const uint8_t *dcls[2];  // degree class per vertex and direction (bit length of the degree)
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share

@@ -110,6 +110,7 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
             ctl->qsize[d][0] = ctl->seeds[d];
             for (int i = 1; i < 4; i++) ctl->qsize[d][i] = 0;
             for (int i = 0; i < 4; i++) ctl->qarcs[d][i] = 0;
+            for (int i = 0; i < 4; i++) ctl->creach[d][i] = 0;
         }
     }
     if (what & 64) {  // the pivot list seeds both directions

@@ -614,8 +614,26 @@ __device__ inline void pull_dirs(const ReachArgs &a, uint32_t &qs0, uint32_t &qs
                                  bool &pull1) {
     qs0 = (a.dirs & 1u) ? a.ctl->qsize[0][a.iter & 3] : 0u;
     qs1 = (a.dirs & 2u) ? a.ctl->qsize[1][a.iter & 3] : 0u;
-    pull0 = qs0 > 0 && (qs0 >= a.pull_thr || a.ctl->qarcs[0][a.iter & 3] >= a.pull_arcs);
-    pull1 = qs1 > 0 && (qs1 >= a.pull_thr || a.ctl->qarcs[1][a.iter & 3] >= a.pull_arcs);
+    if (!a.pull_cost) {
+        pull0 = qs0 > 0 && (qs0 >= a.pull_thr || a.ctl->qarcs[0][a.iter & 3] >= a.pull_arcs);
+        pull1 = qs1 > 0 && (qs1 >= a.pull_thr || a.ctl->qarcs[1][a.iter & 3] >= a.pull_arcs);
+        return;
+    }
+    // Cost model: a push level costs about the arcs of its frontier, a pull
+    // level about the live vertices the direction has not reached (most of
+    // them scan a short row; in a batch of many graphs most never find a
+    // reached neighbour).  Reached so far = the queue inputs of this and all
+    // earlier levels (every claim enters a queue exactly once).
+    const uint32_t live = a.ctl->live;
+    bool p[2];
+    for (int d = 0; d < 2; d++) {
+        const uint32_t qs = d == 0 ? qs0 : qs1;
+        const uint32_t reached = (a.iter > 0 ? a.ctl->creach[d][(a.iter - 1) & 3] : 0u) + qs;
+        const unsigned long long unreached = live > reached ? live - reached : 0u;
+        p[d] = qs > 0 && a.ctl->qarcs[d][a.iter & 3] >= 2ull * unreached;
+    }
+    pull0 = p[0];
+    pull1 = p[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -708,7 +726,12 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     pull_dirs(a, qs0, qs1, pull0, pull1);
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->pull_mask = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
+    if (blockIdx.x == 0 && threadIdx.x < 2) {
+        if (threadIdx.x == 0) a.ctl->pull_mask = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
+        const int d = (int)threadIdx.x;  // vertices reached through this level's input
+        a.ctl->creach[d][a.iter & 3] =
+            (a.iter > 0 ? a.ctl->creach[d][(a.iter - 1) & 3] : 0u) + (d == 0 ? qs0 : qs1);
+    }
     huge_phase<true>(a, it, &st, found, cx, ce, cv);
     st_flush(st, a);
     // push part: queue items of the directions that do not pull

@@ -61,8 +61,11 @@ typedef enum {
 /* Options: alg_opt (scc.h:19-27).  Zero means "default". */
 typedef struct {
     int32_t block_size;        /* workgroup size for sweeps (-b, scc.cu:82); default 256 */
-    int32_t hybrid_policy;     /* 0: switch per level by frontier size; 1: the reference's rule,
-                                  sweep-with-balancing until >= n/100 FB-eliminated, then q */
+    int32_t hybrid_policy;     /* 0: a direction pulls when its frontier reaches hybrid_threshold vertices
+                                  or m / hybrid_arc_div arcs; 1: the reference's rule, sweep-with-
+                                  balancing until >= n/100 FB-eliminated, then q; 2: pulls when its
+                                  frontier's arcs reach twice the live vertices it has not reached
+                                  (the work of a pull level) */
     int32_t hybrid_threshold;  /* h: frontier size (vertices) at or above which a level sweeps */
     int32_t slot_bits;         /* log2 of the election table size; default from n */
     int32_t grid_blocks;       /* workgroups of the grid-stride kernels; default from the CU count */

@@ -410,3 +410,24 @@ def test_speculative_steps_match_waiting_path(ctx, sched):
                 assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}
             ctx.profile_read(reset=True)
     ctx.set_opts(scc.alg_opt())
+
+
+def test_pull_cost_policy(ctx):
+    """hybrid_policy 2 (pull when the frontier's arcs reach twice the live
+    unreached vertices) changes only which levels pull: full state parity."""
+    ctx.set_opts(scc.alg_opt(hybrid_policy=2))
+    try:
+        for g in (scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1), scc.gen_mesh(128, 0.6, seed=3),
+                  scc.gen_batch(48, 64, 2048, seed=4)):
+            for variant in (0, 1, 2):
+                check_against_oracle(ctx, g, "h", variant)
+        g = scc.gen_batch(64, 64, 4096, seed=9)
+        ref, _ = oracle.tarjan(g.row, g.col)
+        ctx.upload(g)
+        ctx.set_batch(g.graph_offsets)
+        for variant in (0, 1, 2):
+            labels, _ = ctx.run("h", variant)
+            assert np.array_equal(labels, ref)
+        ctx.set_batch(None)
+    finally:
+        ctx.set_opts(scc.alg_opt())
