@@ -223,7 +223,7 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
     scc_ctx *c;
     Launch l;
     ReachArgs a{};
-    TrimChain(scc_ctx *cc, bool fresh) : c(cc), l(fixed_launch(cc)) {
+    TrimChain(scc_ctx *cc, bool fresh, bool skip_fb = false) : c(cc), l(fixed_launch(cc)) {
         a.row = c->g.row;
         a.col = c->g.col;
         a.rowT = c->g.rowT;
@@ -234,6 +234,7 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
         a.root = c->d_root;
         a.stamp = c->d_stamp;
         a.fresh = fresh ? 1 : 0;
+        a.skip_fb = skip_fb ? 1 : 0;
         a.dirs = 1;
     }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
@@ -340,9 +341,10 @@ scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
     return SCC_OK;
 }
 
-// resume > 0: continue a chain whose first `resume` levels already ran
-scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false, int64_t resume = 0) {
-    TrimChain tc(c, fresh);
+// resume > 0: continue a chain whose first `resume` levels already ran;
+// after_reach: the trim of an FB step (the reach from the pivots is complete)
+scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false, int64_t resume = 0, bool after_reach = false) {
+    TrimChain tc(c, fresh, after_reach);
     if (!resume) launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
     return run_chain(c, tc, c->guess[fresh ? 3 : 2], 2, iters, 1, resume);
 }
@@ -823,7 +825,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 }
                 launch_chain_guard(c->stream, c->d_ctl, Lr - 1, 1);
                 HIPCHK(c, hipEventRecord(c->spec_ev[1], c->stream));
-                TrimChain tc(c, false);
+                TrimChain tc(c, false, true);
                 launch_ctl_reset(c->stream, c->d_ctl, 128);
                 launch_ctl_reset(c->stream, c->d_ctl, 2 | 1);
                 for (int i = 0; i < Lt; i++) tc(i, nullptr, nullptr);
@@ -863,7 +865,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     if (h.abort == 1) CHK(do_reach(c, 3, sched, c->d_piv, &dr, Lr));
                     st.ms_fwd += ms_since(tp);
                     tp = std::chrono::steady_clock::now();
-                    CHK(do_trim(c, &dt, false, h.abort == 2 ? Lt : 0));
+                    CHK(do_trim(c, &dt, false, h.abort == 2 ? Lt : 0, true));
                     st.ms_trim += ms_since(tp);
                     tp = std::chrono::steady_clock::now();
                     CHK(do_elect(c, 1, &npiv, false));
@@ -902,7 +904,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             }
             st.pivots_total += npiv;
             tp = std::chrono::steady_clock::now();
-            CHK(do_trim(c, &it));
+            CHK(do_trim(c, &it, false, 0, true));
             st.trim_iters += it;
             st.ms_trim += ms_since(tp);
             tp = std::chrono::steady_clock::now();
@@ -929,8 +931,15 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             st.ms_update += ms_since(tp);
         }
         // canonical labels + #SCC (scc.cu:912-919): the device counts P bits as
-        // they are set, and the last wait above read the count
-        launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
+        // they are set, and the last wait above read the count.  With the plain
+        // rule every pivot is the lowest live id of its colour set, so root is
+        // already the minimum of each SCC (trimmed vertices and pairs point to
+        // themselves and to the lower id); the learned rule needs the minimum pass.
+        if (learned)
+            launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
+        else
+            HIPCHK(c, hipMemcpyAsync(c->d_label, c->d_root, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                                     c->stream));
         st.num_scc = (int64_t)c->h_ctl->pcount;
         return SCC_OK;
     };

@@ -53,6 +53,8 @@ struct ReachArgs {
     uint32_t *stamp;    // per-vertex tag of the last trim level that checked it
     uint32_t tag;       // this level's tag (unique per launch)
     int fresh;          // every vertex live, every arc inside one colour: level 0 tests degrees only
+    int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
+                        // two or more vertices, which no trim removes, so those vertices are not checked
 };
 
 struct Graph {  // device CSR + CSR^T

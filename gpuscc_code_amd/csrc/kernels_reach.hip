@@ -896,7 +896,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
     for (int k = 0; k < TRIM_V; k++) {
         int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
         uint32_t w = vv < end ? a.word[vv] : ST_E;
-        if (!(w & ST_E)) cand |= 1u << k;
+        if (!(w & ST_E) && !(a.skip_fb && (w & (ST_P | ST_F | ST_B)) == (ST_F | ST_B))) cand |= 1u << k;
     }
     constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, out-row (d = 0) or in-row
     uint32_t trimmed = 0;
@@ -1013,6 +1013,7 @@ __global__ void k_trim_level(ReachArgs a) {
                     uint32_t cc = cd & 0x7FFFFFFFu;
                     uint32_t wu = a.word[u];
                     if (!(wu & ST_E) && colour_of(wu) == cc &&
+                        !(a.skip_fb && (wu & (ST_P | ST_F | ST_B)) == (ST_F | ST_B)) &&
                         a.stamp[u] != a.tag &&  // plain load first: hubs are reached by many arcs
                         atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
                         found = true;
