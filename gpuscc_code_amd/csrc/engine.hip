@@ -235,6 +235,8 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
         a.stamp = c->d_stamp;
         a.fresh = fresh ? 1 : 0;
         a.skip_fb = skip_fb ? 1 : 0;
+        a.dcls[0] = c->d_dcls;  // self-loop bits for the degree test
+        a.dcls[1] = c->d_dcls + c->g.n;
         a.dirs = 1;
     }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
@@ -726,8 +728,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     Launch l = launch_for(c);
     launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
-    launch_degree_class(l, c->d_row, n, c->d_dcls);
-    launch_degree_class(l, c->d_rowT, n, c->d_dcls + n);
+    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls);
+    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());

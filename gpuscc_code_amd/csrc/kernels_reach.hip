@@ -57,8 +57,8 @@ __device__ inline void st_init(Stage &s) {
     }
 }
 
-// arcs of a vertex of degree class k (bit length k): about 3/4 of 2^k
-__device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << k) >> 2; }
+// arcs of a vertex of degree class k (bit length k, bit 7 the self-loop flag): about 3/4 of 2^k
+__device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << (k & 0x3Fu)) >> 2; }
 
 // A push past the buffers goes straight to the queue (and its arc estimate
 // to the control block).
@@ -850,28 +850,26 @@ constexpr int32_t TRIM_SWEEP_DIV = 16;  // a level after one that trimmed >= n/1
 // than itself (rows are duplicate-free, so two entries suffice).  Returns the
 // round's trimmed vertices as a mask (bit k: base + k * blockDim.x + threadIdx.x).
 __device__ inline uint32_t trim_round_fresh(const ReachArgs &a, int64_t base, int64_t end) {
-    int32_t o0[TRIM_V], od[TRIM_V], i0[TRIM_V], id[TRIM_V];
+    // a row keeps a live neighbour iff it holds a vertex other than v: its
+    // length minus v's self-loop flag (rows are duplicate-free) -- no gathers
+    uint32_t m = 0;
+    int32_t od[TRIM_V], id[TRIM_V];
+    uint32_t so[TRIM_V], si[TRIM_V];
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++) {
         int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
-        o0[k] = i0[k] = 0;
         od[k] = id[k] = 2;  // out of range: never trimmed
+        so[k] = si[k] = 0;
         if (vv < end) {
-            o0[k] = a.row[vv];
-            od[k] = a.row[vv + 1] - o0[k];
-            i0[k] = a.rowT[vv];
-            id[k] = a.rowT[vv + 1] - i0[k];
+            od[k] = a.row[vv + 1] - a.row[vv];
+            id[k] = a.rowT[vv + 1] - a.rowT[vv];
+            so[k] = a.dcls[0][vv] >> 7;
+            si[k] = a.dcls[1][vv] >> 7;
         }
     }
-    uint32_t m = 0;
 #pragma unroll
-    for (int k = 0; k < TRIM_V; k++) {
-        int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
-        int32_t uo = od[k] == 1 ? a.col[o0[k]] : -1, ui = id[k] == 1 ? a.colT[i0[k]] : -1;
-        bool out = od[k] > 1 || (od[k] == 1 && uo != v);
-        bool in = id[k] > 1 || (id[k] == 1 && ui != v);
-        if (!(out && in)) m |= 1u << k;
-    }
+    for (int k = 0; k < TRIM_V; k++)
+        if (od[k] - (int32_t)so[k] <= 0 || id[k] - (int32_t)si[k] <= 0) m |= 1u << k;
     uint32_t w[TRIM_V];
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++)
