@@ -39,6 +39,7 @@ struct scc_ctx {
     int32_t *d_huge[4] = {nullptr, nullptr, nullptr, nullptr};  // deferred huge rows [dir * 2 + parity]
     uint32_t *d_stamp = nullptr;  // trim levels: last level that checked each vertex
     uint8_t *d_dcls = nullptr;    // degree classes: out-degrees [0, n), in-degrees [n, 2n)
+    int32_t *d_head = nullptr;    // first entry of each row: out-rows [0, n), in-rows [n, 2n)
     uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
@@ -237,6 +238,8 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
         a.skip_fb = skip_fb ? 1 : 0;
         a.dcls[0] = c->d_dcls;  // self-loop bits for the degree test
         a.dcls[1] = c->d_dcls + c->g.n;
+        a.head[0] = c->d_head;
+        a.head[1] = c->d_head + c->g.n;
         a.dirs = 1;
     }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
@@ -287,6 +290,8 @@ struct ReachChain {
                                                  : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
         a.dcls[0] = c->d_dcls;
         a.dcls[1] = c->d_dcls + c->g.n;
+        a.head[0] = c->d_head;
+        a.head[1] = c->d_head + c->g.n;
         a.pull_cost = c->opts.hybrid_policy == 2 ? 1 : 0;
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
@@ -471,7 +476,7 @@ void free_graph(scc_ctx *c) {
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
                         &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
                         &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
-                        &c->d_huge[3], (int32_t **)&c->d_stamp, (int32_t **)&c->d_dcls};
+                        &c->d_huge[3], (int32_t **)&c->d_stamp, (int32_t **)&c->d_dcls, &c->d_head};
     for (auto b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -703,6 +708,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     for (auto &q : c->d_huge) HIPCHK(c, hipMalloc(&q, vb));
     HIPCHK(c, hipMalloc(&c->d_stamp, vb));
     HIPCHK(c, hipMalloc(&c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, hipMalloc(&c->d_head, 2 * vb));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -728,8 +734,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     Launch l = launch_for(c);
     launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
-    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls);
-    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n);
+    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head);
+    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());

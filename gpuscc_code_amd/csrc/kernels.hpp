@@ -45,6 +45,8 @@ struct ReachArgs {
     unsigned long long pull_arcs;  // hybrid: ... or one with at least this many (estimated) arcs
     int pull_cost;                 // hybrid: pull when frontier arcs >= 2 x (live - reached) instead
     const uint8_t *dcls[2];  // per vertex and direction: bit length of the degree | 0x80 for a self-loop
+    const int32_t *head[2];  // per vertex and direction: first entry of the row (-1: empty row), read
+                             // in place of col[row[v]] by the scans that stop at the first match
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int instr;          // count work (expanded vertices, scanned arcs, visits)
@@ -73,7 +75,9 @@ void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsi
 void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t bytes);
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 // out[v] = bit length of row[v + 1] - row[v], | 0x80 when the row holds v (a self-loop)
-void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out);
+// and head[v] = col[row[v]] (-1 for an empty row)
+void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
+                         int32_t *head);
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);  // what 512: clear ctl->abort
 // after a chain of last + 1 levels launched without a wait: depth and convergence check
 void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage);

@@ -76,7 +76,7 @@ __global__ void k_fill_i32(int32_t *p, int64_t count, int32_t val) {
 
 // bits 0-5: bit length of the row's length; bit 7: the row holds v itself
 __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *__restrict__ col, int32_t n,
-                               uint8_t *out) {
+                               uint8_t *out, int32_t *head) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (int64_t)gridDim.x * blockDim.x) {
         int32_t e0 = row[v], e1 = row[v + 1];
@@ -84,6 +84,7 @@ __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *_
         bool self = false;
         for (int32_t e = e0; e < e1 && !self; e++) self = col[e] == (int32_t)v;
         out[v] = (uint8_t)((deg ? 32 - __clz(deg) : 0) | (self ? 0x80 : 0));
+        head[v] = e0 < e1 ? col[e0] : -1;
     }
 }
 
@@ -521,8 +522,9 @@ void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsi
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
     hipLaunchKernelGGL(k_fill_i32, GRID(l), p, count, v);
 }
-void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out) {
-    hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out);
+void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
+                         int32_t *head) {
+    hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out, head);
 }
 void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage) {
     hipLaunchKernelGGL(k_chain_guard, dim3(1), dim3(64), 0, s, ctl, last, stage);

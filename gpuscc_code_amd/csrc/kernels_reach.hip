@@ -638,27 +638,28 @@ __device__ inline void pull_dirs(const ReachArgs &a, uint32_t &qs0, uint32_t &qs
 
 // ---------------------------------------------------------------------------
 // Row scans with early exit, several per work-item in lockstep.  Task t scans
-// col[t][e[t] .. e1[t]) for a neighbour u != self[t] whose word w has
+// col[t][e[t] - 1 .. e1[t]) for a neighbour u != self[t] whose word w has
 // (w & mask[t]) == val[t] and colour c[t]; bit t of the result says it found
-// one.  One lane's scans are independent, so their gathers overlap: a lane
-// waits for one round trip per arc position instead of one per arc.  At most
-// max_steps arcs per task: tasks still open stay set in act, e[t] at their
-// next arc.
+// one.  The first entry is not gathered from the column array: the caller
+// passes it as u0[t], read from the head array (one coalesced word per vertex)
+// together with the row bounds, e[t] being the row start + 1.  One lane's
+// scans are independent, so their gathers overlap: a lane waits for one round
+// trip per arc position instead of one per arc.  Tasks to scan are set in act
+// (their rows non-empty).
 // ---------------------------------------------------------------------------
 constexpr int SCAN_ILP = 4;  // vertices per work-item per pass of the sweeps below
 
 template <int T>
 __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const (&col)[T], int32_t (&e)[T],
-                                     const int32_t (&e1)[T], const uint32_t (&mask)[T],
+                                     const int32_t (&e1)[T], const int32_t (&u0)[T], const uint32_t (&mask)[T],
                                      const uint32_t (&val)[T], const uint32_t (&c)[T],
-                                     const int32_t (&self)[T], uint32_t &act, int max_steps,
-                                     unsigned long long &ce) {
+                                     const int32_t (&self)[T], uint32_t act, unsigned long long &ce) {
     uint32_t hit = 0;
-    for (int s = 0; act && s < max_steps; s++) {
+    for (int s = 0; act; s++) {
         uint32_t wu[T];
         int32_t u[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) u[t] = ((act >> t) & 1u) ? col[t][e[t]] : -1;
+        for (int t = 0; t < T; t++) u[t] = ((act >> t) & 1u) ? (s == 0 ? u0[t] : col[t][e[t]]) : -1;
 #pragma unroll
         for (int t = 0; t < T; t++) wu[t] = u[t] >= 0 ? word[u[t]] : 0u;
 #pragma unroll
@@ -668,7 +669,7 @@ __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const 
             if (u[t] != self[t] && (wu[t] & mask[t]) == val[t] && colour_of(wu[t]) == c[t]) {
                 hit |= 1u << t;
                 act &= ~(1u << t);
-            } else if (++e[t] >= e1[t]) {
+            } else if (s > 0 ? ++e[t] >= e1[t] : e[t] >= e1[t]) {
                 act &= ~(1u << t);
             }
         }
@@ -770,7 +771,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
 #pragma unroll
                 for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
                 const int32_t *col[T];
-                int32_t e[T], e1[T], self[T];
+                int32_t e[T], e1[T], u0[T], self[T];
                 uint32_t mask[T], val[T], c[T], act = 0;
 #pragma unroll
                 for (int t = 0; t < T; t++) {
@@ -782,15 +783,17 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                     val[t] = dbit(d);
                     c[t] = colour_of(w[j]);
                     self[t] = -1;
+                    u0[t] = -1;
                     e[t] = e1[t] = 0;
                     if ((d == 0 ? pull0 : pull1) && !(w[j] & (ST_E | dbit(d)))) {
-                        e[t] = row[v[j]];
+                        u0[t] = a.head[1 - d][v[j]];
+                        e[t] = row[v[j]] + 1;
                         e1[t] = row[v[j] + 1];
                         cx++;
-                        if (e[t] < e1[t]) act |= 1u << t;
+                        if (u0[t] >= 0) act |= 1u << t;
                     }
                 }
-                uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
+                uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     if (!((hit >> t) & 1u)) continue;
@@ -824,8 +827,15 @@ __global__ void k_reach_hybrid(ReachArgs a) {
 // level trimmed so many that sweeping again is cheaper.
 // ---------------------------------------------------------------------------
 __device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                               const uint32_t *word, int32_t v, uint32_t c) {
-    for (int32_t e = row[v]; e < row[v + 1]; e++) {
+                               const int32_t *__restrict__ head, const uint32_t *word, int32_t v, uint32_t c) {
+    const int32_t h = head[v];  // the row's first entry, loaded with the row bounds
+    const int32_t e0 = row[v], e1 = row[v + 1];
+    if (h < 0) return false;
+    if (h != v) {
+        uint32_t wh = word[h];
+        if (!(wh & ST_E) && colour_of(wh) == c) return true;
+    }
+    for (int32_t e = e0 + 1; e < e1; e++) {
         int32_t u = col[e];
         if (u == v) continue;
         uint32_t wu = word[u];
@@ -836,7 +846,8 @@ __device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *_
 
 // trims v (colour c) when it has no live same-colour in- or out-neighbour
 __device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c) {
-    if (live_nb(a.row, a.col, a.word, v, c) && live_nb(a.rowT, a.colT, a.word, v, c)) return false;
+    if (live_nb(a.row, a.col, a.head[0], a.word, v, c) && live_nb(a.rowT, a.colT, a.head[1], a.word, v, c))
+        return false;
     a.word[v] = make_word(c, 28);  // P|T|E (scc_routines.cu:381)
     a.root[v] = v;
     return true;
@@ -911,7 +922,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
         const int32_t *col[T];
-        int32_t e[T], e1[T], self[T];
+        int32_t e[T], e1[T], u0[T], self[T];
         uint32_t mask[T], val[T], c[T], act = 0;
 #pragma unroll
         for (int t = 0; t < T; t++) {
@@ -922,15 +933,17 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             val[t] = 0;
             c[t] = colour_of(w[j]);
             self[t] = v[j];
+            u0[t] = -1;
             e[t] = e1[t] = 0;
             if (!(w[j] & ST_E)) {
-                e[t] = row[v[j]];
+                u0[t] = a.head[d][v[j]];
+                e[t] = row[v[j]] + 1;
                 e1[t] = row[v[j] + 1];
-                if (e[t] < e1[t]) act |= 1u << t;
+                if (u0[t] >= 0) act |= 1u << t;
             }
         }
         // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
-        uint32_t hit = scan_rows<T>(a.word, col, e, e1, mask, val, c, self, act, 1 << 30, ce);
+        uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce);
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) {
             if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
