@@ -197,7 +197,7 @@ scc_status run_chain(scc_ctx *c, F &&launch_level, int &guess, int reset_mask, i
                                             (double)h.qarcs[0][i], (double)h.qarcs[1][i],
                                             (double)(kind == 0 ? h.pull_mask : 0),
                                             (double)(h.st_expanded - last[0]), (double)(h.st_edges - last[1]),
-                                            (double)(h.st_visited - last[2])};
+                                            (double)(h.st_visited - last[2]), (double)h.st_maxscan[i]};
             last[0] = h.st_expanded;
             last[1] = h.st_edges;
             last[2] = h.st_visited;
@@ -236,6 +236,7 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
         a.stamp = c->d_stamp;
         a.fresh = fresh ? 1 : 0;
         a.skip_fb = skip_fb ? 1 : 0;
+        a.instr = c->instr;
         a.dcls[0] = c->d_dcls;  // self-loop bits for the degree test
         a.dcls[1] = c->d_dcls + c->g.n;
         a.head[0] = c->d_head;

@@ -56,6 +56,8 @@ struct Ctl {
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     uint32_t creach[2][4];  // vertices reached so far by each direction (ring: level iter writes [iter & 3])
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
+    uint32_t st_maxscan[4];  // instrumented: longest serial row scan (arcs) of one work-item,
+                             // ring: level iter writes [iter & 3]
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

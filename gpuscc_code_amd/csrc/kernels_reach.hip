@@ -225,8 +225,19 @@ __device__ inline bool level_begin(const ReachArgs &a, bool queues) {
             a.ctl->qsize[threadIdx.x][(a.iter + 2) & 3] = 0;
             a.ctl->qarcs[threadIdx.x][(a.iter + 2) & 3] = 0;
         }
+        if (threadIdx.x == 0) a.ctl->st_maxscan[(a.iter + 2) & 3] = 0;
     }
     return true;
+}
+
+// instrumented: the level's longest serial row scan (wave maximum, one atomic per wave)
+__device__ inline void instr_maxscan(const ReachArgs &a, uint32_t steps) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t x = (uint32_t)__shfl_xor((int)steps, o, 64);
+        steps = x > steps ? x : steps;
+    }
+    if ((threadIdx.x & 63) == 0 && steps) atomicMax(&a.ctl->st_maxscan[a.iter & 3], steps);
 }
 
 // a row of at least huge_thr arcs is not expanded by its own lane or
@@ -653,9 +664,11 @@ template <int T>
 __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const (&col)[T], int32_t (&e)[T],
                                      const int32_t (&e1)[T], const int32_t (&u0)[T], const uint32_t (&mask)[T],
                                      const uint32_t (&val)[T], const uint32_t (&c)[T],
-                                     const int32_t (&self)[T], uint32_t act, unsigned long long &ce) {
+                                     const int32_t (&self)[T], uint32_t act, unsigned long long &ce,
+                                     uint32_t &steps) {
     uint32_t hit = 0;
-    for (int s = 0; act; s++) {
+    int s = 0;
+    for (; act; s++) {
         uint32_t wu[T];
         int32_t u[T];
 #pragma unroll
@@ -674,6 +687,7 @@ __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const 
             }
         }
     }
+    steps = (uint32_t)s > steps ? (uint32_t)s : steps;
     return hit;
 }
 
@@ -727,6 +741,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     pull_dirs(a, qs0, qs1, pull0, pull1);
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
+    uint32_t smax = 0;  // longest row scan of this work-item (instrumented)
     if (blockIdx.x == 0 && threadIdx.x < 2) {
         if (threadIdx.x == 0) a.ctl->pull_mask = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
         const int d = (int)threadIdx.x;  // vertices reached through this level's input
@@ -793,7 +808,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                         if (u0[t] >= 0) act |= 1u << t;
                     }
                 }
-                uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce);
+                uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     if (!((hit >> t) & 1u)) continue;
@@ -815,7 +830,10 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     local_rounds(a, (pull0 || pull1) ? 0xFFFFFFFFu : qs0 + qs1, st, lq, found, cx, ce, cv);
     st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
-    if (a.instr) instr_flush(a.ctl, cx, ce, cv, sh);
+    if (a.instr) {
+        instr_flush(a.ctl, cx, ce, cv, sh);
+        instr_maxscan(a, smax);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -827,15 +845,18 @@ __global__ void k_reach_hybrid(ReachArgs a) {
 // level trimmed so many that sweeping again is cheaper.
 // ---------------------------------------------------------------------------
 __device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                               const int32_t *__restrict__ head, const uint32_t *word, int32_t v, uint32_t c) {
+                               const int32_t *__restrict__ head, const uint32_t *word, int32_t v, uint32_t c,
+                               uint32_t &steps) {
     const int32_t h = head[v];  // the row's first entry, loaded with the row bounds
     const int32_t e0 = row[v], e1 = row[v + 1];
     if (h < 0) return false;
+    steps++;
     if (h != v) {
         uint32_t wh = word[h];
         if (!(wh & ST_E) && colour_of(wh) == c) return true;
     }
     for (int32_t e = e0 + 1; e < e1; e++) {
+        steps++;
         int32_t u = col[e];
         if (u == v) continue;
         uint32_t wu = word[u];
@@ -845,9 +866,12 @@ __device__ inline bool live_nb(const int32_t *__restrict__ row, const int32_t *_
 }
 
 // trims v (colour c) when it has no live same-colour in- or out-neighbour
-__device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c) {
-    if (live_nb(a.row, a.col, a.head[0], a.word, v, c) && live_nb(a.rowT, a.colT, a.head[1], a.word, v, c))
-        return false;
+__device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c, uint32_t &smax) {
+    uint32_t steps = 0;
+    const bool keep = live_nb(a.row, a.col, a.head[0], a.word, v, c, steps) &&
+                      live_nb(a.rowT, a.colT, a.head[1], a.word, v, c, steps);
+    smax = steps > smax ? steps : smax;
+    if (keep) return false;
     a.word[v] = make_word(c, 28);  // P|T|E (scc_routines.cu:381)
     a.root[v] = v;
     return true;
@@ -899,7 +923,7 @@ __device__ inline uint32_t trim_round_fresh(const ReachArgs &a, int64_t base, in
 // for a live same-colour neighbour (other than themselves), SCAN_ILP vertices
 // at a time in lockstep; those missing one in either direction are trimmed.
 __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int64_t end,
-                                           unsigned long long &ce, uint32_t &newp) {
+                                           unsigned long long &ce, uint32_t &newp, uint32_t &smax) {
     uint32_t cand = 0;
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++) {
@@ -943,7 +967,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             }
         }
         // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
-        uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce);
+        uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) {
             if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
@@ -965,6 +989,7 @@ __global__ void k_trim_level(ReachArgs a) {
     st_init(st);
     __syncthreads();
     bool found = false;
+    uint32_t smax = 0;  // longest row scan of this work-item (instrumented)
     const uint32_t qs = a.iter == 0 ? 0u : a.ctl->qsize[0][a.iter & 3];
     if (a.iter == 0 || qs >= (uint32_t)(a.n / TRIM_SWEEP_DIV)) {
         // sweep: rounds of TRIM_V vertices per work-item, no barrier until the
@@ -980,7 +1005,7 @@ __global__ void k_trim_level(ReachArgs a) {
                 m = trim_round_fresh(a, base, end);
                 newp = (uint32_t)__popc(m);  // fresh words hold no P
             } else {
-                m = trim_round_scan(a, base, end, ce, newp);
+                m = trim_round_scan(a, base, end, ce, newp, smax);
             }
             if (m) found = true;
             const uint32_t claimed[2] = {m, 0u};
@@ -1026,7 +1051,7 @@ __global__ void k_trim_level(ReachArgs a) {
                     if (!(wu & ST_E) && colour_of(wu) == cc &&
                         !(a.skip_fb && (wu & (ST_P | ST_F | ST_B)) == (ST_F | ST_B)) &&
                         a.stamp[u] != a.tag &&  // plain load first: hubs are reached by many arcs
-                        atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc)) {
+                        atomicExch(&a.stamp[u], a.tag) != a.tag && trim_check(a, u, cc, smax)) {
                         found = true;
                         ntrim++;
                         newp += (wu & ST_P) ? 0u : 1u;  // a trimmed pivot holds P already
@@ -1043,6 +1068,7 @@ __global__ void k_trim_level(ReachArgs a) {
     }
     st_flush(st, a);
     chain_progress(a.ctl, a.iter, found);
+    if (a.instr) instr_maxscan(a, smax);
 }
 
 __global__ void k_clear_bits(uint32_t *word, int32_t n, uint32_t bits) {

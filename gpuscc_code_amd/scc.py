@@ -24,7 +24,7 @@ PIVOT_PLAIN, PIVOT_LEARNED = 0, 1
 ST_B, ST_F, ST_E, ST_T, ST_P, ST_BX, ST_FX = 1, 2, 4, 8, 16, 32, 64
 MLP_PARAMS, MLP_FEATURES, MLP_HIDDEN = 1505, 12, 32
 LEVEL_FIELDS = ("kind", "level", "ms", "queue_f", "queue_b", "arcs_f", "arcs_b", "pulled",
-                "expanded", "edges", "visited")  # one row of scc_profile_levels
+                "expanded", "edges", "visited", "max_scan")  # one row of scc_profile_levels
 COLOUR_MASK = 0x00FFFFFF
 
 

@@ -236,9 +236,11 @@ scc_status scc_profile_copy(scc_ctx *ctx, int64_t bytes);
 /* The level log of flag 4, SCC_LEVEL_FIELDS doubles per row: kind (0 reach,
  * 1 trim), level, kernel ms, input queue F, input queue B, estimated arcs of
  * queue F, of queue B, pulled directions (bit 0 F, bit 1 B), expanded vertices,
- * scanned arcs, new visits (the last three for reach levels).  Copies up to
- * cap rows, sets *rows to the rows logged; reset clears the log. */
-#define SCC_LEVEL_FIELDS 11
+ * scanned arcs, new visits (these three for reach levels), and the longest
+ * early-exit row scan of one work-item in arcs (pull levels and trim levels).
+ * Flag 4 implies flags 1 and 2.  Copies up to cap rows, sets *rows to the
+ * rows logged; reset clears the log. */
+#define SCC_LEVEL_FIELDS 12
 scc_status scc_profile_levels(scc_ctx *ctx, double *out, int64_t cap, int64_t *rows, int reset);
 
 /* ---- host input path (no GPU needed) ------------------------------------- */

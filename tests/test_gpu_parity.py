@@ -431,3 +431,40 @@ def test_pull_cost_policy(ctx):
         ctx.set_batch(None)
     finally:
         ctx.set_opts(scc.alg_opt())
+
+
+def test_level_log(ctx):
+    """Profile flag 4 logs one row per level launched: each chain is a run of
+    levels 0, 1, 2, ...; one reach chain per FB step; a level's longest serial
+    row scan is bounded by its scanned arcs, is zero on push-only levels and
+    on the degree-test trim level, and positive on levels that pulled arcs in
+    both directions.  Logging changes no label."""
+    g = scc.gen_rmat(14, 8, seed=7)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    ctx.set_opts(scc.alg_opt(hybrid_threshold=64))  # frontiers of 64 or more pull
+    ctx.profile(4)
+    try:
+        labels, st = ctx.run("h")
+        rows = ctx.profile_levels()
+    finally:
+        ctx.profile(0)
+        ctx.profile_read(reset=True)
+        ctx.set_opts(scc.alg_opt())
+    assert np.array_equal(labels, ref)
+    f = {k: i for i, k in enumerate(scc.LEVEL_FIELDS)}
+    assert rows.shape[1] == len(f) and len(rows) > 0
+    kind, level = rows[:, f["kind"]], rows[:, f["level"]]
+    assert set(np.unique(kind)) <= {0.0, 1.0}
+    for i in range(len(rows)):
+        if level[i] > 0:
+            assert kind[i - 1] == kind[i] and level[i - 1] == level[i] - 1
+    assert (rows[:, f["ms"]] > 0).all()
+    assert int(np.sum((kind == 0) & (level == 0))) == st["steps"]
+    reach = rows[kind == 0]
+    scan, edges, pulled = reach[:, f["max_scan"]], reach[:, f["edges"]], reach[:, f["pulled"]]
+    assert (scan <= edges).all()
+    assert (scan[pulled == 0] == 0).all()
+    both = (pulled == 3) & (edges > 0)
+    assert both.any() and (scan[both] >= 1).all()
+    assert rows[0, f["kind"]] == 1 and rows[0, f["max_scan"]] == 0  # fresh trim: degree test only
