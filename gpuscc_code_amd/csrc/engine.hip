@@ -220,11 +220,11 @@ void clear_graphs(scc_ctx *c) {
 }
 
 // Level launchers of the two chains (one object per chain run).
-struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs never leave a colour)
+struct TrimChain {
     scc_ctx *c;
     Launch l;
     ReachArgs a{};
-    TrimChain(scc_ctx *cc, bool fresh, bool skip_fb = false) : c(cc), l(fixed_launch(cc)) {
+    explicit TrimChain(scc_ctx *cc, bool skip_fb = false) : c(cc), l(fixed_launch(cc)) {
         a.row = c->g.row;
         a.col = c->g.col;
         a.rowT = c->g.rowT;
@@ -234,11 +234,8 @@ struct TrimChain {  // fresh: the words are as k_init left them (all live, arcs 
         a.ctl = c->d_ctl;
         a.root = c->d_root;
         a.stamp = c->d_stamp;
-        a.fresh = fresh ? 1 : 0;
         a.skip_fb = skip_fb ? 1 : 0;
         a.instr = c->instr;
-        a.dcls[0] = c->d_dcls;  // self-loop bits for the degree test
-        a.dcls[1] = c->d_dcls + c->g.n;
         a.head[0] = c->d_head;
         a.head[1] = c->d_head + c->g.n;
         a.dirs = 1;
@@ -351,10 +348,11 @@ scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
 
 // resume > 0: continue a chain whose first `resume` levels already ran;
 // after_reach: the trim of an FB step (the reach from the pivots is complete)
-scc_status do_trim(scc_ctx *c, int64_t *iters, bool fresh = false, int64_t resume = 0, bool after_reach = false) {
-    TrimChain tc(c, fresh, after_reach);
+// first: trim1 after k_init (its guess slot); after_reach: right after a complete reach
+scc_status do_trim(scc_ctx *c, int64_t *iters, bool first = false, int64_t resume = 0, bool after_reach = false) {
+    TrimChain tc(c, after_reach);
     if (!resume) launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
-    return run_chain(c, tc, c->guess[fresh ? 3 : 2], 2, iters, 1, resume);
+    return run_chain(c, tc, c->guess[first ? 3 : 2], 2, iters, 1, resume);
 }
 
 scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, int64_t *depth,
@@ -779,13 +777,14 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     scc_status s = SCC_OK;
     auto body = [&]() -> scc_status {
         Launch l = launch_for(c);
-        launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs,
+        // k_init runs the first level of trim1 (a degree test) on the words it writes
+        launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
                     c->d_ctl);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();
-        CHK(do_trim(c, &it, true));  // trim1 (scc.cu:232)
-        st.trim_iters += it;
+        CHK(do_trim(c, &it, true));  // the rest of trim1 (scc.cu:232)
+        st.trim_iters += it + 1;
         st.ms_trim += ms_since(tp);
         // The device counts live vertices: after trim-1 every live vertex still
         // has its initial colour, so live vertices mean at least one pivot and
@@ -834,7 +833,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 }
                 launch_chain_guard(c->stream, c->d_ctl, Lr - 1, 1);
                 HIPCHK(c, hipEventRecord(c->spec_ev[1], c->stream));
-                TrimChain tc(c, false, true);
+                TrimChain tc(c, true);
                 launch_ctl_reset(c->stream, c->d_ctl, 128);
                 launch_ctl_reset(c->stream, c->d_ctl, 2 | 1);
                 for (int i = 0; i < Lt; i++) tc(i, nullptr, nullptr);

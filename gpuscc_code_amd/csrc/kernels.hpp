@@ -54,7 +54,6 @@ struct ReachArgs {
     int32_t *root;      // trimmed vertices become their own SCC
     uint32_t *stamp;    // per-vertex tag of the last trim level that checked it
     uint32_t tag;       // this level's tag (unique per launch)
-    int fresh;          // every vertex live, every arc inside one colour: level 0 tests degrees only
     int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
                         // two or more vertices, which no trim removes, so those vertices are not checked
 };
@@ -66,9 +65,12 @@ struct Graph {  // device CSR + CSR^T
 };
 
 // kernels_fb.hip
-// colour 1 everywhere, or graph id + 1 when off (G + 1 offsets) is given; ctl->live = n
+// colour 1 everywhere, or graph id + 1 when off (G + 1 offsets) is given; with cls
+// (degree classes, out [0, n) and in [n, 2n)) the vertices whose out- or in-row
+// holds no vertex but themselves start trimmed (the first level of trim1).
+// ctl->live = live vertices, ctl->pcount = trimmed ones, ctl->abort = 0.
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, Ctl *ctl);
+                 int32_t G, const uint8_t *cls, Ctl *ctl);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);

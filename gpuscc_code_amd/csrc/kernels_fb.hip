@@ -24,18 +24,35 @@ __device__ inline uint32_t graph_of(const int32_t *off, int32_t G, int32_t v) {
 
 // init: colour 1 (thrust::fill, scc.cu:214) or, for a batch, graph id + 1;
 // state 0; root = self
-__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G, Ctl *ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->live = (uint32_t)n;
-        ctl->pcount = 0;
-        ctl->abort = 0;
-    }
+// A row of degree class k (bit length of its length, 0x80: it holds v itself)
+// has no entry other than v iff k is 0, or 1 with the self-loop flag.
+__device__ inline bool row_has_other(uint32_t k) { return (k & 0x3Fu) != 0 && (k & 0xBFu) != 0x81u; }
+
+// Fresh words: colour 1, or graph id + 1 in a batch; root = v.  With the
+// degree classes given, the first level of trim1 (trim1_kernel,
+// scc_routines.cu:276-332) is folded in: every arc lies inside one colour and
+// every vertex is live, so v has a live same-colour neighbour in a direction
+// iff its row there holds a vertex other than v -- a test of two class bytes.
+// Vertices failing it start trimmed (P|T|E, scc_routines.cu:381).  Adds the
+// live and trimmed counts to ctl->live and ctl->pcount (zeroed before).
+__global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
+                       const uint8_t *cls_out, const uint8_t *cls_in, Ctl *ctl) {
+    __shared__ unsigned long long sh[16];
     int64_t beg, end;
     my_range(n, beg, end);
+    unsigned long long nt = 0;
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
         uint32_t c = off ? ((graph_of(off, G, (int32_t)v) + 1u) & COLOUR_MASK) : 1u;
-        word[v] = make_word(c, 0);
+        const bool trim = cls_out && !(row_has_other(cls_out[v]) && row_has_other(cls_in[v]));
+        word[v] = make_word(c, trim ? 28u : 0u);
         root[v] = (int32_t)v;
+        nt += trim ? 1u : 0u;
+    }
+    nt = block_sum(nt, sh);
+    if (threadIdx.x == 0) {
+        const unsigned long long cnt = end > beg ? (unsigned long long)(end - beg) : 0ull;
+        if (cnt > nt) atomicAdd(&ctl->live, (uint32_t)(cnt - nt));
+        if (nt) atomicAdd(&ctl->pcount, (uint32_t)nt);
     }
 }
 
@@ -103,12 +120,17 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
     if (threadIdx.x != 0) return;
     if (what & 512) ctl->abort = 0;  // the host has handled the stage that stopped
     else if (ctl->abort) return;
+    if (what & 1024) {  // before k_init
+        ctl->live = 0;
+        ctl->pcount = 0;
+    }
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
         for (int d = 0; d < 2; d++)
             for (int i = 0; i < 4; i++) ctl->hsize[d][i] = 0;
         ctl->ran = 0;
         ctl->pull_levels = 0;
+        for (int i = 0; i < 4; i++) ctl->st_maxscan[i] = 0;
     }
     if (what & 2) {
         // queue chains: level 0 reads the seed lists
@@ -507,8 +529,9 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, Ctl *ctl) {
-    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, ctl);
+                 int32_t G, const uint8_t *cls, Ctl *ctl) {
+    launch_ctl_reset(l.stream, ctl, 512 | 1024);
+    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr, ctl);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);

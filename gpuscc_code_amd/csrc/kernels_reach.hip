@@ -880,45 +880,6 @@ __device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c, uin
 constexpr int TRIM_V = 16;           // vertices per work-item per round of a trim sweep
 constexpr int32_t TRIM_SWEEP_DIV = 16;  // a level after one that trimmed >= n/16 vertices sweeps
 
-// Fresh words (every vertex live, arcs inside colours): v keeps a live
-// same-colour neighbour in a direction iff its row there holds a vertex other
-// than itself (rows are duplicate-free, so two entries suffice).  Returns the
-// round's trimmed vertices as a mask (bit k: base + k * blockDim.x + threadIdx.x).
-__device__ inline uint32_t trim_round_fresh(const ReachArgs &a, int64_t base, int64_t end) {
-    // a row keeps a live neighbour iff it holds a vertex other than v: its
-    // length minus v's self-loop flag (rows are duplicate-free) -- no gathers
-    uint32_t m = 0;
-    int32_t od[TRIM_V], id[TRIM_V];
-    uint32_t so[TRIM_V], si[TRIM_V];
-#pragma unroll
-    for (int k = 0; k < TRIM_V; k++) {
-        int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
-        od[k] = id[k] = 2;  // out of range: never trimmed
-        so[k] = si[k] = 0;
-        if (vv < end) {
-            od[k] = a.row[vv + 1] - a.row[vv];
-            id[k] = a.rowT[vv + 1] - a.rowT[vv];
-            so[k] = a.dcls[0][vv] >> 7;
-            si[k] = a.dcls[1][vv] >> 7;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < TRIM_V; k++)
-        if (od[k] - (int32_t)so[k] <= 0 || id[k] - (int32_t)si[k] <= 0) m |= 1u << k;
-    uint32_t w[TRIM_V];
-#pragma unroll
-    for (int k = 0; k < TRIM_V; k++)
-        w[k] = ((m >> k) & 1u) ? a.word[base + (int64_t)k * blockDim.x + threadIdx.x] : 0u;
-#pragma unroll
-    for (int k = 0; k < TRIM_V; k++) {
-        if (!((m >> k) & 1u)) continue;
-        int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
-        a.word[v] = make_word(colour_of(w[k]), 28);  // P|T|E (scc_routines.cu:381)
-        a.root[v] = v;
-    }
-    return m;
-}
-
 // General words: the live vertices of the round scan their out- and in-rows
 // for a live same-colour neighbour (other than themselves), SCAN_ILP vertices
 // at a time in lockstep; those missing one in either direction are trimmed.
@@ -1000,13 +961,7 @@ __global__ void k_trim_level(ReachArgs a) {
         const unsigned long long none[2] = {0ull, 0ull};
         for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * TRIM_V) {  // block-uniform
             uint32_t newp = 0;
-            uint32_t m;
-            if (a.iter == 0 && a.fresh) {
-                m = trim_round_fresh(a, base, end);
-                newp = (uint32_t)__popc(m);  // fresh words hold no P
-            } else {
-                m = trim_round_scan(a, base, end, ce, newp, smax);
-            }
+            const uint32_t m = trim_round_scan(a, base, end, ce, newp, smax);
             if (m) found = true;
             const uint32_t claimed[2] = {m, 0u};
             commit_round(a, base, claimed, none, it.shw, st.base, sh, true, newp);
