@@ -61,8 +61,11 @@ struct scc_ctx {
     int adam_t = 0;
     int cur_step = 0;
     bool uploaded = false, labels_valid = false;
+    bool label_in_root = false;  // the last run's labels are the root array itself (plain rule)
     int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
     int64_t syncs = 0;
+    // control-block work queued for the next k_ctl_reset launch (see ctl_op)
+    int pend_bits = 0, pend_last = 0, pend_stage = 0;
     int instr = 0;                  // profile flag 1: device work counters
     int prof_flags = 0;
     scc_profile_data prof{};
@@ -130,7 +133,35 @@ Launch fixed_launch(const scc_ctx *c) {  // kernels with LDS sized for 256 work-
     return l;
 }
 
+// Control-block operations are batched: reset bits and a chain guard queued
+// here go out with the next ctl_op, as one single-work-item kernel instead of
+// several.  read_ctl sends a queued guard (and any reset but bit 64) first;
+// bit 64 (seed lists from the pivot count) may stay queued across a read, as
+// only the next chain start (bit 2, applied after it) reads the seeds.
+void ctl_op(scc_ctx *c, int bits) {
+    launch_ctl_reset(c->stream, c->d_ctl, c->pend_bits | bits, c->pend_last, c->pend_stage);
+    c->pend_bits = 0;
+    c->pend_stage = 0;
+}
+void ctl_queue(scc_ctx *c, int bits) { c->pend_bits |= bits; }
+// the guard of a chain of last + 1 levels launched without a wait (stage 1, 2)
+void ctl_guard(scc_ctx *c, int last, int stage) {
+    if (c->pend_stage) ctl_op(c, 0);
+    c->pend_last = last;
+    c->pend_stage = stage;
+}
+
+// Before a call that writes the root array: labels held there move to the label buffer.
+scc_status detach_labels(scc_ctx *c) {
+    if (c->labels_valid && c->label_in_root && c->g.n > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_label, c->d_root, sizeof(int32_t) * (size_t)c->g.n, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    c->label_in_root = false;
+    return SCC_OK;
+}
+
 scc_status read_ctl(scc_ctx *c) {
+    if (c->pend_stage || (c->pend_bits & ~64)) ctl_op(c, 0);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -151,10 +182,12 @@ scc_status ensure_events(scc_ctx *c, size_t count) {
 // launch_level(iter, ev_start, ev_stop) launches level `iter`; with kernel
 // timing on (profile flag 2) the events bracket the kernel itself.
 // kind 0 = reachability, 1 = trim, 2 = other (for the profile counters).
+// The chain starts with control-block reset reset_mask | 1, or as the caller
+// left it when reset_mask < 0; resume > 0 continues a chain from that level.
 template <class F>
 scc_status run_chain(scc_ctx *c, F &&launch_level, int &guess, int reset_mask, int64_t *levels,
                      int kind = 2, int64_t resume = 0) {
-    if (!resume) launch_ctl_reset(c->stream, c->d_ctl, reset_mask | 1);
+    if (!resume && reset_mask >= 0) ctl_op(c, reset_mask | 1);
     int64_t launched = resume;
     const bool logged = (c->prof_flags & 4) && kind < 2;
     int L = logged ? 1 : std::max(1, std::min(guess, 64));
@@ -348,17 +381,18 @@ scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
 
 // resume > 0: continue a chain whose first `resume` levels already ran;
 // after_reach: the trim of an FB step (the reach from the pivots is complete)
-// first: trim1 after k_init (its guess slot); after_reach: right after a complete reach
+// first: trim1 after k_init (its guess slot; k_init's reset started the chain);
+// after_reach: right after a complete reach.  Level 0 sweeps: it reads no
+// input queue, so the seed lists stay as they are.
 scc_status do_trim(scc_ctx *c, int64_t *iters, bool first = false, int64_t resume = 0, bool after_reach = false) {
     TrimChain tc(c, after_reach);
-    if (!resume) launch_ctl_reset(c->stream, c->d_ctl, 128);  // level 0 seeds nothing from queues
-    return run_chain(c, tc, c->guess[first ? 3 : 2], 2, iters, 1, resume);
+    return run_chain(c, tc, c->guess[first ? 3 : 2], first ? -1 : 2, iters, 1, resume);
 }
 
 scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, int64_t *depth,
                     int64_t resume = 0) {
     ReachChain rc(c, dirs, sched, bseed);
-    if (c->instr && !resume) launch_ctl_reset(c->stream, c->d_ctl, 32);
+    if (c->instr && !resume) ctl_queue(c, 32);
     CHK(run_chain(c, rc, c->guess[dirs == 2 ? 1 : 0], rc.reset_mask(), depth, 0, resume));
     rc.account(*depth);
     return SCC_OK;
@@ -390,7 +424,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     int64_t nslots = (int64_t)1 << c->slot_bits;
     if ((c->epoch & 0xFFFFu) < 2u) launch_fill_u64(l, cur, nslots, ~0ull);  // tag wrap
-    launch_ctl_reset(c->stream, c->d_ctl, 4);
+    ctl_op(c, 4);
     if (learned) {
         if (recolour)
             launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits,
@@ -403,7 +437,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
                       recolour, 1, nullptr, c->d_ctl);
     }
     launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
-    launch_ctl_reset(c->stream, c->d_ctl, 64);  // the pivot list seeds both directions
+    ctl_queue(c, 64);  // the pivot list seeds both directions
     c->prev_tab ^= 1;
     if (!sync) return SCC_OK;
     CHK(read_ctl(c));
@@ -413,7 +447,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
 
 
 scc_status do_count(scc_ctx *c, int what, int64_t *out) {
-    launch_ctl_reset(c->stream, c->d_ctl, 8);
+    ctl_op(c, 8);
     launch_count_state(launch_for(c), c->d_word, c->g.n, what, c->d_ctl);
     CHK(read_ctl(c));
     *out = (int64_t)c->h_ctl->count;
@@ -421,7 +455,7 @@ scc_status do_count(scc_ctx *c, int what, int64_t *out) {
 }
 
 scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
-    launch_ctl_reset(c->stream, c->d_ctl, 8);
+    ctl_op(c, 8);
     launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
     CHK(read_ctl(c));
     *pairs = (int64_t)c->h_ctl->count;
@@ -431,7 +465,7 @@ scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
 // update_wcc (wcc.cu:146-226): labels to fixpoint, fresh colours, election
 scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = false, bool sync = true) {
     Launch l = launch_for(c);
-    launch_ctl_reset(c->stream, c->d_ctl, 16);
+    ctl_op(c, 16);
     launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
     launch_wcc_init(l, c->d_tmp, c->g.n);
     int wg = 4;
@@ -772,12 +806,16 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     const int32_t n = c->g.n;
     c->syncs = 0;
     c->cur_step = 0;
+    if (c->label_in_root) c->labels_valid = false;  // the run rewrites the root array
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t_start = std::chrono::steady_clock::now();
     scc_status s = SCC_OK;
     auto body = [&]() -> scc_status {
         Launch l = launch_for(c);
-        // k_init runs the first level of trim1 (a degree test) on the words it writes
+        // k_init runs the first level of trim1 (a degree test) on the words it
+        // writes; its control-block reset also starts trim1's chain
+        c->pend_bits = c->pend_stage = 0;
+        ctl_op(c, 512 | 1024 | 2 | 1);
         launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
                     c->d_ctl);
         if (n == 0) return SCC_OK;
@@ -825,19 +863,18 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 Lt = std::max(1, std::min(Lt, 1024));
                 HIPCHK(c, hipEventRecord(c->spec_ev[0], c->stream));
                 ReachChain rc(c, 3, sched, c->d_piv);
-                launch_ctl_reset(c->stream, c->d_ctl, rc.reset_mask() | 1);
+                ctl_op(c, rc.reset_mask() | 1);
                 if (c->opts.reach_graphs > 0) {
                     CHK(launch_reach_levels(c, rc, Lr));
                 } else {
                     for (int i = 0; i < Lr; i++) rc(i, nullptr, nullptr);
                 }
-                launch_chain_guard(c->stream, c->d_ctl, Lr - 1, 1);
+                ctl_guard(c, Lr - 1, 1);
                 HIPCHK(c, hipEventRecord(c->spec_ev[1], c->stream));
                 TrimChain tc(c, true);
-                launch_ctl_reset(c->stream, c->d_ctl, 128);
-                launch_ctl_reset(c->stream, c->d_ctl, 2 | 1);
+                ctl_op(c, 2 | 1);  // with the reach chain's guard
                 for (int i = 0; i < Lt; i++) tc(i, nullptr, nullptr);
-                launch_chain_guard(c->stream, c->d_ctl, Lt - 1, 2);
+                ctl_guard(c, Lt - 1, 2);  // goes out with the election's first reset
                 HIPCHK(c, hipEventRecord(c->spec_ev[2], c->stream));
                 c->cur_step = (int)st.steps;
                 int64_t unused = 0;
@@ -854,7 +891,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     unsynced = false;
                 }
                 if (np == 0) {
-                    launch_ctl_reset(c->stream, c->d_ctl, 512);
+                    ctl_op(c, 512);
                     return fail(c, SCC_E_STATE, "live vertices left but no pivot elected");
                 }
                 st.pivots_total += np;
@@ -869,7 +906,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 } else {  // undo the election's bookkeeping and finish the step with waits
                     c->epoch--;
                     c->prev_tab ^= 1;
-                    launch_ctl_reset(c->stream, c->d_ctl, 512);
+                    ctl_op(c, 512);
                     if (h.abort == 1) CHK(do_reach(c, 3, sched, c->d_piv, &dr, Lr));
                     st.ms_fwd += ms_since(tp);
                     tp = std::chrono::steady_clock::now();
@@ -945,9 +982,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // themselves and to the lower id); the learned rule needs the minimum pass.
         if (learned)
             launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
-        else
-            HIPCHK(c, hipMemcpyAsync(c->d_label, c->d_root, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice,
-                                     c->stream));
+        c->label_in_root = !learned;
         st.num_scc = (int64_t)c->h_ctl->pcount;
         return SCC_OK;
     };
@@ -958,7 +993,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     st.host_syncs = c->syncs;
     c->labels_valid = true;
     if (labels_out && n > 0) {
-        HIPCHK(c, hipMemcpy(labels_out, c->d_label, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(labels_out, c->label_in_root ? c->d_root : c->d_label, sizeof(int32_t) * (size_t)n,
+                            hipMemcpyDeviceToHost));
         std::vector<int32_t> cnt((size_t)n, 0);
         for (int32_t v = 0; v < n; v++) cnt[labels_out[v]]++;
         for (int32_t v = 0; v < n; v++) {
@@ -1013,7 +1049,7 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
     HIPCHK(c, hipMalloc(&c->d_goff, sizeof(int32_t) * ((size_t)G + 1)));
     HIPCHK(c, hipMemcpy(c->d_goff, off, sizeof(int32_t) * ((size_t)G + 1), hipMemcpyHostToDevice));
     if (c->g.n > 0) {
-        launch_ctl_reset(c->stream, c->d_ctl, 8);
+        ctl_op(c, 8);
         launch_batch_check(launch_for(c), c->g, c->d_goff, G, c->d_ctl);
         CHK(read_ctl(c));
         if (c->h_ctl->count != 0) {
@@ -1138,6 +1174,7 @@ scc_status scc_mlp_train(scc_ctx *c, const float *X, const float *y, int64_t N, 
 
 scc_status scc_phase_recolour(scc_ctx *c) {
     if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     if (c->g.n == 0) return SCC_OK;
     Launch l = launch_for(c);
@@ -1187,16 +1224,17 @@ scc_status scc_profile_copy(scc_ctx *c, int64_t bytes) {
 scc_status scc_labels_device(scc_ctx *c, const int32_t **dptr) {
     if (!c || !dptr) return SCC_E_INPUT;
     if (!c->labels_valid) return fail(c, SCC_E_STATE, "no labels yet");
-    *dptr = c->d_label;
+    *dptr = c->label_in_root ? c->d_root : c->d_label;
     return SCC_OK;
 }
 
 // ---- phase-level entry points ---------------------------------------------
 scc_status scc_state_set(scc_ctx *c, const uint32_t *words, const int32_t *roots) {
     if (!c || !words) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     size_t vb = sizeof(uint32_t) * (size_t)c->g.n;
-    launch_ctl_reset(c->stream, c->d_ctl, 512);  // phase-level use starts from a clean control block
+    ctl_op(c, 512);  // phase-level use starts from a clean control block
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (vb == 0) return SCC_OK;
     HIPCHK(c, hipMemcpy(c->d_word, words, vb, hipMemcpyHostToDevice));
@@ -1223,6 +1261,7 @@ scc_status scc_state_get(scc_ctx *c, uint32_t *words, int32_t *roots) {
 
 scc_status scc_phase_trim(scc_ctx *c, int64_t *iterations) {
     if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     int64_t it = 0;
     if (c->g.n > 0) CHK(do_trim(c, &it));
@@ -1239,7 +1278,7 @@ scc_status scc_phase_reach(scc_ctx *c, int dir, int schedule, int64_t *depth) {
     uint32_t dirs = dir == 0 ? 1u : dir == 1 ? 2u : 3u;
     if (c->g.n > 0) {
         if (schedule >= SCC_SCHED_D) {  // seeds: live vertices holding the bit
-            launch_ctl_reset(c->stream, c->d_ctl, 128);
+            ctl_op(c, 128);
             if (dirs & 1u) launch_collect(launch_for(c), c->d_word, c->g.n, ST_F, c->d_piv, c->d_ctl);
             if (dirs & 2u) launch_collect(launch_for(c), c->d_word, c->g.n, ST_B, c->d_seedB, c->d_ctl);
         }
@@ -1251,6 +1290,7 @@ scc_status scc_phase_reach(scc_ctx *c, int dir, int schedule, int64_t *depth) {
 
 scc_status scc_phase_update(scc_ctx *c, int recolour, int64_t *pivots) {
     if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     int64_t np = 0;
     if (c->g.n > 0) CHK(do_elect(c, recolour ? 1 : 0, &np));
@@ -1260,6 +1300,7 @@ scc_status scc_phase_update(scc_ctx *c, int recolour, int64_t *pivots) {
 
 scc_status scc_phase_trim2(scc_ctx *c, int64_t *pairs) {
     if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     int64_t p = 0;
     if (c->g.n > 0) CHK(do_trim2(c, &p));
@@ -1269,6 +1310,7 @@ scc_status scc_phase_trim2(scc_ctx *c, int64_t *pairs) {
 
 scc_status scc_phase_wcc(scc_ctx *c, int64_t *iterations, int64_t *pivots) {
     if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
     if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
     int64_t it = 0, np = 0;
     if (c->g.n > 0) CHK(do_wcc(c, &it, &np));

@@ -68,7 +68,7 @@ struct Graph {  // device CSR + CSR^T
 // colour 1 everywhere, or graph id + 1 when off (G + 1 offsets) is given; with cls
 // (degree classes, out [0, n) and in [n, 2n)) the vertices whose out- or in-row
 // holds no vertex but themselves start trimmed (the first level of trim1).
-// ctl->live = live vertices, ctl->pcount = trimmed ones, ctl->abort = 0.
+// Adds the live and trimmed counts to ctl->live and ctl->pcount (reset bit 1024 zeroes them).
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, Ctl *ctl);
 // ctl->count += arcs that cross graphs of the batch
@@ -80,9 +80,9 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 // and head[v] = col[row[v]] (-1 for an empty row)
 void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
                          int32_t *head);
-void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what);  // what 512: clear ctl->abort
-// after a chain of last + 1 levels launched without a wait: depth and convergence check
-void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage);
+// control-block resets (bits: k_ctl_reset), after a chain guard when stage > 0 (a chain of
+// last + 1 levels launched without a wait: its depth, and abort = stage unless it converged)
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,

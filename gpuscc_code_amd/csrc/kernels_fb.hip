@@ -105,24 +105,34 @@ __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *_
     }
 }
 
-// End of a chain launched without a host wait: records its depth in
-// depth[stage - 1] and sets abort = stage when its last level still made
-// progress (the chain may need more levels).
-__global__ void k_chain_guard(Ctl *ctl, int last, int stage) {
-    if (threadIdx.x != 0 || ctl->abort) return;
-    uint32_t ran = ctl->ran;
-    ctl->depth[stage - 1] = ran;
-    if (ran == (uint32_t)(last + 1) && ctl->changed[last & 3] != 0) ctl->abort = (uint32_t)stage;
-}
-
-// chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in graphs)
-__global__ void k_ctl_reset(Ctl *ctl, int what) {
+// Chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in
+// graphs), preceded by a chain guard when guard_stage > 0: the chain that
+// ended with level guard_last records its depth in depth[guard_stage - 1] and
+// sets abort = guard_stage when that level still made progress (the chain may
+// need more levels).  With abort set the resets are skipped, unless bit 512
+// clears it.  Bit order of application: 1024, 64, 128, 1, 2, 4, 8, 16, 32
+// (seeds before the queue rings read them, npiv before it is cleared).
+__global__ void k_ctl_reset(Ctl *ctl, int what, int guard_last, int guard_stage) {
     if (threadIdx.x != 0) return;
+    if (guard_stage > 0 && !ctl->abort) {
+        uint32_t ran = ctl->ran;
+        ctl->depth[guard_stage - 1] = ran;
+        if (ran == (uint32_t)(guard_last + 1) && ctl->changed[guard_last & 3] != 0)
+            ctl->abort = (uint32_t)guard_stage;
+    }
     if (what & 512) ctl->abort = 0;  // the host has handled the stage that stopped
     else if (ctl->abort) return;
     if (what & 1024) {  // before k_init
         ctl->live = 0;
         ctl->pcount = 0;
+    }
+    if (what & 64) {  // the pivot list seeds both directions
+        ctl->seeds[0] = ctl->npiv;
+        ctl->seeds[1] = ctl->npiv;
+    }
+    if (what & 128) {
+        ctl->seeds[0] = 0;
+        ctl->seeds[1] = 0;
     }
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
@@ -140,14 +150,6 @@ __global__ void k_ctl_reset(Ctl *ctl, int what) {
             for (int i = 0; i < 4; i++) ctl->qarcs[d][i] = 0;
             for (int i = 0; i < 4; i++) ctl->creach[d][i] = 0;
         }
-    }
-    if (what & 64) {  // the pivot list seeds both directions
-        ctl->seeds[0] = ctl->npiv;
-        ctl->seeds[1] = ctl->npiv;
-    }
-    if (what & 128) {
-        ctl->seeds[0] = 0;
-        ctl->seeds[1] = 0;
     }
     if (what & 4) {
         ctl->npiv_prev = ctl->npiv;
@@ -178,11 +180,16 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
                          const uint16_t *__restrict__ score, Ctl *ctl) {
     __shared__ unsigned long long sh[16];
+    __shared__ uint32_t wslot[16];
+    __shared__ unsigned long long wmin[16];
     if (ctl && __atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
     my_range(n, beg, end);
     const int lane = threadIdx.x & (WAVE - 1);
     unsigned long long gone = 0;  // F and B: eliminated into the pivot's SCC
+    // wave-uniform pending minimum of one slot, carried across rounds
+    uint32_t pslot = 0xFFFFFFFFu;
+    unsigned long long pmin = ~0ull;
     for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * UPD_V) {  // block-uniform
         uint32_t w[UPD_V];
 #pragma unroll
@@ -218,9 +225,10 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                 key[k] = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
             }
         }
-        // One atomicMin per wave and round when every candidate of the wave
-        // targets one slot (the first election sends all live vertices there);
-        // otherwise one per candidate.
+        // When every candidate of the wave targets one slot (the first election
+        // sends all live vertices there), their minimum joins the wave's
+        // pending minimum (combined over the workgroup at the end: one
+        // atomicMin per workgroup); otherwise one atomicMin per candidate.
         uint32_t sl_lane = cmask ? sl[__ffs(cmask) - 1] : 0u;
         bool lane_same = true;
         unsigned long long lane_min = ~0ull;
@@ -240,13 +248,41 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                 unsigned long long x = __shfl_xor(m, o, WAVE);
                 m = x < m ? x : m;
             }
-            if (lane == first && m < cur[sl0]) atomicMin(&cur[sl0], m);
+            if (sl0 != pslot) {
+                if (pslot != 0xFFFFFFFFu && lane == 0 && pmin < cur[pslot]) atomicMin(&cur[pslot], pmin);
+                pslot = sl0;
+                pmin = m;
+            } else {
+                pmin = m < pmin ? m : pmin;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < UPD_V; k++)
                 if (((cmask >> k) & 1u) && key[k] < cur[sl[k]]) atomicMin(&cur[sl[k]], key[k]);
         }
     }
+    // the waves' pending minima: one atomicMin when they share their slot
+    const int wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    if (lane == 0) {
+        wslot[wid] = pslot;
+        wmin[wid] = pmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s0 = 0xFFFFFFFFu;
+        unsigned long long m0 = ~0ull;
+        bool one = true;
+        for (int i = 0; i < nw; i++) {
+            if (wslot[i] == 0xFFFFFFFFu) continue;
+            if (s0 == 0xFFFFFFFFu) s0 = wslot[i];
+            one = one && wslot[i] == s0;
+            m0 = wmin[i] < m0 ? wmin[i] : m0;
+        }
+        if (one && s0 != 0xFFFFFFFFu && m0 < cur[s0]) atomicMin(&cur[s0], m0);
+        wslot[0] = one ? 0xFFFFFFFFu : 0u;  // 0: the waves flush their own
+    }
+    __syncthreads();
+    if (wslot[0] == 0u && lane == 0 && pslot != 0xFFFFFFFFu && pmin < cur[pslot]) atomicMin(&cur[pslot], pmin);
     if (recolour && ctl) {
         gone = block_sum(gone, sh);
         if (threadIdx.x == 0 && gone) atomicSub(&ctl->live, (uint32_t)gone);
@@ -530,7 +566,6 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, Ctl *ctl) {
-    launch_ctl_reset(l.stream, ctl, 512 | 1024);
     hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr, ctl);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
@@ -549,11 +584,8 @@ void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col
                          int32_t *head) {
     hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out, head);
 }
-void launch_chain_guard(hipStream_t s, Ctl *ctl, int last, int stage) {
-    hipLaunchKernelGGL(k_chain_guard, dim3(1), dim3(64), 0, s, ctl, last, stage);
-}
-void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what);
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int guard_stage) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what, guard_last, guard_stage);
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,

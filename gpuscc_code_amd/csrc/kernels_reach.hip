@@ -764,13 +764,17 @@ __global__ void k_reach_hybrid(ReachArgs a) {
         my_range(a.n, beg, end);
         constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, direction d
         for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * PULL_V) {  // block-uniform
-            uint32_t cand = 0;
+            uint32_t cm[2] = {0u, 0u};  // candidates per pulling direction
 #pragma unroll
             for (int k = 0; k < PULL_V; k++) {
                 int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
                 uint32_t w = vv < end ? a.word[vv] : ST_E;
-                if (!(w & ST_E) && ((pull0 && !(w & ST_F)) || (pull1 && !(w & ST_B)))) cand |= 1u << k;
+                if (!(w & ST_E)) {
+                    if (pull0 && !(w & ST_F)) cm[0] |= 1u << k;
+                    if (pull1 && !(w & ST_B)) cm[1] |= 1u << k;
+                }
             }
+            uint32_t cand = cm[0] | cm[1];
             uint32_t claimed[2] = {0u, 0u};
             unsigned long long arcs[2] = {0ull, 0ull};
             while (cand) {  // per work-item
@@ -782,6 +786,8 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                     cand &= cand ? cand - 1 : 0u;
                     v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
                 }
+                // the words (colours) load together with the heads and row bounds:
+                // which rows to probe is known from the round's first load
                 uint32_t w[SCAN_ILP];
 #pragma unroll
                 for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
@@ -796,11 +802,10 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                     col[t] = dcol(a, 1 - d);
                     mask[t] = dbit(d) | ST_E;
                     val[t] = dbit(d);
-                    c[t] = colour_of(w[j]);
                     self[t] = -1;
                     u0[t] = -1;
                     e[t] = e1[t] = 0;
-                    if ((d == 0 ? pull0 : pull1) && !(w[j] & (ST_E | dbit(d)))) {
+                    if (kk[j] >= 0 && ((cm[d] >> kk[j]) & 1u)) {
                         u0[t] = a.head[1 - d][v[j]];
                         e[t] = row[v[j]] + 1;
                         e1[t] = row[v[j] + 1];
@@ -808,7 +813,14 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                         if (u0[t] >= 0) act |= 1u << t;
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
                 uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
+                // claims: the class bytes load with the atomics
+                uint32_t cls[T];
+#pragma unroll
+                for (int t = 0; t < T; t++)
+                    cls[t] = ((hit >> t) & 1u) ? a.dcls[t / SCAN_ILP][v[t % SCAN_ILP]] : 0u;
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     if (!((hit >> t) & 1u)) continue;
@@ -816,7 +828,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                     uint32_t old = atomicOr(&a.word[v[j]], dbit(d));
                     if (!(old & dbit(d))) {
                         claimed[d] |= 1u << kk[j];
-                        arcs[d] += class_arcs(a.dcls[d][v[j]]);
+                        arcs[d] += class_arcs(cls[t]);
                     }
                 }
             }
@@ -903,6 +915,8 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             cand &= cand ? cand - 1 : 0u;
             v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
         }
+        // the words (colours) load together with the heads and row bounds: only
+        // this work-item writes a candidate's word in a sweep level
         uint32_t w[SCAN_ILP];
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
@@ -916,22 +930,23 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             col[t] = d == 0 ? a.col : a.colT;
             mask[t] = ST_E;
             val[t] = 0;
-            c[t] = colour_of(w[j]);
             self[t] = v[j];
             u0[t] = -1;
             e[t] = e1[t] = 0;
-            if (!(w[j] & ST_E)) {
+            if (v[j] >= 0) {
                 u0[t] = a.head[d][v[j]];
                 e[t] = row[v[j]] + 1;
                 e1[t] = row[v[j] + 1];
                 if (u0[t] >= 0) act |= 1u << t;
             }
         }
+#pragma unroll
+        for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
         // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
         uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) {
-            if (!(w[j] & ST_E) && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
+            if (v[j] >= 0 && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
                 a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
                 a.root[v[j]] = v[j];
                 trimmed |= 1u << kk[j];

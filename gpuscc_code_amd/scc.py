@@ -212,6 +212,12 @@ class Context:
         self._chk(lib().scc_state_set(self._h, _p(w, _lib._u32p),
                                       None if r is None else _p(r, _lib._i32p)))
 
+    def labels_device(self):
+        """Device address (int) of the last run's labels (scc_labels_device)."""
+        ptr = _lib._i32p()
+        self._chk(lib().scc_labels_device(self._h, ctypes.byref(ptr)))
+        return ctypes.cast(ptr, ctypes.c_void_p).value or 0
+
     def get_state(self):
         w = np.empty(max(self.n, 1), np.uint32)
         r = np.empty(max(self.n, 1), np.int32)

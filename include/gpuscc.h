@@ -152,7 +152,10 @@ scc_status scc_run(scc_ctx *ctx, int schedule, int variant, int pivot_rule, int3
  * num_graphs = 0 clears the setting.  Labels stay global vertex ids. */
 scc_status scc_set_batch(scc_ctx *ctx, const int32_t *graph_offsets, int32_t num_graphs);
 
-/* Device address of the labels of the last run (valid until the next upload). */
+/* Device address of the labels of the last run.  Valid until the next run,
+ * phase-level call, scc_state_set or upload (after a plain-rule run the
+ * labels are the root array itself; a later call that writes roots first moves
+ * them to the label buffer, so ask again for the address after such a call). */
 scc_status scc_labels_device(scc_ctx *ctx, const int32_t **dptr);
 
 /* ---- phase-level entry points (tests, autotuner) -------------------------

@@ -468,3 +468,34 @@ def test_level_log(ctx):
     both = (pulled == 3) & (edges > 0)
     assert both.any() and (scan[both] >= 1).all()
     assert rows[0, f["kind"]] == 1  # trim1 first (its degree-test level runs inside the initialisation)
+
+
+def _device_i32(addr, n):
+    """Copy n int32 from a device address (hipMemcpy through the HIP runtime)."""
+    import ctypes
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    out = np.empty(n, np.int32)
+    rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(addr), ctypes.c_size_t(4 * n),
+                       ctypes.c_int(2))  # hipMemcpyDeviceToHost
+    assert rc == 0
+    return out
+
+
+def test_labels_device_across_phase_calls(ctx):
+    """After a plain-rule run the labels live in the root array; calls that
+    write roots move them to the label buffer first, so the labels at the
+    address asked for after such a call are still the run's."""
+    from gpuscc_code_amd import learned
+    g = scc.gen_rmat(12, 8, seed=11)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    ctx.mlp_set(learned.init_params(7))
+    for pivot in (scc.PIVOT_PLAIN, scc.PIVOT_LEARNED):
+        labels, _ = ctx.run("h", 0, pivot)
+        assert np.array_equal(labels, ref)
+        assert np.array_equal(_device_i32(ctx.labels_device(), g.n), ref)
+        w, r = ctx.get_state()
+        ctx.set_state(w, np.zeros_like(r))  # writes every root
+        assert np.array_equal(_device_i32(ctx.labels_device(), g.n), ref)
+        ctx.trim()
+        assert np.array_equal(_device_i32(ctx.labels_device(), g.n), ref)
