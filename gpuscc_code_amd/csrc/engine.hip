@@ -40,6 +40,7 @@ struct scc_ctx {
     uint32_t *d_stamp = nullptr;  // trim levels: last level that checked each vertex
     uint8_t *d_dcls = nullptr;    // degree classes: out-degrees [0, n), in-degrees [n, 2n)
     int32_t *d_head = nullptr;    // first entry of each row: out-rows [0, n), in-rows [n, 2n)
+    int4 *d_ell = nullptr;        // first four entries of each row, same halves
     uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
@@ -323,6 +324,8 @@ struct ReachChain {
         a.dcls[1] = c->d_dcls + c->g.n;
         a.head[0] = c->d_head;
         a.head[1] = c->d_head + c->g.n;
+        a.ell[0] = c->d_ell;
+        a.ell[1] = c->d_ell + c->g.n;
         a.pull_cost = c->opts.hybrid_policy == 2 ? 1 : 0;
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
@@ -516,6 +519,8 @@ void free_graph(scc_ctx *c) {
     }
     if (c->d_word) (void)hipFree(c->d_word);
     c->d_word = nullptr;
+    if (c->d_ell) (void)hipFree(c->d_ell);
+    c->d_ell = nullptr;
     for (auto &t : c->d_slots) {
         if (t) (void)hipFree(t);
         t = nullptr;
@@ -742,6 +747,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_stamp, vb));
     HIPCHK(c, hipMalloc(&c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMalloc(&c->d_head, 2 * vb));
+    HIPCHK(c, hipMalloc(&c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -767,8 +773,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     Launch l = launch_for(c);
     launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
-    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head);
-    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n);
+    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head, c->d_ell);
+    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n, c->d_ell + n);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());

@@ -92,8 +92,11 @@ __device__ inline void chain_progress(Ctl *ctl, int iter, bool found) {
     if (__any(found) && (threadIdx.x & (WAVE - 1)) == 0) ctl->changed[iter & 3] = 1;
 }
 
-// exclusive prefix sum over the workgroup (blockDim.x <= 1024, wave64)
-__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &total) {
+// exclusive prefix sum over the workgroup (nt = its size <= 1024, wave64).  Kernels
+// in loops pass nt from their arguments: blockDim.x is a vector load from the
+// dispatch packet, which the compiler may repeat in every iteration.
+__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &total,
+                                           uint32_t nt = blockDim.x) {
     int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     uint32_t inc = x;
     for (int o = 1; o < WAVE; o <<= 1) {
@@ -103,7 +106,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &
     __syncthreads();
     if (lane == WAVE - 1) shw[wid] = inc;
     __syncthreads();
-    int nw = (blockDim.x + WAVE - 1) / WAVE;
+    int nw = (int)((nt + WAVE - 1) / WAVE);
     uint32_t pre = 0, tot = 0;
     for (int i = 0; i < nw; i++) {
         uint32_t s = shw[i];
@@ -114,9 +117,9 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *shw, uint32_t &
     return pre + inc - x;
 }
 
-// Block-wide sum / max through LDS (blockDim.x <= 1024).
+// Block-wide sum / max through LDS (nt = workgroup size <= 1024).
 template <typename T>
-__device__ inline T block_sum(T v, T *sh) {
+__device__ inline T block_sum(T v, T *sh, uint32_t nt = blockDim.x) {
     for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
     int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     __syncthreads();
@@ -124,7 +127,7 @@ __device__ inline T block_sum(T v, T *sh) {
     __syncthreads();
     T r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x + WAVE - 1) / WAVE; i++) r += sh[i];
+        for (int i = 0; i < (int)(nt + WAVE - 1) / WAVE; i++) r += sh[i];
     return r;  // valid in thread 0
 }
 

@@ -44,9 +44,13 @@ struct ReachArgs {
     uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
     unsigned long long pull_arcs;  // hybrid: ... or one with at least this many (estimated) arcs
     int pull_cost;                 // hybrid: pull when frontier arcs >= 2 x (live - reached) instead
-    const uint8_t *dcls[2];  // per vertex and direction: bit length of the degree | 0x80 for a self-loop
+    const uint8_t *dcls[2];  // per vertex and direction: bit length of the degree | CLS_FITS | 0x80 (self-loop)
     const int32_t *head[2];  // per vertex and direction: first entry of the row (-1: empty row), read
                              // in place of col[row[v]] by the scans that stop at the first match
+    const int4 *ell[2];      // per vertex and direction: the row's first four entries (-1 past its
+                             // end); complete when the class byte has CLS_FITS (at most four)
+    uint32_t nt;        // work-items per workgroup (set by the launcher; blockDim.x without its
+                        // per-use load from the dispatch packet)
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int instr;          // count work (expanded vertices, scanned arcs, visits)
@@ -76,10 +80,12 @@ void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
 void launch_stream_copy(const Launch &l, const void *src, void *dst, int64_t bytes);
 void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
-// out[v] = bit length of row[v + 1] - row[v], | 0x80 when the row holds v (a self-loop)
-// and head[v] = col[row[v]] (-1 for an empty row)
+// out[v] = bit length of row[v + 1] - row[v], | 0x80 when the row holds v (a self-loop),
+// | CLS_FITS when it has at most four entries; head[v] = col[row[v]] (-1 for an empty
+// row); ell[v] = the row's first four entries, -1 past its end
+constexpr uint32_t CLS_FITS = 0x40;
 void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
-                         int32_t *head);
+                         int32_t *head, int4 *ell);
 // control-block resets (bits: k_ctl_reset), after a chain guard when stage > 0 (a chain of
 // last + 1 levels launched without a wait: its depth, and abort = stage unless it converged)
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0);

@@ -91,17 +91,24 @@ __global__ void k_fill_i32(int32_t *p, int64_t count, int32_t val) {
         p[i] = val;
 }
 
-// bits 0-5: bit length of the row's length; bit 7: the row holds v itself
+// bits 0-5: bit length of the row's length; bit 6 (CLS_FITS): at most four
+// entries, all held in ell[v]; bit 7: the row holds v itself
 __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *__restrict__ col, int32_t n,
-                               uint8_t *out, int32_t *head) {
+                               uint8_t *out, int32_t *head, int4 *ell) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (int64_t)gridDim.x * blockDim.x) {
         int32_t e0 = row[v], e1 = row[v + 1];
         uint32_t deg = (uint32_t)(e1 - e0);
         bool self = false;
         for (int32_t e = e0; e < e1 && !self; e++) self = col[e] == (int32_t)v;
-        out[v] = (uint8_t)((deg ? 32 - __clz(deg) : 0) | (self ? 0x80 : 0));
+        out[v] = (uint8_t)((deg ? 32 - __clz(deg) : 0) | (deg <= 4 ? CLS_FITS : 0) | (self ? 0x80 : 0));
         head[v] = e0 < e1 ? col[e0] : -1;
+        int4 q;
+        q.x = e0 < e1 ? col[e0] : -1;
+        q.y = e0 + 1 < e1 ? col[e0 + 1] : -1;
+        q.z = e0 + 2 < e1 ? col[e0 + 2] : -1;
+        q.w = e0 + 3 < e1 ? col[e0 + 3] : -1;
+        ell[v] = q;
     }
 }
 
@@ -581,8 +588,8 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
     hipLaunchKernelGGL(k_fill_i32, GRID(l), p, count, v);
 }
 void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
-                         int32_t *head) {
-    hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out, head);
+                         int32_t *head, int4 *ell) {
+    hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out, head, ell);
 }
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int guard_stage) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what, guard_last, guard_stage);

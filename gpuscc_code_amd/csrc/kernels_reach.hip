@@ -30,8 +30,22 @@ constexpr int EXP_ILP = 4;          // arcs per work-item per pass of a balanced
 
 __device__ inline uint32_t dbit(int d) { return d == 0 ? ST_F : ST_B; }
 __device__ inline uint32_t dxbit(int d) { return d == 0 ? ST_FX : ST_BX; }
-__device__ inline const int32_t *drow(const ReachArgs &a, int d) { return d == 0 ? a.row : a.rowT; }
-__device__ inline const int32_t *dcol(const ReachArgs &a, int d) { return d == 0 ? a.col : a.colT; }
+// Per-direction arrays of the arguments: both members are read first and the
+// value is selected.  Written as `d == 0 ? a.x[0] : a.x[1]` (or a.x[d]), the
+// compiler merges the two reads into one read at a per-lane address -- a
+// vector load of the pointer, one more dependent round trip before every use;
+// reading both keeps them scalar loads hoisted to the kernel's start.
+template <typename T>
+__device__ inline T dpick(int d, T x0, T x1) {
+    return d == 0 ? x0 : x1;
+}
+__device__ inline const int32_t *drow(const ReachArgs &a, int d) { return dpick(d, a.row, a.rowT); }
+__device__ inline const int32_t *dcol(const ReachArgs &a, int d) { return dpick(d, a.col, a.colT); }
+__device__ inline const uint8_t *ddcls(const ReachArgs &a, int d) { return dpick(d, a.dcls[0], a.dcls[1]); }
+__device__ inline const int4 *dell(const ReachArgs &a, int d) { return dpick(d, a.ell[0], a.ell[1]); }
+__device__ inline int32_t *doutq(const ReachArgs &a, int d) { return dpick(d, a.outq[0], a.outq[1]); }
+__device__ inline int32_t *dhout(const ReachArgs &a, int d) { return dpick(d, a.hout[0], a.hout[1]); }
+__device__ inline const int32_t *dhin(const ReachArgs &a, int d) { return dpick(d, a.hin[0], a.hin[1]); }
 
 __device__ inline bool try_claim(uint32_t *word, int32_t u, uint32_t c, uint32_t bit) {
     uint32_t wu = word[u];
@@ -64,8 +78,8 @@ __device__ inline unsigned long long class_arcs(uint32_t k) { return (3ull << (k
 // to the control block).
 __device__ inline void st_overflow(int d, int32_t v, const ReachArgs &a) {
     uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
-    a.outq[d][g] = v;
-    if (a.dcls[d]) atomicAdd(&a.ctl->qarcs[d][(a.iter + 1) & 3], class_arcs(a.dcls[d][v]));
+    doutq(a, d)[g] = v;
+    if (ddcls(a, d)) atomicAdd(&a.ctl->qarcs[d][(a.iter + 1) & 3], class_arcs(ddcls(a, d)[v]));
 }
 
 // One lane's push: one LDS atomic.  The arc estimate of the next frontier
@@ -108,12 +122,12 @@ __device__ inline void st_flush(Stage &s, const ReachArgs &a) {
     if (threadIdx.x == 1 && c1) s.base[1] = atomicAdd(&a.ctl->qsize[1][(a.iter + 1) & 3], c1);
     __syncthreads();
     unsigned long long ar0 = 0, ar1 = 0;
-    for (uint32_t i = threadIdx.x; i < c0; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < c0; i += a.nt) {
         int32_t v = s.buf[0][i];
         a.outq[0][s.base[0] + i] = v;
         if (a.dcls[0]) ar0 += class_arcs(a.dcls[0][v]);
     }
-    for (uint32_t i = threadIdx.x; i < c1; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < c1; i += a.nt) {
         int32_t v = s.buf[1][i];
         a.outq[1][s.base[1] + i] = v;
         if (a.dcls[1]) ar1 += class_arcs(a.dcls[1][v]);
@@ -133,7 +147,7 @@ __device__ inline void st_flush(Stage &s, const ReachArgs &a) {
 // pushes per work-item) could overflow a buffer
 __device__ inline void st_maybe_flush(Stage &s, const ReachArgs &a, uint32_t per_item = 1) {
     __syncthreads();
-    const uint32_t room = per_item * blockDim.x;
+    const uint32_t room = per_item * a.nt;
     bool full = s.cnt[0] > SQ_CAP - room || s.cnt[1] > SQ_CAP - room;
     __syncthreads();  // every work-item has read the counts before anyone pushes again
     if (full) st_flush(s, a);
@@ -180,12 +194,12 @@ __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nite
                                     Stage *st, bool &found, unsigned long long &cv) {
     // EXP_ILP arcs per work-item and pass: their column ids, then their words,
     // then the claims of the candidates, each group of loads in flight together
-    for (uint32_t base = 0; base < total; base += blockDim.x * EXP_ILP) {  // block-uniform trip count
+    for (uint32_t base = 0; base < total; base += a.nt * EXP_ILP) {  // block-uniform trip count
         int32_t u[EXP_ILP];
         uint32_t cd[EXP_ILP];
 #pragma unroll
         for (int j = 0; j < EXP_ILP; j++) {
-            uint32_t e = base + (uint32_t)j * blockDim.x + threadIdx.x;
+            uint32_t e = base + (uint32_t)j * a.nt + threadIdx.x;
             u[j] = -1;
             cd[j] = 0;
             if (e < total) {
@@ -244,7 +258,7 @@ __device__ inline void instr_maxscan(const ReachArgs &a, uint32_t steps) {
 // workgroup: it is deferred to the next level, where all workgroups split it
 __device__ inline void push_huge(const ReachArgs &a, int d, int32_t v) {
     uint32_t g = atomicAdd(&a.ctl->hsize[d][(a.iter + 1) & 3], 1u);
-    a.hout[d][g] = v;
+    dhout(a, d)[g] = v;
 }
 
 constexpr uint32_t HCHUNK = 1024;  // arcs per chunk of a huge row
@@ -264,13 +278,13 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
         const int32_t *row = drow(a, d), *col = dcol(a, d);
         uint32_t bit = dbit(d);
         uint32_t acc = 0;
-        const uint32_t step = blockDim.x < ITEM_CAP ? blockDim.x : ITEM_CAP;  // rows per pass
+        const uint32_t step = a.nt < ITEM_CAP ? a.nt : ITEM_CAP;  // rows per pass
         for (uint32_t base = 0; base < hs; base += step) {
             uint32_t i = base + threadIdx.x;
             uint32_t nch = 0, c = 0;
             int32_t e0 = 0, e1 = 0;
             if (threadIdx.x < step && i < hs) {
-                int32_t v = a.hin[d][i];
+                int32_t v = dhin(a, d)[i];
                 e0 = row[v];
                 e1 = row[v + 1];
                 nch = ((uint32_t)(e1 - e0) + HCHUNK - 1) / HCHUNK;
@@ -281,7 +295,7 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
                 }
             }
             uint32_t total;
-            uint32_t off = block_excl_scan(nch, it.shw, total);
+            uint32_t off = block_excl_scan(nch, it.shw, total, a.nt);
             if (threadIdx.x < step) {
                 it.off[threadIdx.x] = off;
                 it.start[threadIdx.x] = e0;
@@ -295,7 +309,7 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
                 int32_t cs = it.start[j] + (int32_t)((local - it.off[j]) * HCHUNK);
                 int32_t cend = cs + (int32_t)HCHUNK < it.lim[j] ? cs + (int32_t)HCHUNK : it.lim[j];
                 uint32_t cc = it.cd[j];
-                for (int32_t pb = cs; pb < cend; pb += (int32_t)blockDim.x) {  // block-uniform
+                for (int32_t pb = cs; pb < cend; pb += (int32_t)a.nt) {  // block-uniform
                     int32_t e = pb + (int32_t)threadIdx.x;
                     if (e < cend && try_claim(a.word, col[e], cc, bit)) {
                         found = true;
@@ -324,7 +338,7 @@ __global__ void k_reach_sweep(ReachArgs a) {
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
     huge_phase<false>(a, it, nullptr, found, cx, ce, cv);
-    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
+    for (int64_t vv = beg + threadIdx.x; vv < end; vv += a.nt) {
         int32_t v = (int32_t)vv;
         uint32_t w = a.word[v];
         if (w & ST_E) continue;
@@ -367,7 +381,7 @@ __global__ void k_reach_topo(ReachArgs a) {
     bool found = false;
     unsigned long long cx = 0, ce = 0, cv = 0;
     huge_phase<false>(a, it, nullptr, found, cx, ce, cv);
-    for (int64_t base = beg; base < end; base += blockDim.x) {
+    for (int64_t base = beg; base < end; base += a.nt) {
         int64_t vv = base + threadIdx.x;
         uint32_t deg[2] = {0, 0}, c = 0;
         int32_t start[2] = {0, 0};
@@ -396,7 +410,7 @@ __global__ void k_reach_topo(ReachArgs a) {
             }
         }
         uint32_t total;
-        uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total);
+        uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total, a.nt);
         it.off[2 * threadIdx.x] = off;
         it.off[2 * threadIdx.x + 1] = off + deg[0];
         it.start[2 * threadIdx.x] = start[0];
@@ -404,7 +418,7 @@ __global__ void k_reach_topo(ReachArgs a) {
         it.cd[2 * threadIdx.x] = c;
         it.cd[2 * threadIdx.x + 1] = c | 0x80000000u;
         __syncthreads();
-        if (total > 0) expand_items<false>(a, it, 2 * blockDim.x, total, nullptr, found, cv);
+        if (total > 0) expand_items<false>(a, it, 2 * a.nt, total, nullptr, found, cv);
         __syncthreads();
     }
     chain_progress(a.ctl, a.iter, found);
@@ -431,12 +445,13 @@ union ItemsLocal {
     LocalQ lq;
 };
 
-// expands item v of direction d serially; new visits are staged
+// expands item v (colour c) of direction d serially over its CSR row; new
+// visits are staged
 template <bool WAVE_PUSH>
-__device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, bool &found,
-                                     unsigned long long &cx, unsigned long long &ce,
+__device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, uint32_t c,
+                                     bool &found, unsigned long long &cx, unsigned long long &ce,
                                      unsigned long long &cv) {
-    uint32_t c = colour_of(a.word[v]), bit = dbit(d);
+    const uint32_t bit = dbit(d);
     const int32_t *row = drow(a, d), *col = dcol(a, d);
     int32_t e0 = row[v], e1 = row[v + 1];
     if ((uint32_t)(e1 - e0) >= a.huge_thr) {
@@ -473,6 +488,43 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32
     }
 }
 
+// Expands item v of direction d whose row fits its four-entry head (class byte
+// k has CLS_FITS): the column ids come with v's own word in the first round
+// trip, so a short row costs three dependent accesses (head, target words,
+// claims) instead of five (row bounds, own word, column ids, words, claims).
+template <bool WAVE_PUSH>
+__device__ inline void expand_fit(const ReachArgs &a, Stage &st, int d, int32_t v, uint32_t c, int4 nb,
+                                  bool &found, unsigned long long &cx, unsigned long long &ce,
+                                  unsigned long long &cv) {
+    const uint32_t bit = dbit(d);
+    const int32_t u[4] = {nb.x, nb.y, nb.z, nb.w};
+    const uint32_t deg = (nb.x >= 0 ? 1u : 0u) + (nb.y >= 0 ? 1u : 0u) + (nb.z >= 0 ? 1u : 0u) +
+                         (nb.w >= 0 ? 1u : 0u);
+    if (deg >= a.huge_thr) {
+        push_huge(a, d, v);
+        found = true;
+        return;
+    }
+    cx++;
+    ce += deg;
+    uint32_t wu[4], old[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        old[j] = bit;
+        if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (old[j] & bit) continue;
+        found = true;
+        cv++;
+        if (WAVE_PUSH) st_push_wave(st, d, u[j], a);
+        else st_push(st, d, u[j], a);
+    }
+}
+
 constexpr int32_t LOCAL_MAX_DEG = 64;       // longer rows leave the local rounds for the next level
 constexpr uint32_t LOCAL_ITEMS_PER_WG = 8;  // local rounds only when a level's frontier is this thin
 constexpr int LOCAL_FIRST_LEVEL = 4;        // ... and the reach is this many levels deep
@@ -492,23 +544,34 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
         uint32_t c0 = st.cnt[0] < SQ_CAP ? st.cnt[0] : SQ_CAP;
         uint32_t c1 = st.cnt[1] < SQ_CAP ? st.cnt[1] : SQ_CAP;
         if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
-        for (uint32_t k = threadIdx.x; k < c0; k += blockDim.x) lq.v[k] = st.buf[0][k];
-        for (uint32_t k = threadIdx.x; k < c1; k += blockDim.x) lq.v[SQ_CAP + k] = st.buf[1][k];
+        for (uint32_t k = threadIdx.x; k < c0; k += a.nt) lq.v[k] = st.buf[0][k];
+        for (uint32_t k = threadIdx.x; k < c1; k += a.nt) lq.v[SQ_CAP + k] = st.buf[1][k];
         __syncthreads();
         if (threadIdx.x < 2) st.cnt[threadIdx.x] = 0;
         __syncthreads();
-        for (uint32_t k0 = 0; k0 < c0 + c1; k0 += blockDim.x) {
+        for (uint32_t k0 = 0; k0 < c0 + c1; k0 += a.nt) {
             uint32_t k = k0 + threadIdx.x;
             if (k < c0 + c1) {
                 int d = k < c0 ? 0 : 1;
                 int32_t v = d == 0 ? lq.v[k] : lq.v[SQ_CAP + (k - c0)];
-                const int32_t *row = drow(a, d);
-                if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
-                    if (a.dcls[d]) atomicAdd(&st.arcs[d], class_arcs(a.dcls[d][v]));
-                    uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
-                    a.outq[d][g] = v;
+                // class byte, own word and row head in one round trip (the empty
+                // asm needs the head before the branch, so its load is not sunk
+                // into the branch behind the other two)
+                const uint32_t cls = ddcls(a, d)[v];
+                const uint32_t c = colour_of(a.word[v]);
+                const int4 nb = dell(a, d)[v];
+                asm volatile("" ::"v"(nb.x));
+                if (cls & CLS_FITS) {
+                    expand_fit<false>(a, st, d, v, c, nb, found, cx, ce, cv);
                 } else {
-                    expand_serial<false>(a, st, d, v, found, cx, ce, cv);
+                    const int32_t *row = drow(a, d);
+                    if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
+                        atomicAdd(&st.arcs[d], class_arcs(cls));
+                        uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
+                        doutq(a, d)[g] = v;
+                    } else {
+                        expand_serial<false>(a, st, d, v, c, found, cx, ce, cv);
+                    }
                 }
             }
             st_maybe_flush(st, a);
@@ -534,12 +597,17 @@ __global__ void k_reach_queue(ReachArgs a) {
     uint32_t tot = qs0 + qs1;
     uint32_t chunk = (tot + gridDim.x - 1) / gridDim.x;
     uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < tot ? b0 + chunk : tot;
-    for (uint32_t base = b0; base < b1; base += blockDim.x) {
+    for (uint32_t base = b0; base < b1; base += a.nt) {
         uint32_t i = base + threadIdx.x;
         if (i < b1) {
             int d = i < qs0 ? 0 : 1;
             int32_t v = d == 0 ? a.inq[0][i] : a.inq[1][i - qs0];
-            expand_serial<true>(a, st, d, v, found, cx, ce, cv);
+            const uint32_t cls = ddcls(a, d)[v];
+            const uint32_t c = colour_of(a.word[v]);
+            const int4 nb = dell(a, d)[v];
+            asm volatile("" ::"v"(nb.x));
+            if (cls & CLS_FITS) expand_fit<true>(a, st, d, v, c, nb, found, cx, ce, cv);
+            else expand_serial<true>(a, st, d, v, c, found, cx, ce, cv);
         }
         st_maybe_flush(st, a);
     }
@@ -555,7 +623,7 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
                                       unsigned long long &ce, unsigned long long &cv) {
     uint32_t chunk = (tot + gridDim.x - 1) / gridDim.x;
     uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < tot ? b0 + chunk : tot;
-    for (uint32_t base = b0; base < b1; base += blockDim.x) {
+    for (uint32_t base = b0; base < b1; base += a.nt) {
         uint32_t i = base + threadIdx.x;
         uint32_t deg = 0, cd = 0;
         int32_t start = 0;
@@ -576,12 +644,12 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
             }
         }
         uint32_t total;
-        uint32_t off = block_excl_scan(deg, it.shw, total);
+        uint32_t off = block_excl_scan(deg, it.shw, total, a.nt);
         it.off[threadIdx.x] = off;
         it.start[threadIdx.x] = start;
         it.cd[threadIdx.x] = cd;
         __syncthreads();
-        if (total > 0) expand_items<true>(a, it, blockDim.x, total, &st, found, cv);
+        if (total > 0) expand_items<true>(a, it, a.nt, total, &st, found, cv);
         st_maybe_flush(st, a);
     }
 }
@@ -702,10 +770,10 @@ __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint
                                    unsigned long long *sh, bool trims = false, uint32_t newp = 0) {
     const uint32_t c0 = __popc(claimed[0]), c1 = __popc(claimed[1]);
     uint32_t total;  // packed counts: a workgroup claims at most 16 * 1024 < 2^16 per direction
-    const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total);
-    unsigned long long ar0 = block_sum(arcs[0], sh);
-    unsigned long long ar1 = block_sum(arcs[1], sh);
-    unsigned long long np = trims ? block_sum((unsigned long long)newp, sh) : 0ull;
+    const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total, a.nt);
+    unsigned long long ar0 = block_sum(arcs[0], sh, a.nt);
+    unsigned long long ar1 = block_sum(arcs[1], sh, a.nt);
+    unsigned long long np = trims ? block_sum((unsigned long long)newp, sh, a.nt) : 0ull;
     if (threadIdx.x == 0) {
         const int slot = (a.iter + 1) & 3;
         sbase[0] = (total & 0xFFFFu) ? atomicAdd(&a.ctl->qsize[0][slot], total & 0xFFFFu) : 0u;
@@ -721,7 +789,7 @@ __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint
     for (int d = 0; d < 2; d++) {
         for (uint32_t m = claimed[d]; m; m &= m - 1) {
             int k = __ffs(m) - 1;
-            a.outq[d][o[d]++] = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+            a.outq[d][o[d]++] = (int32_t)(base + (int64_t)k * a.nt + threadIdx.x);
         }
     }
     __syncthreads();  // sbase is rewritten by the next round
@@ -763,11 +831,11 @@ __global__ void k_reach_hybrid(ReachArgs a) {
         int64_t beg, end;
         my_range(a.n, beg, end);
         constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, direction d
-        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * PULL_V) {  // block-uniform
+        for (int64_t base = beg; base < end; base += (int64_t)a.nt * PULL_V) {  // block-uniform
             uint32_t cm[2] = {0u, 0u};  // candidates per pulling direction
 #pragma unroll
             for (int k = 0; k < PULL_V; k++) {
-                int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+                int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
                 uint32_t w = vv < end ? a.word[vv] : ST_E;
                 if (!(w & ST_E)) {
                     if (pull0 && !(w & ST_F)) cm[0] |= 1u << k;
@@ -784,7 +852,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                 for (int j = 0; j < SCAN_ILP; j++) {
                     kk[j] = cand ? __ffs(cand) - 1 : -1;
                     cand &= cand ? cand - 1 : 0u;
-                    v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
+                    v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * a.nt + threadIdx.x) : -1;
                 }
                 // the words (colours) load together with the heads and row bounds:
                 // which rows to probe is known from the round's first load
@@ -900,7 +968,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
     uint32_t cand = 0;
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++) {
-        int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
+        int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
         uint32_t w = vv < end ? a.word[vv] : ST_E;
         if (!(w & ST_E) && !(a.skip_fb && (w & (ST_P | ST_F | ST_B)) == (ST_F | ST_B))) cand |= 1u << k;
     }
@@ -913,7 +981,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
         for (int j = 0; j < SCAN_ILP; j++) {
             kk[j] = cand ? __ffs(cand) - 1 : -1;
             cand &= cand ? cand - 1 : 0u;
-            v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * blockDim.x + threadIdx.x) : -1;
+            v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * a.nt + threadIdx.x) : -1;
         }
         // the words (colours) load together with the heads and row bounds: only
         // this work-item writes a candidate's word in a sweep level
@@ -974,7 +1042,7 @@ __global__ void k_trim_level(ReachArgs a) {
         my_range(a.n, beg, end);
         unsigned long long ce = 0;
         const unsigned long long none[2] = {0ull, 0ull};
-        for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * TRIM_V) {  // block-uniform
+        for (int64_t base = beg; base < end; base += (int64_t)a.nt * TRIM_V) {  // block-uniform
             uint32_t newp = 0;
             const uint32_t m = trim_round_scan(a, base, end, ce, newp, smax);
             if (m) found = true;
@@ -987,7 +1055,7 @@ __global__ void k_trim_level(ReachArgs a) {
         unsigned long long ntrim = 0, newp = 0;
         uint32_t chunk = (qs + gridDim.x - 1) / gridDim.x;
         uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < qs ? b0 + chunk : qs;
-        for (uint32_t base = b0; base < b1; base += blockDim.x) {
+        for (uint32_t base = b0; base < b1; base += a.nt) {
             // each trimmed vertex contributes its out-row and its in-row
             uint32_t i = base + threadIdx.x;
             uint32_t deg[2] = {0, 0}, c = 0;
@@ -1001,7 +1069,7 @@ __global__ void k_trim_level(ReachArgs a) {
                 deg[1] = (uint32_t)(a.rowT[x + 1] - start[1]);
             }
             uint32_t total;
-            uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total);
+            uint32_t off = block_excl_scan(deg[0] + deg[1], it.shw, total, a.nt);
             it.off[2 * threadIdx.x] = off;
             it.off[2 * threadIdx.x + 1] = off + deg[0];
             it.start[2 * threadIdx.x] = start[0];
@@ -1009,10 +1077,10 @@ __global__ void k_trim_level(ReachArgs a) {
             it.cd[2 * threadIdx.x] = c;
             it.cd[2 * threadIdx.x + 1] = c | 0x80000000u;
             __syncthreads();
-            for (uint32_t pb = 0; pb < total; pb += blockDim.x) {  // block-uniform
+            for (uint32_t pb = 0; pb < total; pb += a.nt) {  // block-uniform
                 uint32_t e = pb + threadIdx.x;
                 if (e < total) {
-                    uint32_t k = find_item(it.off, 2 * blockDim.x, e);
+                    uint32_t k = find_item(it.off, 2 * a.nt, e);
                     uint32_t cd = it.cd[k];
                     const int32_t *col = (cd >> 31) ? a.colT : a.col;
                     int32_t u = col[it.start[k] + (int32_t)(e - it.off[k])];
@@ -1031,8 +1099,8 @@ __global__ void k_trim_level(ReachArgs a) {
                 st_maybe_flush(st, a);
             }
         }
-        ntrim = block_sum(ntrim, sh);
-        newp = block_sum(newp, sh);
+        ntrim = block_sum(ntrim, sh, a.nt);
+        newp = block_sum(newp, sh, a.nt);
         if (threadIdx.x == 0 && ntrim) atomicSub(&a.ctl->live, (uint32_t)ntrim);
         if (threadIdx.x == 0 && newp) atomicAdd(&a.ctl->pcount, (uint32_t)newp);
     }
@@ -1055,7 +1123,9 @@ __global__ void k_clear_bits(uint32_t *word, int32_t n, uint32_t bits) {
 // itself (hipExtLaunchKernelGGL), so their interval is the kernel's duration
 // ---------------------------------------------------------------------------
 template <typename K>
-static void launch_reach_kernel(K kernel, const Launch &l, const ReachArgs &a) {
+static void launch_reach_kernel(K kernel, const Launch &l, const ReachArgs &args) {
+    ReachArgs a = args;
+    a.nt = l.block;
     if (l.ev_start)
         hipExtLaunchKernelGGL(kernel, dim3(l.grid), dim3(l.block), 0, l.stream, l.ev_start, l.ev_stop,
                               0, a);
