@@ -92,6 +92,13 @@ __device__ inline void st_push(Stage &s, int d, int32_t v, const ReachArgs &a) {
     else st_overflow(d, v, a);
 }
 
+// the same with the two counts at cnt (the local rounds rotate their counts)
+__device__ inline void st_push_at(Stage &s, uint32_t *cnt, int d, int32_t v, const ReachArgs &a) {
+    uint32_t i = atomicAdd(&cnt[d], 1u);
+    if (i < SQ_CAP) s.buf[d][i] = v;
+    else st_overflow(d, v, a);
+}
+
 // Wave-aggregated push, for passes where many lanes push together: the active
 // lanes pushing direction d ballot, the lowest of them reserves their slots
 // with one LDS atomic, and each lane writes at that base plus the number of
@@ -436,6 +443,7 @@ __global__ void k_reach_topo(ReachArgs a) {
 // ---------------------------------------------------------------------------
 struct LocalQ {
     int32_t v[2 * SQ_CAP];
+    uint32_t rc[3][2];  // staging counts of the local rounds, rotating (see local_rounds)
 };
 
 // The local rounds start after the last use of the items of a level, so the
@@ -446,11 +454,11 @@ union ItemsLocal {
 };
 
 // expands item v (colour c) of direction d serially over its CSR row; new
-// visits are staged
+// visits are staged (wave-aggregated, or one by one with the counts at cnt)
 template <bool WAVE_PUSH>
-__device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32_t v, uint32_t c,
-                                     bool &found, unsigned long long &cx, unsigned long long &ce,
-                                     unsigned long long &cv) {
+__device__ inline void expand_serial(const ReachArgs &a, Stage &st, uint32_t *cnt, int d, int32_t v,
+                                     uint32_t c, bool &found, unsigned long long &cx,
+                                     unsigned long long &ce, unsigned long long &cv) {
     const uint32_t bit = dbit(d);
     const int32_t *row = drow(a, d), *col = dcol(a, d);
     int32_t e0 = row[v], e1 = row[v + 1];
@@ -483,7 +491,7 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32
             found = true;
             cv++;
             if (WAVE_PUSH) st_push_wave(st, d, u[j], a);
-            else st_push(st, d, u[j], a);
+            else st_push_at(st, cnt, d, u[j], a);
         }
     }
 }
@@ -493,9 +501,9 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, int d, int32
 // trip, so a short row costs three dependent accesses (head, target words,
 // claims) instead of five (row bounds, own word, column ids, words, claims).
 template <bool WAVE_PUSH>
-__device__ inline void expand_fit(const ReachArgs &a, Stage &st, int d, int32_t v, uint32_t c, int4 nb,
-                                  bool &found, unsigned long long &cx, unsigned long long &ce,
-                                  unsigned long long &cv) {
+__device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, int d, int32_t v,
+                                  uint32_t c, int4 nb, bool &found, unsigned long long &cx,
+                                  unsigned long long &ce, unsigned long long &cv) {
     const uint32_t bit = dbit(d);
     const int32_t u[4] = {nb.x, nb.y, nb.z, nb.w};
     const uint32_t deg = (nb.x >= 0 ? 1u : 0u) + (nb.y >= 0 ? 1u : 0u) + (nb.z >= 0 ? 1u : 0u) +
@@ -521,7 +529,7 @@ __device__ inline void expand_fit(const ReachArgs &a, Stage &st, int d, int32_t 
         found = true;
         cv++;
         if (WAVE_PUSH) st_push_wave(st, d, u[j], a);
-        else st_push(st, d, u[j], a);
+        else st_push_at(st, cnt, d, u[j], a);
     }
 }
 
@@ -531,7 +539,12 @@ constexpr int LOCAL_FIRST_LEVEL = 4;        // ... and the reach is this many le
 
 // Workgroup-local rounds on the visits staged by this workgroup: each round
 // moves the staged vertices into lq and expands them one per lane; rows longer
-// than LOCAL_MAX_DEG go to the next level's queue instead.  All work-items call this.
+// than LOCAL_MAX_DEG go to the next level's queue instead.  Two barriers per
+// round: the staging counts rotate over three slots -- round r reads slot r,
+// pushes into slot r + 1 and clears slot r + 2, which round r - 1 read before
+// the round's first barrier -- so no barrier is needed to clear a count before
+// the round's pushes.  A push past the buffers goes to the queue (st_push_at),
+// so the rounds need no flushes either.  All work-items call this.
 __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, Stage &st, LocalQ &lq,
                                     bool &found, unsigned long long &cx, unsigned long long &ce,
                                     unsigned long long &cv) {
@@ -539,16 +552,24 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
     // (high-diameter graphs) would leave most workgroups idle per level.  The
     // first levels of a reach are thin on every graph, so they never qualify.
     if (a.iter < LOCAL_FIRST_LEVEL || level_items > LOCAL_ITEMS_PER_WG * gridDim.x) return;
+    __syncthreads();  // the level's pushes are staged; the items sharing lq's memory are done with
+    if (threadIdx.x < 2) {
+        lq.rc[0][threadIdx.x] = st.cnt[threadIdx.x];
+        lq.rc[1][threadIdx.x] = 0;
+        lq.rc[2][threadIdx.x] = 0;
+    }
+    int ci = 0;  // slot of this round's input counts
     for (int r = 0; r < a.local_rounds; r++) {
-        __syncthreads();
-        uint32_t c0 = st.cnt[0] < SQ_CAP ? st.cnt[0] : SQ_CAP;
-        uint32_t c1 = st.cnt[1] < SQ_CAP ? st.cnt[1] : SQ_CAP;
+        __syncthreads();  // the pushes of the round before are complete
+        uint32_t c0 = lq.rc[ci][0] < SQ_CAP ? lq.rc[ci][0] : SQ_CAP;
+        uint32_t c1 = lq.rc[ci][1] < SQ_CAP ? lq.rc[ci][1] : SQ_CAP;
         if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
+        const int co = ci == 2 ? 0 : ci + 1, cz = co == 2 ? 0 : co + 1;
         for (uint32_t k = threadIdx.x; k < c0; k += a.nt) lq.v[k] = st.buf[0][k];
         for (uint32_t k = threadIdx.x; k < c1; k += a.nt) lq.v[SQ_CAP + k] = st.buf[1][k];
-        __syncthreads();
-        if (threadIdx.x < 2) st.cnt[threadIdx.x] = 0;
-        __syncthreads();
+        if (threadIdx.x < 2) lq.rc[cz][threadIdx.x] = 0;
+        __syncthreads();  // the buffers are copied: this round's pushes may overwrite them
+        uint32_t *out = lq.rc[co];
         for (uint32_t k0 = 0; k0 < c0 + c1; k0 += a.nt) {
             uint32_t k = k0 + threadIdx.x;
             if (k < c0 + c1) {
@@ -562,7 +583,7 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                 const int4 nb = dell(a, d)[v];
                 asm volatile("" ::"v"(nb.x));
                 if (cls & CLS_FITS) {
-                    expand_fit<false>(a, st, d, v, c, nb, found, cx, ce, cv);
+                    expand_fit<false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
                 } else {
                     const int32_t *row = drow(a, d);
                     if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
@@ -570,13 +591,16 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                         uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
                         doutq(a, d)[g] = v;
                     } else {
-                        expand_serial<false>(a, st, d, v, c, found, cx, ce, cv);
+                        expand_serial<false>(a, st, out, d, v, c, found, cx, ce, cv);
                     }
                 }
             }
-            st_maybe_flush(st, a);
         }
+        ci = co;
     }
+    // what the rounds leave staged is counted in slot ci; st_flush takes it from st.cnt
+    __syncthreads();
+    if (threadIdx.x < 2) st.cnt[threadIdx.x] = lq.rc[ci][threadIdx.x];
 }
 
 __global__ void k_reach_queue(ReachArgs a) {
@@ -606,8 +630,8 @@ __global__ void k_reach_queue(ReachArgs a) {
             const uint32_t c = colour_of(a.word[v]);
             const int4 nb = dell(a, d)[v];
             asm volatile("" ::"v"(nb.x));
-            if (cls & CLS_FITS) expand_fit<true>(a, st, d, v, c, nb, found, cx, ce, cv);
-            else expand_serial<true>(a, st, d, v, c, found, cx, ce, cv);
+            if (cls & CLS_FITS) expand_fit<true>(a, st, st.cnt, d, v, c, nb, found, cx, ce, cv);
+            else expand_serial<true>(a, st, st.cnt, d, v, c, found, cx, ce, cv);
         }
         st_maybe_flush(st, a);
     }
