@@ -12,6 +12,8 @@ roofline: algorithmic bytes of the reachability level kernels (from device
 work counters in an untimed counting run) over their summed duration, timed
 with HIP events on the library's stream during the timed region.
 cpu_baseline: the oracle's single-threaded Tarjan on the same graph.
+learned_pivot: the same decompositions with the trained MLP pivot scorer
+(time, FB steps, plain time / learned time, labels bit-exact).
 
 Prints one JSON line.
 """
@@ -126,6 +128,9 @@ def main():
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
     ap.add_argument("--no-tuned", action="store_true",
                     help="library defaults instead of the autotuned options (gpuscc_code_amd/tuned.json)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-pivot leg (same workload and options, pivots elected by the "
+                         "trained MLP scorer, gpuscc_code_amd/pivot_mlp.npy)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -201,6 +206,24 @@ def main():
     ms_per_step = t_total * 1e3 / args.steps
     mteps = g.m / (ms_per_step * 1e3)
 
+    # learned-pivot leg: the same decomposition with pivots elected by the trained
+    # scorer (its static features are computed by the warm-up run, once per graph)
+    learned = None
+    if not args.no_learned:
+        from gpuscc_code_amd import learned as learned_mod
+        ctx.mlp_set(learned_mod.load_params())
+        for _ in range(max(args.warmup, 1)):
+            _, st_l = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED, labels=False)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st_l = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED, labels=False)
+        ms_l = (time.perf_counter() - t0) * 1e3 / args.steps
+        lab_l, _ = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED)
+        learned = {"ms_per_step": round(ms_l, 3), "value": round(g.m / (ms_l * 1e3), 2), "unit": "MTEPS",
+                   "ms_per_graph": round(ms_l / num_graphs, 6), "fb_steps": st_l["steps"],
+                   "plain_fb_steps": stats[-1]["steps"],
+                   "plain_over_learned": round(ms_per_step / ms_l, 3), "labels": lab_l}
+
     if args.levels:  # one more decomposition, levels launched one at a time
         ctx.profile(4)
         ctx.run(args.schedule, args.variant, labels=False)
@@ -212,6 +235,8 @@ def main():
     import oracle
     ref, tst = oracle.tarjan(g.row, g.col)
     bit_exact = bool(np.array_equal(labels, ref))
+    if learned is not None:
+        learned["bit_exact"] = bool(np.array_equal(learned.pop("labels"), ref))
 
     per_run_bytes = algorithmic_bytes(args.schedule, g.n, work)
     launches = prof["reach_launches"] / args.steps
@@ -246,7 +271,7 @@ def main():
         "trim_iters": last["trim_iters"], "host_syncs": last["host_syncs"],
         "phase_ms": {k: round(statistics.mean(s[k] for s in stats), 3)
                      for k in ("ms_fwd", "ms_bwd", "ms_trim", "ms_update")},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "learned_pivot": learned, "roofline": roofline, "cpu_baseline": cpu,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

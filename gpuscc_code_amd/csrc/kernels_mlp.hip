@@ -24,6 +24,11 @@ static_assert(OFF_B3 + 1 == MLP_PARAMS, "parameter layout");
 
 constexpr float LOG_SCALE = 1.0f / 16.0f;  // log1p of degrees up to ~2^23 maps below ~1
 constexpr int32_t DEG_SAMPLE = 256;        // rows longer than this are sampled for within-colour degrees
+// The per-vertex network keeps its feature vector and hidden layer in
+// registers: these kernels run 256-wide workgroups (one wave per SIMD), which
+// lets the compiler use up to 512 VGPRs per wave instead of spilling them.
+constexpr int MLP_BLOCK = 256;
+constexpr int FEAT_ILP = 8;                // arcs per step of a feature row scan: their loads in flight together
 
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -49,16 +54,29 @@ __global__ void k_static_features(const int32_t *__restrict__ row, const int32_t
         int32_t v = (int32_t)vv;
         int32_t o0 = row[v], o1 = row[v + 1], i0 = rowT[v], i1 = rowT[v + 1];
         float h_out = 0.f, h_in = 0.f;
+        // neighbour degrees FEAT_ILP at a time, summed in row order
         int32_t oe = o1 - o0 > DEG_SAMPLE ? o0 + DEG_SAMPLE : o1;
-        for (int32_t e = o0; e < oe; e++) {
-            int32_t u = col[e];
-            h_out += (float)(row[u + 1] - row[u]);
+        for (int32_t e = o0; e < oe; e += FEAT_ILP) {
+            int32_t u[FEAT_ILP], d[FEAT_ILP];
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++) u[j] = e + j < oe ? col[e + j] : -1;
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++) d[j] = u[j] >= 0 ? row[u[j] + 1] - row[u[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++)
+                if (u[j] >= 0) h_out += (float)d[j];
         }
         if (o1 - o0 > DEG_SAMPLE) h_out *= (float)(o1 - o0) / (float)DEG_SAMPLE;
         int32_t ie = i1 - i0 > DEG_SAMPLE ? i0 + DEG_SAMPLE : i1;
-        for (int32_t e = i0; e < ie; e++) {
-            int32_t u = colT[e];
-            h_in += (float)(rowT[u + 1] - rowT[u]);
+        for (int32_t e = i0; e < ie; e += FEAT_ILP) {
+            int32_t u[FEAT_ILP], d[FEAT_ILP];
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++) u[j] = e + j < ie ? colT[e + j] : -1;
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++) d[j] = u[j] >= 0 ? rowT[u[j] + 1] - rowT[u[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < FEAT_ILP; j++)
+                if (u[j] >= 0) h_in += (float)d[j];
         }
         if (i1 - i0 > DEG_SAMPLE) h_in *= (float)(i1 - i0) / (float)DEG_SAMPLE;
         out[v] = make_float4(log1pf((float)(o1 - o0)), log1pf((float)(i1 - i0)), log1pf(h_out),
@@ -100,11 +118,15 @@ __device__ inline uint32_t live_same_colour(const int32_t *__restrict__ row,
     deg = (uint32_t)(e1 - e0);
     int32_t ee = e1 - e0 > DEG_SAMPLE ? e0 + DEG_SAMPLE : e1;
     uint32_t k = 0;
-    for (int32_t e = e0; e < ee; e++) {
-        int32_t u = col[e];
-        if (u == v) continue;
-        uint32_t wu = word[u];
-        k += (!(wu & ST_E) && colour_of(wu) == c) ? 1u : 0u;
+    for (int32_t e = e0; e < ee; e += FEAT_ILP) {  // FEAT_ILP arcs: column ids, then words, then tests
+        int32_t u[FEAT_ILP];
+        uint32_t wu[FEAT_ILP];
+#pragma unroll
+        for (int j = 0; j < FEAT_ILP; j++) u[j] = e + j < ee ? col[e + j] : -1;
+#pragma unroll
+        for (int j = 0; j < FEAT_ILP; j++) wu[j] = u[j] >= 0 && u[j] != v ? word[u[j]] : ST_E;  // no self-loop
+#pragma unroll
+        for (int j = 0; j < FEAT_ILP; j++) k += (!(wu[j] & ST_E) && colour_of(wu[j]) == c) ? 1u : 0u;
     }
     if (e1 - e0 > DEG_SAMPLE) k = (uint32_t)((uint64_t)k * (uint64_t)(e1 - e0) / DEG_SAMPLE);
     return k;
@@ -160,7 +182,7 @@ __device__ inline void load_params(const uint16_t *pb, float *sp) {
 // Features (optional output) and, with parameters, the fused inference: the
 // score of a live vertex is round(sigmoid(logit) * 65535); eliminated vertices
 // score 0 and get zero features.
-__global__ void k_score(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+__global__ __launch_bounds__(MLP_BLOCK) void k_score(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                         const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
                         const uint32_t *word, const float4 *stat, const uint32_t *cnt, int slot_bits,
                         float step_f, int32_t n, const uint16_t *params, uint16_t *score,
@@ -195,7 +217,7 @@ __global__ void k_score(const int32_t *__restrict__ row, const int32_t *__restri
 }
 
 // inference on given feature rows (tests): logits out
-__global__ void k_mlp_infer(const float *X, int64_t N, const uint16_t *params, float *logit) {
+__global__ __launch_bounds__(MLP_BLOCK) void k_mlp_infer(const float *X, int64_t N, const uint16_t *params, float *logit) {
     __shared__ float sp[MLP_PARAMS];
     load_params(params, sp);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -210,7 +232,7 @@ __global__ void k_mlp_infer(const float *X, int64_t N, const uint16_t *params, f
 // training: per-sample forward + backward of the binary cross-entropy of
 // sigmoid(logit) against y, mean over the batch
 // ---------------------------------------------------------------------------
-__global__ void k_mlp_fwdbwd(const float *X, const float *y, const int32_t *idx, int64_t N,
+__global__ __launch_bounds__(MLP_BLOCK) void k_mlp_fwdbwd(const float *X, const float *y, const int32_t *idx, int64_t N,
                              const uint16_t *params, uint16_t *H1, uint16_t *H2, float *dH1,
                              float *dH2, float *ds, float *Xb, float *loss_sum) {
     __shared__ float sp[MLP_PARAMS];
@@ -325,6 +347,8 @@ __global__ void k_fill_f32(float *p, int64_t count, float val) {
 
 // ---------------------------------------------------------------------------
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
+// the same number of work-items in MLP_BLOCK-wide workgroups
+#define GRID_MLP(l) dim3(((l).grid * (l).block + MLP_BLOCK - 1) / MLP_BLOCK), dim3(MLP_BLOCK), 0, (l).stream
 
 void launch_static_features(const Launch &l, const Graph &g, float4 *out) {
     hipLaunchKernelGGL(k_static_features, GRID(l), g.row, g.col, g.rowT, g.colT, g.n, out);
@@ -335,15 +359,15 @@ void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint3
 void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat,
                   const uint32_t *cnt, int slot_bits, float step_f, const uint16_t *params,
                   uint16_t *score, float *feat_out) {
-    hipLaunchKernelGGL(k_score, GRID(l), g.row, g.col, g.rowT, g.colT, word, stat, cnt, slot_bits,
+    hipLaunchKernelGGL(k_score, GRID_MLP(l), g.row, g.col, g.rowT, g.colT, word, stat, cnt, slot_bits,
                        step_f, g.n, params, score, feat_out);
 }
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit) {
-    hipLaunchKernelGGL(k_mlp_infer, GRID(l), X, N, params, logit);
+    hipLaunchKernelGGL(k_mlp_infer, GRID_MLP(l), X, N, params, logit);
 }
 void launch_mlp_fwdbwd(const Launch &l, const float *X, const float *y, const int32_t *idx, int64_t N,
                        const uint16_t *params, MlpWork &w) {
-    hipLaunchKernelGGL(k_mlp_fwdbwd, GRID(l), X, y, idx, N, params, w.H1, w.H2, w.dH1, w.dH2, w.ds,
+    hipLaunchKernelGGL(k_mlp_fwdbwd, GRID_MLP(l), X, y, idx, N, params, w.H1, w.H2, w.dH1, w.dH2, w.ds,
                        w.Xb, w.loss);
 }
 void launch_mlp_grad(hipStream_t s, int64_t N, MlpWork &w) {
