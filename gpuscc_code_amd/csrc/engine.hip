@@ -80,10 +80,10 @@ struct scc_ctx {
         int sched, levels;
         uint32_t pull_thr, huge_thr;
         unsigned long long pull_arcs;
-        int local_rounds;
+        int local_rounds, local_first;
         bool operator==(const GraphKey &o) const {
             return sched == o.sched && levels == o.levels && pull_thr == o.pull_thr && huge_thr == o.huge_thr &&
-                   pull_arcs == o.pull_arcs && local_rounds == o.local_rounds;
+                   pull_arcs == o.pull_arcs && local_rounds == o.local_rounds && local_first == o.local_first;
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> reach_graphs;
@@ -317,6 +317,7 @@ struct ReachChain {
         a.huge_thr = c->opts.huge_row > 0 ? (uint32_t)c->opts.huge_row
                                           : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
         a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
+        a.local_first = c->cur_step > 0 ? 0 : LOCAL_FIRST_LEVEL;  // cur_step: elections before this reach
         const int div = c->opts.hybrid_arc_div > 0 ? c->opts.hybrid_arc_div : 14;
         a.pull_arcs = c->opts.hybrid_arc_div < 0 ? ~0ull
                                                  : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
@@ -362,7 +363,8 @@ struct ReachChain {
 // upload, options per set_opts), so the L launches are captured once as a HIP
 // graph and replayed as one launch.  The chain holds no memset nodes.
 scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
-    const scc_ctx::GraphKey key{rc.sched, L, rc.a.pull_thr, rc.a.huge_thr, rc.a.pull_arcs, rc.a.local_rounds};
+    const scc_ctx::GraphKey key{rc.sched, L, rc.a.pull_thr, rc.a.huge_thr, rc.a.pull_arcs, rc.a.local_rounds,
+                                rc.a.local_first};
     for (auto &kv : c->reach_graphs)
         if (kv.first == key) {
             HIPCHK(c, hipGraphLaunch(kv.second, c->stream));

@@ -53,6 +53,7 @@ struct ReachArgs {
                         // per-use load from the dispatch packet)
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
+    int local_first;    // first level that may run local rounds
     int instr;          // count work (expanded vertices, scanned arcs, visits)
     // trim levels
     int32_t *root;      // trimmed vertices become their own SCC
@@ -61,6 +62,12 @@ struct ReachArgs {
     int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
                         // two or more vertices, which no trim removes, so those vertices are not checked
 };
+
+// Local rounds start at this level in the first FB step: its first levels are
+// thin on every graph (one pivot), and a hub among them would be expanded by a
+// single work-item.  Later steps start from many pivots on small colour sets,
+// where local rounds from level 0 finish most reaches in one launch.
+constexpr int LOCAL_FIRST_LEVEL = 4;
 
 struct Graph {  // device CSR + CSR^T
     int32_t n;

@@ -535,7 +535,6 @@ __device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, 
 
 constexpr int32_t LOCAL_MAX_DEG = 64;       // longer rows leave the local rounds for the next level
 constexpr uint32_t LOCAL_ITEMS_PER_WG = 8;  // local rounds only when a level's frontier is this thin
-constexpr int LOCAL_FIRST_LEVEL = 4;        // ... and the reach is this many levels deep
 
 // Workgroup-local rounds on the visits staged by this workgroup: each round
 // moves the staged vertices into lq and expands them one per lane; rows longer
@@ -549,9 +548,8 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                                     bool &found, unsigned long long &cx, unsigned long long &ce,
                                     unsigned long long &cv) {
     // a thick frontier keeps every workgroup busy without them; a thin one
-    // (high-diameter graphs) would leave most workgroups idle per level.  The
-    // first levels of a reach are thin on every graph, so they never qualify.
-    if (a.iter < LOCAL_FIRST_LEVEL || level_items > LOCAL_ITEMS_PER_WG * gridDim.x) return;
+    // (high-diameter graphs) would leave most workgroups idle per level.
+    if (a.iter < a.local_first || level_items > LOCAL_ITEMS_PER_WG * gridDim.x) return;
     __syncthreads();  // the level's pushes are staged; the items sharing lq's memory are done with
     if (threadIdx.x < 2) {
         lq.rc[0][threadIdx.x] = st.cnt[threadIdx.x];
