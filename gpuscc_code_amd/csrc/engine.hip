@@ -272,6 +272,8 @@ struct TrimChain {
         a.instr = c->instr;
         a.head[0] = c->d_head;
         a.head[1] = c->d_head + c->g.n;
+        a.ell[0] = c->d_ell;
+        a.ell[1] = c->d_ell + c->g.n;
         a.dirs = 1;
     }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
