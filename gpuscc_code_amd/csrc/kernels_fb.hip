@@ -35,18 +35,44 @@ __device__ inline bool row_has_other(uint32_t k) { return (k & 0x3Fu) != 0 && (k
 // iff its row there holds a vertex other than v -- a test of two class bytes.
 // Vertices failing it start trimmed (P|T|E, scc_routines.cu:381).  Adds the
 // live and trimmed counts to ctl->live and ctl->pcount (zeroed before).
+// Rounds of INIT_V vertices per work-item: the class bytes of a round load
+// together (a loop of one vertex per iteration waits for each load in turn).
+// In a batch the graph of a work-item's first vertex is found by binary search
+// and later vertices (increasing ids) advance from it.
+constexpr int INIT_V = 8;
+
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
                        const uint8_t *cls_out, const uint8_t *cls_in, Ctl *ctl) {
     __shared__ unsigned long long sh[16];
     int64_t beg, end;
     my_range(n, beg, end);
+    const uint32_t bs = blockDim.x;
     unsigned long long nt = 0;
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
-        uint32_t c = off ? ((graph_of(off, G, (int32_t)v) + 1u) & COLOUR_MASK) : 1u;
-        const bool trim = cls_out && !(row_has_other(cls_out[v]) && row_has_other(cls_in[v]));
-        word[v] = make_word(c, trim ? 28u : 0u);
-        root[v] = (int32_t)v;
-        nt += trim ? 1u : 0u;
+    uint32_t g = 0;  // batch: graph of the current vertex
+    if (off && beg + threadIdx.x < end) g = graph_of(off, G, (int32_t)(beg + threadIdx.x));
+    for (int64_t base = beg; base < end; base += (int64_t)bs * INIT_V) {
+        uint32_t co[INIT_V], ci[INIT_V];
+#pragma unroll
+        for (int k = 0; k < INIT_V; k++) {
+            const int64_t v = base + (int64_t)k * bs + threadIdx.x;
+            const bool in = cls_out && v < end;
+            co[k] = in ? cls_out[v] : 0u;
+            ci[k] = in ? cls_in[v] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < INIT_V; k++) {
+            const int64_t v = base + (int64_t)k * bs + threadIdx.x;
+            if (v >= end) continue;
+            uint32_t c = 1u;
+            if (off) {
+                while (off[g + 1] <= v) g++;
+                c = (g + 1u) & COLOUR_MASK;
+            }
+            const bool trim = cls_out && !(row_has_other(co[k]) && row_has_other(ci[k]));
+            word[v] = make_word(c, trim ? 28u : 0u);
+            root[v] = (int32_t)v;
+            nt += trim ? 1u : 0u;
+        }
     }
     nt = block_sum(nt, sh);
     if (threadIdx.x == 0) {
@@ -100,7 +126,13 @@ __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *_
         int32_t e0 = row[v], e1 = row[v + 1];
         uint32_t deg = (uint32_t)(e1 - e0);
         bool self = false;
-        for (int32_t e = e0; e < e1 && !self; e++) self = col[e] == (int32_t)v;
+        for (int32_t e = e0; e < e1 && !self; e += 8) {  // eight column ids in flight per step
+            int32_t u[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) u[j] = e + j < e1 ? col[e + j] : -1;
+#pragma unroll
+            for (int j = 0; j < 8; j++) self = self || u[j] == (int32_t)v;
+        }
         out[v] = (uint8_t)((deg ? 32 - __clz(deg) : 0) | (deg <= 4 ? CLS_FITS : 0) | (self ? 0x80 : 0));
         head[v] = e0 < e1 ? col[e0] : -1;
         int4 q;
