@@ -27,7 +27,8 @@ class SccOpts(ctypes.Structure):
                 ("hybrid_threshold", ctypes.c_int32), ("slot_bits", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("sequential_reach", ctypes.c_int32),
                 ("huge_row", ctypes.c_int32), ("local_rounds", ctypes.c_int32),
-                ("hybrid_arc_div", ctypes.c_int32), ("reach_graphs", ctypes.c_int32)]
+                ("hybrid_arc_div", ctypes.c_int32), ("reach_graphs", ctypes.c_int32),
+                ("reach_block", ctypes.c_int32)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",

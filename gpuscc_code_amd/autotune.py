@@ -37,10 +37,11 @@ KNOBS = {
     "huge_row": lambda n, cus: [0, 1024, 16384],
     "block_size": lambda n, cus: [0, 128, 512],
     "reach_graphs": lambda n, cus: [0, 1],
+    "reach_block": lambda n, cus: [0, 512],
     "hybrid_policy": lambda n, cus: [0, 2],
 }
-ORDER = ["hybrid_policy", "grid_blocks", "hybrid_arc_div", "hybrid_threshold", "local_rounds", "huge_row",
-         "block_size", "reach_graphs"]
+ORDER = ["hybrid_policy", "grid_blocks", "reach_block", "hybrid_arc_div", "hybrid_threshold", "local_rounds",
+         "huge_row", "block_size", "reach_graphs"]
 
 
 def log(*a):

@@ -124,9 +124,9 @@ Launch launch_for(const scc_ctx *c) {
     return Launch{c->stream, grid, block};
 }
 
-Launch fixed_launch(const scc_ctx *c) {  // kernels with LDS sized for 256 work-items
+Launch fixed_launch(const scc_ctx *c) {  // level kernels: LDS sized for up to 512 work-items
     Launch l = launch_for(c);
-    l.block = 256;
+    l.block = c->opts.reach_block == 512 ? 512 : 256;
     uint64_t need = ((uint64_t)std::max<int32_t>(c->g.n, 1) + 255) / 256;
     uint32_t grid = c->opts.grid_blocks > 0 ? (uint32_t)c->opts.grid_blocks : (uint32_t)c->cu_count * 4u;
     if (grid > need) grid = (uint32_t)std::max<uint64_t>(need, 1);
@@ -617,8 +617,9 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
     if (!c) return SCC_E_OOM;
     c->device = device;
     if (opts) c->opts = *opts;
-    if (c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
-                                    (c->opts.block_size & 63) != 0)) {
+    if ((c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
+                                     (c->opts.block_size & 63) != 0)) ||
+        (c->opts.reach_block != 0 && c->opts.reach_block != 256 && c->opts.reach_block != 512)) {
         delete c;
         return SCC_E_INPUT;
     }
@@ -656,6 +657,8 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
     if (opts->block_size != 0 &&
         (opts->block_size < 64 || opts->block_size > 1024 || (opts->block_size & 63) != 0))
         return fail(c, SCC_E_INPUT, "block_size must be a multiple of 64 in [64, 1024]");
+    if (opts->reach_block != 0 && opts->reach_block != 256 && opts->reach_block != 512)
+        return fail(c, SCC_E_INPUT, "reach_block must be 0, 256 or 512");
     if (opts->grid_blocks < 0) return fail(c, SCC_E_INPUT, "grid_blocks < 0");
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");

@@ -186,7 +186,7 @@ __device__ inline uint32_t find_item(const uint32_t *off, uint32_t cnt, uint32_t
 // arcs t, t + blockDim, ... of their concatenation (the CTA gather of
 // reach_t.cu:195-265 and reach_l.cu:190-245, without their 256-arc windows)
 // ---------------------------------------------------------------------------
-constexpr uint32_t ITEM_CAP = 512;  // two items per work-item of a 256-wide workgroup
+constexpr uint32_t ITEM_CAP = 1024;  // two items per work-item of a workgroup of up to 512
 
 struct Items {
     uint32_t off[ITEM_CAP];

@@ -41,6 +41,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     local_rounds: int = 0
     hybrid_arc_div: int = 0
     reach_graphs: int = 0
+    reach_block: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -54,6 +55,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.local_rounds = self.local_rounds
         o.hybrid_arc_div = self.hybrid_arc_div
         o.reach_graphs = self.reach_graphs
+        o.reach_block = self.reach_block
         return o
 
 

@@ -80,6 +80,8 @@ typedef struct {
                                   from degree classes) reach m / this (0 = default 14, -1 = off) */
     int32_t reach_graphs;      /* 1: the reach chains of speculative steps are captured once as HIP
                                   graphs and replayed (one launch per chain); 0 = launched level by level */
+    int32_t reach_block;       /* workgroup size of the level kernels (reach t/d/l/h, trim): 256 or 512
+                                  (0 = 256) */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */

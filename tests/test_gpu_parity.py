@@ -433,6 +433,24 @@ def test_pull_cost_policy(ctx):
         ctx.set_opts(scc.alg_opt())
 
 
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_wide_level_workgroups(ctx, sched):
+    """reach_block 512 (512-wide workgroups for the level kernels: item and
+    staging capacities at their limits) changes only the launch shape: full
+    state parity, and a forced huge-row split and forced pulls on top."""
+    try:
+        for opts in (scc.alg_opt(reach_block=512), scc.alg_opt(reach_block=512, huge_row=8, hybrid_threshold=64)):
+            ctx.set_opts(opts)
+            for g in (scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1), scc.gen_mesh(128, 0.6, seed=3),
+                      demo_graph(), random_digraph(3000, 12000, seed=12, self_loops=True)):
+                for variant in (0, 1, 2):
+                    check_against_oracle(ctx, g, sched, variant)
+        with pytest.raises(scc.SccError):
+            ctx.set_opts(scc.alg_opt(reach_block=384))
+    finally:
+        ctx.set_opts(scc.alg_opt())
+
+
 def test_level_log(ctx):
     """Profile flag 4 logs one row per level launched: each chain is a run of
     levels 0, 1, 2, ...; one reach chain per FB step; a level's longest serial
