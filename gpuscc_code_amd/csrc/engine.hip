@@ -41,6 +41,7 @@ struct scc_ctx {
     uint8_t *d_dcls = nullptr;    // degree classes: out-degrees [0, n), in-degrees [n, 2n)
     int32_t *d_head = nullptr;    // first entry of each row: out-rows [0, n), in-rows [n, 2n)
     int4 *d_ell = nullptr;        // first four entries of each row, same halves
+    uint8_t *d_hlive = nullptr;   // first entry of each row left live by k_init, same halves
     uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
@@ -258,6 +259,7 @@ struct TrimChain {
     scc_ctx *c;
     Launch l;
     ReachArgs a{};
+    bool after_init = false;  // the chain of trim1 right after k_init: its sweep may use hlive
     explicit TrimChain(scc_ctx *cc, bool skip_fb = false) : c(cc), l(fixed_launch(cc)) {
         a.row = c->g.row;
         a.col = c->g.col;
@@ -283,6 +285,7 @@ struct TrimChain {
         }
         a.iter = it;
         a.tag = c->trim_tag;
+        a.hlive = after_init && it == 0 ? c->d_hlive : nullptr;
         a.inq[0] = c->d_q[(it - 1) & 1];
         a.outq[0] = c->d_q[it & 1];
         a.inq[1] = a.inq[0];
@@ -391,8 +394,10 @@ scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
 // first: trim1 after k_init (its guess slot; k_init's reset started the chain);
 // after_reach: right after a complete reach.  Level 0 sweeps: it reads no
 // input queue, so the seed lists stay as they are.
+// first: trim1 right after k_init (states exactly as k_init left them)
 scc_status do_trim(scc_ctx *c, int64_t *iters, bool first = false, int64_t resume = 0, bool after_reach = false) {
     TrimChain tc(c, after_reach);
+    tc.after_init = first;
     return run_chain(c, tc, c->guess[first ? 3 : 2], first ? -1 : 2, iters, 1, resume);
 }
 
@@ -525,6 +530,8 @@ void free_graph(scc_ctx *c) {
     c->d_word = nullptr;
     if (c->d_ell) (void)hipFree(c->d_ell);
     c->d_ell = nullptr;
+    if (c->d_hlive) (void)hipFree(c->d_hlive);
+    c->d_hlive = nullptr;
     for (auto &t : c->d_slots) {
         if (t) (void)hipFree(t);
         t = nullptr;
@@ -755,6 +762,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMalloc(&c->d_head, 2 * vb));
     HIPCHK(c, hipMalloc(&c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, hipMalloc(&c->d_hlive, 2 * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -782,6 +790,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
     launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head, c->d_ell);
     launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n, c->d_ell + n);
+    launch_head_live(l, c->d_dcls, c->d_head, n, c->d_hlive);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());

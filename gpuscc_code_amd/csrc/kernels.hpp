@@ -61,6 +61,9 @@ struct ReachArgs {
     uint32_t tag;       // this level's tag (unique per launch)
     int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
                         // two or more vertices, which no trim removes, so those vertices are not checked
+    const uint8_t *hlive;  // first sweep of trim1 only (null otherwise): per vertex and direction
+                           // ([0, n) out-rows, [n, 2n) in-rows), 1 when the row's first entry is
+                           // another vertex that k_init leaves live -- a live same-colour neighbour
 };
 
 // Local rounds start at this level in the first FB step: its first levels are
@@ -93,6 +96,10 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v);
 constexpr uint32_t CLS_FITS = 0x40;
 void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
                          int32_t *head, int4 *ell);
+// hlive[v] (and hlive[n + v]) = 1 when the first entry of v's out-row (in-row) is a vertex
+// other than v whose own rows hold another vertex each (k_init leaves it live); cls, head:
+// out-rows [0, n), in-rows [n, 2n)
+void launch_head_live(const Launch &l, const uint8_t *cls, const int32_t *head, int32_t n, uint8_t *hlive);
 // control-block resets (bits: k_ctl_reset), after a chain guard when stage > 0 (a chain of
 // last + 1 levels launched without a wait: its depth, and abort = stage unless it converged)
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0);

@@ -144,6 +144,19 @@ __global__ void k_degree_class(const int32_t *__restrict__ row, const int32_t *_
     }
 }
 
+// Upload-time index for the first sweep of trim1: whether the first entry of a
+// row is a vertex that k_init leaves live.  Every arc joins two vertices of one
+// colour when a run starts, so such a row already holds a live same-colour
+// neighbour and the sweep needs no gather for it.
+__global__ void k_head_live(const uint8_t *__restrict__ cls, const int32_t *__restrict__ head, int32_t n,
+                            uint8_t *hlive) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (int64_t)n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = (int32_t)(i < n ? i : i - n), h = head[i];
+        hlive[i] = (h >= 0 && h != v && row_has_other(cls[h]) && row_has_other(cls[(int64_t)n + h])) ? 1 : 0;
+    }
+}
+
 // Chain / counter resets (a kernel, not hipMemsetAsync, so it is safe in
 // graphs), preceded by a chain guard when guard_stage > 0: the chain that
 // ended with level guard_last records its depth in depth[guard_stage - 1] and
@@ -622,6 +635,9 @@ void launch_fill_i32(const Launch &l, int32_t *p, int64_t count, int32_t v) {
 void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col, int32_t n, uint8_t *out,
                          int32_t *head, int4 *ell) {
     hipLaunchKernelGGL(k_degree_class, GRID(l), row, col, n, out, head, ell);
+}
+void launch_head_live(const Launch &l, const uint8_t *cls, const int32_t *head, int32_t n, uint8_t *hlive) {
+    hipLaunchKernelGGL(k_head_live, GRID(l), cls, head, n, hlive);
 }
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int guard_stage) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what, guard_last, guard_stage);

@@ -1012,7 +1012,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
         for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
         const int32_t *col[T];
         int32_t e[T], e1[T], u0[T], self[T];
-        uint32_t mask[T], val[T], c[T], act = 0;
+        uint32_t mask[T], val[T], c[T], act = 0, known = 0;
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const int d = t / SCAN_ILP, j = t % SCAN_ILP;
@@ -1024,6 +1024,11 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             u0[t] = -1;
             e[t] = e1[t] = 0;
             if (v[j] >= 0) {
+                // first sweep of trim1: a first entry k_init left live needs no scan
+                if (a.hlive && a.hlive[(int64_t)d * a.n + v[j]]) {
+                    known |= 1u << t;
+                    continue;
+                }
                 u0[t] = a.head[d][v[j]];
                 e[t] = row[v[j]] + 1;
                 e1[t] = row[v[j] + 1];
@@ -1033,7 +1038,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
 #pragma unroll
         for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
         // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
-        uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
+        uint32_t hit = known | scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) {
             if (v[j] >= 0 && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
