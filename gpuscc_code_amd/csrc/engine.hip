@@ -259,7 +259,7 @@ struct TrimChain {
     scc_ctx *c;
     Launch l;
     ReachArgs a{};
-    bool after_init = false;  // the chain of trim1 right after k_init: its sweep may use hlive
+    bool after_init = false;  // the chain of trim1 right after k_init: level 0 checks k_init's list
     explicit TrimChain(scc_ctx *cc, bool skip_fb = false) : c(cc), l(fixed_launch(cc)) {
         a.row = c->g.row;
         a.col = c->g.col;
@@ -285,7 +285,7 @@ struct TrimChain {
         }
         a.iter = it;
         a.tag = c->trim_tag;
-        a.hlive = after_init && it == 0 ? c->d_hlive : nullptr;
+        a.init_list = after_init && it == 0 ? 1 : 0;
         a.inq[0] = c->d_q[(it - 1) & 1];
         a.outq[0] = c->d_q[it & 1];
         a.inq[1] = a.inq[0];
@@ -835,11 +835,14 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     auto body = [&]() -> scc_status {
         Launch l = launch_for(c);
         // k_init runs the first level of trim1 (a degree test) on the words it
-        // writes; its control-block reset also starts trim1's chain
+        // writes and lists, as the input of trim1's first level, the live
+        // vertices whose first out- or in-entry may not stay live; its
+        // control-block reset also starts trim1's chain (qsize[0][0] = 0, where
+        // k_init counts its list)
         c->pend_bits = c->pend_stage = 0;
-        ctl_op(c, 512 | 1024 | 2 | 1);
+        ctl_op(c, 512 | 1024 | 128 | 2 | 1);
         launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
-                    c->d_ctl);
+                    c->d_hlive, c->d_q[1], c->d_ctl);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();

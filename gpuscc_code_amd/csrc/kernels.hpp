@@ -61,9 +61,7 @@ struct ReachArgs {
     uint32_t tag;       // this level's tag (unique per launch)
     int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
                         // two or more vertices, which no trim removes, so those vertices are not checked
-    const uint8_t *hlive;  // first sweep of trim1 only (null otherwise): per vertex and direction
-                           // ([0, n) out-rows, [n, 2n) in-rows), 1 when the row's first entry is
-                           // another vertex that k_init leaves live -- a live same-colour neighbour
+    int init_list;      // the first level of trim1: check only the vertices k_init listed in inq[0]
 };
 
 // Local rounds start at this level in the first FB step: its first levels are
@@ -83,8 +81,11 @@ struct Graph {  // device CSR + CSR^T
 // (degree classes, out [0, n) and in [n, 2n)) the vertices whose out- or in-row
 // holds no vertex but themselves start trimmed (the first level of trim1).
 // Adds the live and trimmed counts to ctl->live and ctl->pcount (reset bit 1024 zeroes them).
+// With hlive (k_head_live) the live vertices whose first out- or in-entry is not known to
+// stay live are appended to unk, their count added to ctl->qsize[0][0] (zeroed before):
+// the first level of trim1 checks only them.
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, const uint8_t *cls, Ctl *ctl);
+                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);

@@ -41,24 +41,34 @@ __device__ inline bool row_has_other(uint32_t k) { return (k & 0x3Fu) != 0 && (k
 // and later vertices (increasing ids) advance from it.
 constexpr int INIT_V = 8;
 
+// With hlive given, the live vertices whose out- or in-row does not start with
+// another vertex that stays live (hlive, k_head_live) are appended to unk (count
+// in ctl->qsize[0][0]): the first level of trim1 checks only those, since every
+// other live vertex has a live same-colour neighbour both ways.
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
-                       const uint8_t *cls_out, const uint8_t *cls_in, Ctl *ctl) {
+                       const uint8_t *cls_out, const uint8_t *cls_in, const uint8_t *hlive, int32_t *unk,
+                       Ctl *ctl) {
     __shared__ unsigned long long sh[16];
+    __shared__ uint32_t shw[16];
+    __shared__ uint32_t sbase;
     int64_t beg, end;
     my_range(n, beg, end);
     const uint32_t bs = blockDim.x;
     unsigned long long nt = 0;
     uint32_t g = 0;  // batch: graph of the current vertex
     if (off && beg + threadIdx.x < end) g = graph_of(off, G, (int32_t)(beg + threadIdx.x));
-    for (int64_t base = beg; base < end; base += (int64_t)bs * INIT_V) {
-        uint32_t co[INIT_V], ci[INIT_V];
+    for (int64_t base = beg; base < end; base += (int64_t)bs * INIT_V) {  // block-uniform
+        uint32_t co[INIT_V], ci[INIT_V], ho[INIT_V], hi[INIT_V];
 #pragma unroll
         for (int k = 0; k < INIT_V; k++) {
             const int64_t v = base + (int64_t)k * bs + threadIdx.x;
             const bool in = cls_out && v < end;
             co[k] = in ? cls_out[v] : 0u;
             ci[k] = in ? cls_in[v] : 0u;
+            ho[k] = in && hlive ? hlive[v] : 0u;
+            hi[k] = in && hlive ? hlive[(int64_t)n + v] : 0u;
         }
+        uint32_t um = 0;  // live vertices of this round the first trim level must check
 #pragma unroll
         for (int k = 0; k < INIT_V; k++) {
             const int64_t v = base + (int64_t)k * bs + threadIdx.x;
@@ -72,6 +82,18 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
             word[v] = make_word(c, trim ? 28u : 0u);
             root[v] = (int32_t)v;
             nt += trim ? 1u : 0u;
+            if (!trim && !(ho[k] && hi[k])) um |= 1u << k;
+        }
+        if (hlive) {  // one prefix sum and one atomicAdd per round and workgroup
+            uint32_t total;
+            const uint32_t pre = block_excl_scan((uint32_t)__popc(um), shw, total, bs);
+            if (total == 0) continue;  // block-uniform
+            if (threadIdx.x == 0) sbase = atomicAdd(&ctl->qsize[0][0], total);
+            __syncthreads();
+            uint32_t o = sbase + pre;
+            for (uint32_t m = um; m; m &= m - 1)
+                unk[o++] = (int32_t)(base + (int64_t)(__ffs(m) - 1) * bs + threadIdx.x);
+            __syncthreads();  // sbase is rewritten by the next round
         }
     }
     nt = block_sum(nt, sh);
@@ -617,8 +639,9 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, const uint8_t *cls, Ctl *ctl) {
-    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr, ctl);
+                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl) {
+    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr, cls ? hlive : nullptr,
+                       unk, ctl);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);

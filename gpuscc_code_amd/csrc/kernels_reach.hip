@@ -1012,7 +1012,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
         for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
         const int32_t *col[T];
         int32_t e[T], e1[T], u0[T], self[T];
-        uint32_t mask[T], val[T], c[T], act = 0, known = 0;
+        uint32_t mask[T], val[T], c[T], act = 0;
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const int d = t / SCAN_ILP, j = t % SCAN_ILP;
@@ -1024,11 +1024,6 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
             u0[t] = -1;
             e[t] = e1[t] = 0;
             if (v[j] >= 0) {
-                // first sweep of trim1: a first entry k_init left live needs no scan
-                if (a.hlive && a.hlive[(int64_t)d * a.n + v[j]]) {
-                    known |= 1u << t;
-                    continue;
-                }
                 u0[t] = a.head[d][v[j]];
                 e[t] = row[v[j]] + 1;
                 e1[t] = row[v[j] + 1];
@@ -1038,7 +1033,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
 #pragma unroll
         for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
         // live same-colour neighbours: bit j (out-row), bit SCAN_ILP + j (in-row)
-        uint32_t hit = known | scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
+        uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
 #pragma unroll
         for (int j = 0; j < SCAN_ILP; j++) {
             if (v[j] >= 0 && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
@@ -1061,8 +1056,32 @@ __global__ void k_trim_level(ReachArgs a) {
     __syncthreads();
     bool found = false;
     uint32_t smax = 0;  // longest row scan of this work-item (instrumented)
-    const uint32_t qs = a.iter == 0 ? 0u : a.ctl->qsize[0][a.iter & 3];
-    if (a.iter == 0 || qs >= (uint32_t)(a.n / TRIM_SWEEP_DIV)) {
+    const uint32_t qs = a.iter == 0 && !a.init_list ? 0u : a.ctl->qsize[0][a.iter & 3];
+    if (a.init_list) {
+        // the first level of trim1: only the vertices k_init listed (a first out- or
+        // in-entry not known to stay live) are checked, each with full row scans
+        unsigned long long ntrim = 0;
+        const uint32_t chunk = (qs + gridDim.x - 1) / gridDim.x;
+        const uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < qs ? b0 + chunk : qs;
+        for (uint32_t base = b0; base < b1; base += a.nt) {  // block-uniform
+            const uint32_t i = base + threadIdx.x;
+            if (i < b1) {
+                const int32_t v = a.inq[0][i];
+                const uint32_t w = a.word[v];
+                if (!(w & ST_E) && trim_check(a, v, colour_of(w), smax)) {
+                    found = true;
+                    ntrim++;
+                    st_push(st, 0, v, a);
+                }
+            }
+            st_maybe_flush(st, a);
+        }
+        ntrim = block_sum(ntrim, sh, a.nt);
+        if (threadIdx.x == 0 && ntrim) {  // no pivots exist yet: every trim is a new P
+            atomicSub(&a.ctl->live, (uint32_t)ntrim);
+            atomicAdd(&a.ctl->pcount, (uint32_t)ntrim);
+        }
+    } else if (a.iter == 0 || qs >= (uint32_t)(a.n / TRIM_SWEEP_DIV)) {
         // sweep: rounds of TRIM_V vertices per work-item, no barrier until the
         // round's trims are queued
         int64_t beg, end;
