@@ -62,6 +62,7 @@ struct scc_ctx {
     bool stat_valid = false;
     int adam_t = 0;
     int cur_step = 0;
+    bool one_colour = false;  // scc_run's first FB step: every arc joins two vertices of one colour
     bool uploaded = false, labels_valid = false;
     bool label_in_root = false;  // the last run's labels are the root array itself (plain rule)
     int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
@@ -81,10 +82,11 @@ struct scc_ctx {
         int sched, levels;
         uint32_t pull_thr, huge_thr;
         unsigned long long pull_arcs;
-        int local_rounds, local_first;
+        int local_rounds, local_first, one_colour;
         bool operator==(const GraphKey &o) const {
             return sched == o.sched && levels == o.levels && pull_thr == o.pull_thr && huge_thr == o.huge_thr &&
-                   pull_arcs == o.pull_arcs && local_rounds == o.local_rounds && local_first == o.local_first;
+                   pull_arcs == o.pull_arcs && local_rounds == o.local_rounds && local_first == o.local_first &&
+                   one_colour == o.one_colour;
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> reach_graphs;
@@ -323,6 +325,7 @@ struct ReachChain {
                                           : (sched == SCC_SCHED_Q || sched == SCC_SCHED_D ? 1024u : 4096u);
         a.local_rounds = c->opts.local_rounds < 0 ? 0 : (c->opts.local_rounds > 0 ? c->opts.local_rounds : 64);
         a.local_first = c->cur_step > 0 ? 0 : LOCAL_FIRST_LEVEL;  // cur_step: elections before this reach
+        a.one_colour = c->one_colour ? 1 : 0;
         const int div = c->opts.hybrid_arc_div > 0 ? c->opts.hybrid_arc_div : 14;
         a.pull_arcs = c->opts.hybrid_arc_div < 0 ? ~0ull
                                                  : std::max<unsigned long long>(1, (unsigned long long)c->g.m / div);
@@ -369,7 +372,7 @@ struct ReachChain {
 // graph and replayed as one launch.  The chain holds no memset nodes.
 scc_status launch_reach_levels(scc_ctx *c, ReachChain &rc, int L) {
     const scc_ctx::GraphKey key{rc.sched, L, rc.a.pull_thr, rc.a.huge_thr, rc.a.pull_arcs, rc.a.local_rounds,
-                                rc.a.local_first};
+                                rc.a.local_first, rc.a.one_colour};
     for (auto &kv : c->reach_graphs)
         if (kv.first == key) {
             HIPCHK(c, hipGraphLaunch(kv.second, c->stream));
@@ -874,6 +877,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                              !by_elim && c->prof_flags == 0;
         while (unsynced || npiv > 0) {
             st.steps++;
+            c->one_colour = st.steps == 1;  // no recolouring yet: every arc stays inside a colour
             int sched = schedule == SCC_SCHED_B ? SCC_SCHED_H : schedule;
             if (by_elim) sched = ne < one_pct ? (schedule == SCC_SCHED_C ? SCC_SCHED_L : SCC_SCHED_T) : SCC_SCHED_Q;
             int64_t d = 0;
@@ -1012,6 +1016,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return SCC_OK;
     };
     s = body();
+    c->one_colour = false;
     if (s != SCC_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_total = ms_since(t_start);

@@ -54,6 +54,8 @@ struct ReachArgs {
     uint32_t huge_thr;  // rows of at least this many arcs are split over all workgroups
     int local_rounds;   // d: rounds of workgroup-local expansion after the level's share
     int local_first;    // first level that may run local rounds
+    int one_colour;     // every arc joins two vertices of one colour (the first FB step of a run): the
+                        // local rounds claim without reading the target's word first
     int instr;          // count work (expanded vertices, scanned arcs, visits)
     // trim levels
     int32_t *root;      // trimmed vertices become their own SCC

@@ -500,7 +500,13 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, uint32_t *cn
 // k has CLS_FITS): the column ids come with v's own word in the first round
 // trip, so a short row costs three dependent accesses (head, target words,
 // claims) instead of five (row bounds, own word, column ids, words, claims).
-template <bool WAVE_PUSH>
+//
+// Without PRECHECK (only when every arc joins two vertices of one colour) the
+// claims go out without reading the target words first: a target that is
+// already reached keeps its word, and one that is eliminated (E) gets a bit no
+// later kernel reads (every one skips E) and is not pushed, since the claim's
+// old value shows E -- two dependent accesses instead of three.
+template <bool WAVE_PUSH, bool PRECHECK = true>
 __device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, int d, int32_t v,
                                   uint32_t c, int4 nb, bool &found, unsigned long long &cx,
                                   unsigned long long &ce, unsigned long long &cv) {
@@ -515,17 +521,23 @@ __device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, 
     }
     cx++;
     ce += deg;
-    uint32_t wu[4], old[4];
+    uint32_t old[4];
+    if (PRECHECK) {
+        uint32_t wu[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
+        for (int j = 0; j < 4; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        old[j] = bit;
-        if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+        for (int j = 0; j < 4; j++) {
+            old[j] = bit;
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) old[j] = u[j] >= 0 ? atomicOr(&a.word[u[j]], bit) : bit;
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        if (old[j] & bit) continue;
+        if (old[j] & (bit | ST_E)) continue;
         found = true;
         cv++;
         if (WAVE_PUSH) st_push_wave(st, d, u[j], a);
@@ -581,7 +593,8 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
                 const int4 nb = dell(a, d)[v];
                 asm volatile("" ::"v"(nb.x));
                 if (cls & CLS_FITS) {
-                    expand_fit<false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
+                    if (a.one_colour) expand_fit<false, false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
+                    else expand_fit<false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
                 } else {
                     const int32_t *row = drow(a, d);
                     if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
