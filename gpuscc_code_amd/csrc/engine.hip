@@ -164,10 +164,18 @@ scc_status detach_labels(scc_ctx *c) {
     return SCC_OK;
 }
 
+// The device copies the control block into the pinned mirror with the queued
+// guard and resets (a queued bit 64 alone stays queued for the next reset), then
+// the host waits.
 scc_status read_ctl(scc_ctx *c) {
-    if (c->pend_stage || (c->pend_bits & ~64)) ctl_op(c, 0);
+    if (c->pend_stage || (c->pend_bits & ~64)) {
+        launch_ctl_reset(c->stream, c->d_ctl, c->pend_bits, c->pend_last, c->pend_stage, c->h_ctl);
+        c->pend_bits = 0;
+        c->pend_stage = 0;
+    } else {
+        launch_ctl_reset(c->stream, c->d_ctl, 0, 0, 0, c->h_ctl);
+    }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->syncs++;
     return SCC_OK;

@@ -104,8 +104,9 @@ void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col
 // out-rows [0, n), in-rows [n, 2n)
 void launch_head_live(const Launch &l, const uint8_t *cls, const int32_t *head, int32_t n, uint8_t *hlive);
 // control-block resets (bits: k_ctl_reset), after a chain guard when stage > 0 (a chain of
-// last + 1 levels launched without a wait: its depth, and abort = stage unless it converged)
-void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0);
+// last + 1 levels launched without a wait: its depth, and abort = stage unless it converged);
+// with publish, the block is then copied to that (pinned host) mirror
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0, Ctl *publish = nullptr);
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,

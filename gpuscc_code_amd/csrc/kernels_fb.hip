@@ -186,8 +186,7 @@ __global__ void k_head_live(const uint8_t *__restrict__ cls, const int32_t *__re
 // need more levels).  With abort set the resets are skipped, unless bit 512
 // clears it.  Bit order of application: 1024, 64, 128, 1, 2, 4, 8, 16, 32
 // (seeds before the queue rings read them, npiv before it is cleared).
-__global__ void k_ctl_reset(Ctl *ctl, int what, int guard_last, int guard_stage) {
-    if (threadIdx.x != 0) return;
+__device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int guard_stage) {
     if (guard_stage > 0 && !ctl->abort) {
         uint32_t ran = ctl->ran;
         ctl->depth[guard_stage - 1] = ran;
@@ -238,6 +237,17 @@ __global__ void k_ctl_reset(Ctl *ctl, int what, int guard_last, int guard_stage)
         ctl->st_edges = 0;
         ctl->st_visited = 0;
     }
+}
+
+// With publish (the host's pinned mirror of the block), the block is copied
+// there after the resets, so a host wait needs no separate copy.
+__global__ void k_ctl_reset(Ctl *ctl, int what, int guard_last, int guard_stage, Ctl *publish) {
+    if (threadIdx.x == 0) ctl_reset_body(ctl, what, guard_last, guard_stage);
+    if (!publish) return;
+    __syncthreads();
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(ctl);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(publish);
+    for (uint32_t i = threadIdx.x; i < sizeof(Ctl) / 4; i += blockDim.x) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -662,8 +672,8 @@ void launch_degree_class(const Launch &l, const int32_t *row, const int32_t *col
 void launch_head_live(const Launch &l, const uint8_t *cls, const int32_t *head, int32_t n, uint8_t *hlive) {
     hipLaunchKernelGGL(k_head_live, GRID(l), cls, head, n, hlive);
 }
-void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int guard_stage) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what, guard_last, guard_stage);
+void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int guard_stage, Ctl *publish) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, what, guard_last, guard_stage, publish);
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
