@@ -573,7 +573,11 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
         __syncthreads();  // the pushes of the round before are complete
         uint32_t c0 = lq.rc[ci][0] < SQ_CAP ? lq.rc[ci][0] : SQ_CAP;
         uint32_t c1 = lq.rc[ci][1] < SQ_CAP ? lq.rc[ci][1] : SQ_CAP;
-        if (c0 + c1 == 0) break;  // block-uniform: read after the barrier
+        // block-uniform (read after the barrier): stop when nothing is staged, or when
+        // the workgroup's own frontier outgrew one item per work-item -- a wide
+        // frontier is the next level's work (balanced over all workgroups), and its
+        // pushes would overflow the staging buffers one global atomic at a time
+        if (c0 + c1 == 0 || lq.rc[ci][0] + lq.rc[ci][1] > a.nt) break;
         const int co = ci == 2 ? 0 : ci + 1, cz = co == 2 ? 0 : co + 1;
         for (uint32_t k = threadIdx.x; k < c0; k += a.nt) lq.v[k] = st.buf[0][k];
         for (uint32_t k = threadIdx.x; k < c1; k += a.nt) lq.v[SQ_CAP + k] = st.buf[1][k];
