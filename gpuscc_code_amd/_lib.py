@@ -109,6 +109,7 @@ SIGNATURES = {
     "scc_profile_copy": (_st, [_vp, ctypes.c_int64]),
     "scc_profile_levels": (_st, [_vp, _dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
     "scc_load_file": (_st, [ctypes.c_char_p, _gp]),
+    "scc_save_graph": (_st, [_gp, ctypes.c_char_p]),
     "scc_csr_from_arcs": (_st, [ctypes.c_int32, ctypes.c_int64, _i32p, _i32p, ctypes.c_int, _gp]),
     "scc_gen_rmat": (_st, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, ctypes.c_uint64, ctypes.c_int, _gp]),

@@ -30,7 +30,8 @@ static void usage() {
                 "\t   d for data-driven, l for load-balance, h for hybrid, c for combine,\n"
                 "\t   b for best (direction-optimising), a for all of them.\n"
                 "\t   topology-driven with load balance (t) in default if not specific.\n"
-                "\t-f input directed graph\n"
+                "\t-f input directed graph (edge list, MatrixMarket, or a binary graph written by -o)\n"
+                "\t-o also write the loaded graph as a binary graph file (loads without parsing text)\n"
                 "\t-j also print one JSON line per run\n"
                 "\t-V write trivial.txt, scc_dist.txt, parallel_scc.txt, state_range_root.txt\n");
 }
@@ -134,13 +135,13 @@ static int run_one(scc_ctx *ctx, int alg, int variant, const scc_host_graph &g, 
 
 int main(int argc, char **argv) {
     bool tarjan = false, json = false, all = false, verify = false;
-    const char *file = nullptr;
+    const char *file = nullptr, *save_to = nullptr;
     int alg = SCC_SCHED_T;  // the reference's default (scc.cu:80)
     int variant = SCC_BIN_VARIANT;
     scc_opts opts;
     std::memset(&opts, 0, sizeof(opts));
     int c;
-    while ((c = getopt(argc, argv, "htjVf:a:w:b:v:")) != -1) {
+    while ((c = getopt(argc, argv, "htjVf:a:w:b:v:o:")) != -1) {
         switch (c) {
         case 'h': break;  // header flag: accepted, unused (as in load.cpp)
         case 't': tarjan = true; break;
@@ -165,6 +166,7 @@ int main(int argc, char **argv) {
         case 'w': break;  // warp size: accepted, unused (never read by the reference either)
         case 'b': opts.block_size = (int32_t)std::strtol(optarg, nullptr, 0); break;
         case 'v': variant = std::atoi(optarg); break;
+        case 'o': save_to = optarg; break;  // binary CSR file for later runs (no reference counterpart)
         default: usage(); return 1;
         }
     }
@@ -189,6 +191,15 @@ int main(int argc, char **argv) {
                 g.min_out, g.average);
     std::printf("    ->nodes: %u\n", (unsigned)g.n);
     std::printf("    ->edges: %u\n", (unsigned)g.m_reported);
+    if (save_to) {
+        s = gpuscc::save_graph(&g, save_to, &err);
+        if (s != SCC_OK) {
+            std::fprintf(stderr, "error: %s\n", err.c_str());
+            gpuscc::graph_free(&g);
+            return 1;
+        }
+        std::printf("saved binary graph to %s\n", save_to);
+    }
     if (tarjan) {
         std::printf("tarjan decomposition...\n");
         gpuscc::TarjanResult r = gpuscc::tarjan_count(g.n, g.row, g.col);

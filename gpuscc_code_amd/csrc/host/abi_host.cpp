@@ -8,6 +8,10 @@ scc_status scc_load_file(const char *path, scc_host_graph *g) {
     return gpuscc::load_file(path, g, nullptr);
 }
 
+scc_status scc_save_graph(const scc_host_graph *g, const char *path) {
+    return gpuscc::save_graph(g, path, nullptr);
+}
+
 scc_status scc_csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t *dst,
                              int drop_self_loops, scc_host_graph *g) {
     return gpuscc::csr_from_arcs(n, m, src, dst, drop_self_loops, g);

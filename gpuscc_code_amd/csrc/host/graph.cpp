@@ -7,6 +7,7 @@
 // so scale-24 R-MAT (268M arcs) loads in seconds.
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -196,8 +197,247 @@ scc_status csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t
 }
 
 // -------------------------------------------------------------------------
+// binary CSR files: a loaded graph saved once (save_graph) loads again without
+// parsing text.  Little-endian layout: the 8-byte magic, int64 n, m,
+// m_reported, int32 flags (1 pattern, 2 symmetric, 4 mtx), num_graphs, max_in,
+// min_in, max_out, min_out, double average, then int32 row[n + 1], col[m] and,
+// for a batch, graph_offsets[num_graphs + 1].  CSR^T is rebuilt on load.
+// -------------------------------------------------------------------------
+static constexpr char kBinMagic[8] = {'G', 'P', 'U', 'S', 'C', 'S', 'R', '1'};
+static constexpr size_t kBinHeader = 64;
+
+scc_status save_graph(const scc_host_graph *g, const char *path, std::string *err) {
+    if (!g || !path || g->n < 0 || g->m < 0 || (g->n > 0 && !g->row) || (g->m > 0 && !g->col))
+        return SCC_E_INPUT;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        if (err) *err = std::string("cannot create ") + path;
+        return SCC_E_INPUT;
+    }
+    unsigned char h[kBinHeader];
+    std::memset(h, 0, sizeof(h));
+    std::memcpy(h, kBinMagic, 8);
+    const int64_t n64 = g->n, m64 = g->m, mr = g->m_reported;
+    const int32_t fl = (g->pattern ? 1 : 0) | (g->symmetric ? 2 : 0) | (g->mtx ? 4 : 0);
+    const int32_t ng = g->graph_offsets ? g->num_graphs : 0;
+    const int32_t st4[4] = {g->max_in, g->min_in, g->max_out, g->min_out};
+    std::memcpy(h + 8, &n64, 8);
+    std::memcpy(h + 16, &m64, 8);
+    std::memcpy(h + 24, &mr, 8);
+    std::memcpy(h + 32, &fl, 4);
+    std::memcpy(h + 36, &ng, 4);
+    std::memcpy(h + 40, st4, 16);
+    std::memcpy(h + 56, &g->average, 8);
+    bool ok = std::fwrite(h, 1, sizeof(h), f) == sizeof(h);
+    const int32_t zero = 0;
+    const int32_t *rowp = g->n > 0 ? g->row : &zero;
+    ok = ok && std::fwrite(rowp, sizeof(int32_t), (size_t)g->n + 1, f) == (size_t)g->n + 1;
+    if (g->m > 0) ok = ok && std::fwrite(g->col, sizeof(int32_t), (size_t)g->m, f) == (size_t)g->m;
+    if (ng > 0) ok = ok && std::fwrite(g->graph_offsets, sizeof(int32_t), (size_t)ng + 1, f) == (size_t)ng + 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok && err) *err = std::string("write failed: ") + path;
+    return ok ? SCC_OK : SCC_E_INPUT;
+}
+
+// Checks everything the library assumes of a CSR (offsets from 0 to m, never
+// decreasing; ids in range; rows strictly increasing), then rebuilds CSR^T.
+static scc_status load_binary(const std::string &buf, scc_host_graph *g, std::string *err) {
+    auto bad = [&](const char *msg) {
+        if (err) *err = msg;
+        return SCC_E_INPUT;
+    };
+    if (buf.size() < kBinHeader) return bad("binary graph: truncated header");
+    const unsigned char *h = (const unsigned char *)buf.data();
+    int64_t n64, m64, mr;
+    int32_t fl, ng, st4[4];
+    double avg;
+    std::memcpy(&n64, h + 8, 8);
+    std::memcpy(&m64, h + 16, 8);
+    std::memcpy(&mr, h + 24, 8);
+    std::memcpy(&fl, h + 32, 4);
+    std::memcpy(&ng, h + 36, 4);
+    std::memcpy(st4, h + 40, 16);
+    std::memcpy(&avg, h + 56, 8);
+    if (n64 < 0 || n64 >= INT32_MAX - 1 || m64 < 0 || m64 >= INT32_MAX || ng < 0 || ng > n64 + 1)
+        return bad("binary graph: sizes out of range");
+    const int32_t n = (int32_t)n64;
+    const size_t words = (size_t)n + 1 + (size_t)m64 + (ng > 0 ? (size_t)ng + 1 : 0);
+    if (buf.size() != kBinHeader + 4 * words) return bad("binary graph: file size does not match its header");
+    const int32_t *w = (const int32_t *)(buf.data() + kBinHeader);
+    std::vector<int32_t> row(w, w + n + 1), col(w + n + 1, w + n + 1 + m64);
+    if (row[0] != 0 || row[n] != m64) return bad("binary graph: row offsets do not span the arcs");
+    std::atomic<bool> fine{true};
+    parallel_for(n, [&](int64_t b, int64_t e, int) {
+        for (int64_t v = b; v < e && fine.load(std::memory_order_relaxed); v++) {
+            if (row[v + 1] < row[v]) { fine = false; break; }
+            for (int32_t i = row[v]; i < row[v + 1]; i++)
+                if (col[i] < 0 || col[i] >= n || (i > row[v] && col[i] <= col[i - 1])) { fine = false; break; }
+        }
+    });
+    if (!fine) return bad("binary graph: rows out of order, duplicated or out of range");
+    std::vector<int32_t> off;
+    if (ng > 0) {
+        off.assign(w + n + 1 + m64, w + n + 1 + m64 + ng + 1);
+        if (off[0] != 0 || off[ng] != n) return bad("binary graph: bad graph offsets");
+        for (int32_t i = 0; i < ng; i++)
+            if (off[i + 1] < off[i]) return bad("binary graph: bad graph offsets");
+    }
+    zero_graph(g);
+    g->pattern = fl & 1;
+    g->symmetric = (fl >> 1) & 1;
+    g->mtx = (fl >> 2) & 1;
+    g->m_reported = mr;
+    g->average = avg;
+    scc_status s = finish_graph(n, row, col, g);
+    if (s != SCC_OK) return s;
+    g->max_in = st4[0];
+    g->min_in = st4[1];
+    g->max_out = st4[2];
+    g->min_out = st4[3];
+    if (ng > 0) {
+        g->num_graphs = ng;
+        g->graph_offsets = to_malloc(off);
+        if (!g->graph_offsets) return SCC_E_OOM;
+    }
+    return SCC_OK;
+}
+
+// -------------------------------------------------------------------------
 // load.cpp-compatible parser (load.cpp:18-207)
 // -------------------------------------------------------------------------
+static inline bool is_space(char c) {
+    return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+}
+
+// One "%lld" of sscanf on [p, e): white space, an optional sign, at least one
+// digit (values beyond the long long range clamp, as strtoll does).  Returns
+// the position after the number, or null when there is none.
+static const char *scan_ll(const char *p, const char *e, long long &out) {
+    while (p < e && is_space(*p)) p++;
+    bool neg = false;
+    if (p < e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+    if (p >= e || *p < '0' || *p > '9') return nullptr;
+    unsigned long long v = 0;
+    bool big = false;
+    for (; p < e && *p >= '0' && *p <= '9'; p++) {
+        if (v > (ULLONG_MAX - 9) / 10) big = true;
+        else v = v * 10 + (unsigned long long)(*p - '0');
+    }
+    if (big || v > (unsigned long long)LLONG_MAX) out = neg ? LLONG_MIN : LLONG_MAX;
+    else out = neg ? -(long long)v : (long long)v;
+    return p;
+}
+
+// The nnz entry lines that follow the header (load.cpp:70-92): line "a b" is
+// the arc b -> a (1-based ids in [1, n]); with symmetric, a -> b as well unless
+// a == b (counted into *reported, load.cpp:83-89).  The text from pos on is cut
+// into chunks at line ends and parsed in parallel: a first pass counts each
+// chunk's lines, so every line has its global index, and the first failing line
+// (the one a sequential reader meets first) gives the error.
+static scc_status parse_entries(const std::string &buf, size_t pos, int32_t n, int64_t nnz, bool symmetric,
+                                std::vector<int32_t> &src, std::vector<int32_t> &dst, int64_t &reported,
+                                std::string &msg) {
+    const char *base = buf.data();
+    const size_t beg = std::min(pos, buf.size()), end = buf.size();
+    const size_t bytes = end - beg;
+    const int nc = (int)std::max<size_t>(1, std::min<size_t>((size_t)host_threads(), bytes / 4096));
+    std::vector<size_t> cb(nc + 1);
+    cb[0] = beg;
+    cb[nc] = end;
+    for (int k = 1; k < nc; k++) {  // each cut just after a line end
+        size_t x = std::max(beg + bytes * (size_t)k / (size_t)nc, cb[k - 1]);
+        const void *nl = x < end ? std::memchr(base + x, '\n', end - x) : nullptr;
+        cb[k] = nl ? (size_t)((const char *)nl - base) + 1 : end;
+    }
+    auto run = [&](auto &&fn) {  // fn(chunk) for every chunk, in parallel
+        std::vector<std::thread> th;
+        for (int k = 1; k < nc; k++) th.emplace_back([&, k] { fn(k); });
+        fn(0);
+        for (auto &t : th) t.join();
+    };
+    // pass 1: lines per chunk (a last line without its line end counts when non-empty)
+    std::vector<int64_t> lines(nc + 1, 0);
+    run([&](int k) {
+        int64_t c = 0;
+        for (const char *p = base + cb[k], *e = base + cb[k + 1]; p < e;) {
+            const void *nl = std::memchr(p, '\n', (size_t)(e - p));
+            c++;
+            p = nl ? (const char *)nl + 1 : e;
+        }
+        lines[k + 1] = c;
+    });
+    for (int k = 0; k < nc; k++) lines[k + 1] += lines[k];  // lines[k]: index of chunk k's first line
+    // pass 2: the entries; chunk k keeps its arcs and its first failing line
+    std::vector<std::vector<int32_t>> cs(nc), cd(nc);
+    std::vector<int64_t> err_line(nc, INT64_MAX), mirrored(nc, 0);
+    std::vector<int> err_kind(nc, 0);
+    run([&](int k) {
+        int64_t li = lines[k];
+        if (li >= nnz) return;
+        const int64_t cap = std::min<int64_t>(nnz, lines[k + 1]) - li;
+        cs[k].reserve((size_t)cap * (symmetric ? 2 : 1));
+        cd[k].reserve((size_t)cap * (symmetric ? 2 : 1));
+        for (const char *p = base + cb[k], *e = base + cb[k + 1]; p < e && li < nnz; li++) {
+            const void *nl = std::memchr(p, '\n', (size_t)(e - p));
+            const char *le = nl ? (const char *)nl : e;
+            long long a = 0, b = 0;
+            const char *q = scan_ll(p, le, a);
+            if (q) q = scan_ll(q, le, b);
+            if (!q) {
+                err_line[k] = li;
+                err_kind[k] = 1;
+                return;
+            }
+            if (a < 1 || a > n || b < 1 || b > n) {
+                err_line[k] = li;
+                err_kind[k] = 2;
+                return;
+            }
+            const int32_t d = (int32_t)(a - 1), s = (int32_t)(b - 1);
+            cs[k].push_back(s);
+            cd[k].push_back(d);
+            if (symmetric && d != s) {
+                cs[k].push_back(d);
+                cd[k].push_back(s);
+                mirrored[k]++;
+            }
+            p = nl ? le + 1 : e;
+        }
+    });
+    int64_t first = INT64_MAX;
+    int kind = 0;
+    for (int k = 0; k < nc; k++)
+        if (err_line[k] < first) {
+            first = err_line[k];
+            kind = err_kind[k];
+        }
+    if (first == INT64_MAX && lines[nc] < nnz) {
+        msg = "file ends before nnz entries";
+        return SCC_E_INPUT;
+    }
+    if (kind == 1) {
+        msg = "bad entry line " + std::to_string(first + 1);
+        return SCC_E_INPUT;
+    }
+    if (kind == 2) {
+        msg = "vertex id out of range on entry line " + std::to_string(first + 1);
+        return SCC_E_INPUT;
+    }
+    std::vector<size_t> off(nc + 1, 0);
+    for (int k = 0; k < nc; k++) {
+        off[k + 1] = off[k] + cs[k].size();
+        reported += mirrored[k];
+    }
+    src.resize(off[nc]);
+    dst.resize(off[nc]);
+    run([&](int k) {
+        if (!cs[k].empty()) {
+            std::memcpy(src.data() + off[k], cs[k].data(), sizeof(int32_t) * cs[k].size());
+            std::memcpy(dst.data() + off[k], cd[k].data(), sizeof(int32_t) * cd[k].size());
+        }
+    });
+    return SCC_OK;
+}
 scc_status load_file(const char *path, scc_host_graph *g, std::string *err) {
     if (!g || !path) return SCC_E_INPUT;
     zero_graph(g);
@@ -224,6 +464,8 @@ scc_status load_file(const char *path, scc_host_graph *g, std::string *err) {
         pos = e + 1;
         return true;
     };
+    if (buf.size() >= sizeof(kBinMagic) && std::memcmp(buf.data(), kBinMagic, sizeof(kBinMagic)) == 0)
+        return load_binary(buf, g, err);
     std::string line;
     if (!next_line(line)) { return bad("empty file", SCC_E_INPUT); }
     // load.cpp:30-33: the first line carries the MatrixMarket qualifiers
@@ -248,26 +490,11 @@ scc_status load_file(const char *path, scc_host_graph *g, std::string *err) {
     }
     int32_t n = (int32_t)m_rows;
     std::vector<int32_t> src, dst;
-    src.reserve((size_t)nnz * (symmetric ? 2 : 1));
-    dst.reserve((size_t)nnz * (symmetric ? 2 : 1));
     int64_t reported = nnz;
-    for (long long i = 0; i < nnz; i++) {
-        if (!next_line(line)) { return bad("file ends before nnz entries", SCC_E_INPUT); }
-        long long a, b;
-        if (std::sscanf(line.c_str(), "%lld %lld", &a, &b) != 2) {
-            return bad("bad entry line " + std::to_string(i + 1), SCC_E_INPUT);
-        }
-        if (a < 1 || a > n || b < 1 || b > n) {
-            return bad("vertex id out of range on entry line " + std::to_string(i + 1), SCC_E_INPUT);
-        }
-        // load.cpp:79-92: the line "dst src" is the arc src -> dst
-        int32_t d = (int32_t)(a - 1), s = (int32_t)(b - 1);
-        if (symmetric) {
-            if (d != s) { reported++; src.push_back(d); dst.push_back(s); }
-            src.push_back(s); dst.push_back(d);
-        } else {
-            src.push_back(s); dst.push_back(d);
-        }
+    {
+        std::string msg;
+        scc_status es = parse_entries(buf, pos, n, (int64_t)nnz, symmetric, src, dst, reported, msg);
+        if (es != SCC_OK) return bad(msg, es);
     }
     std::vector<int32_t> row, col;
     scc_status st = bucket_rows(n, (int64_t)src.size(), src.data(), dst.data(), 0, row, col);

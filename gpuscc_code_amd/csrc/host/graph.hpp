@@ -13,6 +13,7 @@ uint64_t rng_draw(uint64_t seed, uint64_t stream, uint64_t j);
 scc_status csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t *dst,
                          int drop_self, scc_host_graph *g);
 scc_status load_file(const char *path, scc_host_graph *g, std::string *err);
+scc_status save_graph(const scc_host_graph *g, const char *path, std::string *err);
 scc_status gen_rmat(int scale, int ef, double a, double b, double c, uint64_t seed, int permute,
                     scc_host_graph *g);
 scc_status gen_mesh(int side, double p, uint64_t seed, scc_host_graph *g);

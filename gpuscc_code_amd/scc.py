@@ -102,6 +102,27 @@ def load(path) -> Graph:
     return _from_host(g)
 
 
+def save(g: Graph, path):
+    """Writes g as a binary CSR file (scc_save_graph) that load() reads without parsing text."""
+    row = np.ascontiguousarray(g.row, np.int32)
+    col = np.ascontiguousarray(g.col, np.int32)
+    info = g.info or {}
+    h = SccHostGraph()
+    h.n, h.m = g.n, len(col)
+    h.row = row.ctypes.data_as(_lib._i32p)
+    h.col = col.ctypes.data_as(_lib._i32p) if len(col) else None
+    h.m_reported = g.m_reported or len(col)
+    for k in ("pattern", "symmetric", "mtx", "max_in", "min_in", "max_out", "min_out"):
+        setattr(h, k, int(info.get(k, 0)))
+    h.average = float(info.get("average", 0.0))
+    off = None
+    if g.graph_offsets is not None and len(g.graph_offsets) > 1:
+        off = np.ascontiguousarray(g.graph_offsets, np.int32)
+        h.num_graphs = len(off) - 1
+        h.graph_offsets = off.ctypes.data_as(_lib._i32p)
+    check(lib().scc_save_graph(ctypes.byref(h), str(path).encode()))
+
+
 def csr_from_arcs(n, src, dst, drop_self_loops=True) -> Graph:
     src = np.ascontiguousarray(src, np.int32)
     dst = np.ascontiguousarray(dst, np.int32)

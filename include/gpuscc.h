@@ -250,8 +250,14 @@ scc_status scc_profile_levels(scc_ctx *ctx, double *out, int64_t cap, int64_t *r
 
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
- * "a b" is the arc b->a; self-loops kept, duplicates dropped. */
+ * "a b" is the arc b->a; self-loops kept, duplicates dropped.  Entry lines are
+ * parsed in parallel.  A file written by scc_save_graph is read as binary CSR
+ * (recognised by its magic bytes) and checked, with CSR^T rebuilt. */
 scc_status scc_load_file(const char *path, scc_host_graph *g);
+/* Writes g (CSR, statistics, batch offsets) as a binary CSR file that
+ * scc_load_file reads without parsing text.  No reference counterpart: it
+ * serves repeated runs on graphs whose text takes long to parse. */
+scc_status scc_save_graph(const scc_host_graph *g, const char *path);
 /* Builds CSR + CSR^T from arcs src[i]->dst[i] (0-based): rows sorted,
  * duplicates removed, self-loops dropped when drop_self_loops != 0. */
 scc_status scc_csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t *dst,
