@@ -78,3 +78,76 @@ def test_gpu_verify_files(tmp_path):
     detail = (tmp_path / "state_range_root.txt").read_text().splitlines()
     assert detail[0].startswith("ID") and len(detail) == 14
     assert "number=6" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# the log tabulator (process_rst.cpp:7-41) and its batch script (runall)
+# ---------------------------------------------------------------------------
+REF_TAB = os.path.join(ROOT, "oracle", "_ref", "process_rst")
+
+
+def tabulate(tool, log_text, tmp_path, name):
+    log = tmp_path / f"{name}.log"
+    log.write_text(log_text)
+    out = tmp_path / f"{name}.tab"
+    r = subprocess.run([tool, str(log), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return out.read_text()
+
+
+def run_log(path, times, binary="./scc"):
+    lines = [f"{binary} -t -a a -f {path}", "loading graph...", "    ->nodes: 13", "    ->edges: 19",
+             "tarjan decomposition..."]
+    for t in times:
+        lines += ["    ->count of SCC: 6", f"    ->time: {t}"]
+    return "\n".join(lines) + "\n"
+
+
+def test_process_rst_rows():
+    """Seven "->time:" values per row after the graph's file name (the sixth field
+    of the command line), written as the reference's ostream prints floats."""
+    import tempfile
+    from pathlib import Path
+    with tempfile.TemporaryDirectory() as d:
+        log = run_log("../scc_dataset/cage13.mtx", [0.5, 0.25, 1.0, 13.633, 0.000012, 2.5, 100.0]) + \
+            run_log("../scc_dataset/web.txt", [1, 2, 3, 4, 5, 6, 7])
+        got = tabulate(os.path.join(BIN, "process_rst"), log, Path(d), "a")
+    assert got == ("../scc_dataset/cage13.mtx 0.5 0.25 1 13.633 1.2e-05 2.5 100 \n"
+                   "../scc_dataset/web.txt 1 2 3 4 5 6 7 \n")
+
+
+def test_process_rst_matches_reference_tabulator(tmp_path):
+    if not os.path.exists(REF_TAB):
+        pytest.skip("reference tabulator not built (needs /root/reference)")
+    rng = __import__("numpy").random.default_rng(3)
+    logs = {
+        "aligned": "".join(run_log(f"d/g{k}.txt", rng.random(7).round(6).tolist()) for k in range(4)),
+        # -t -a a prints eight times per graph (Tarjan and seven strategies): rows drift, as in the reference
+        "eight": "".join(run_log(f"d/g{k}.mtx", rng.random(8).round(5).tolist()) for k in range(5)),
+        "odd": "./scc -f short\n    ->time: 1.5\n    ->time: oops\ntime: 2\n" + run_log("x y", [3] * 6, "./scc_w") +
+               "no match here\n",
+        "empty": "",
+    }
+    for name, text in logs.items():
+        assert tabulate(os.path.join(BIN, "process_rst"), text, tmp_path, "b_" + name) == \
+            tabulate(REF_TAB, text, tmp_path, "r_" + name), name
+
+
+def test_process_rst_errors(tmp_path):
+    r = subprocess.run([os.path.join(BIN, "process_rst")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "usage" in r.stderr
+    r = subprocess.run([os.path.join(BIN, "process_rst"), "/nonexistent.log", str(tmp_path / "o")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "cannot open" in r.stderr
+
+
+def test_runall_prints_each_command_line(tmp_path):
+    """runall DIR runs scc and scc_w on every file with -t -a a; the Tarjan part
+    runs on the CPU, so its lines appear even where the GPU part cannot run."""
+    d = tmp_path / "ds"
+    d.mkdir()
+    (d / "demo.graph").write_text(open(DEMO).read())
+    r = subprocess.run([os.path.join(BIN, "runall"), str(d)], capture_output=True, text=True, timeout=120)
+    out = r.stdout
+    assert f"./scc -t -a a -f {d}/demo.graph" in out and f"./scc_w -t -a a -f {d}/demo.graph" in out
+    assert out.count("    ->count of SCC: 6") >= 2
