@@ -44,45 +44,71 @@ constexpr int INIT_V = 8;
 // With hlive given, the live vertices whose out- or in-row does not start with
 // another vertex that stays live (hlive, k_head_live) are appended to unk (count
 // in ctl->qsize[0][0]): the first level of trim1 checks only those, since every
-// other live vertex has a live same-colour neighbour both ways.
+// other live vertex has a live same-colour neighbour both ways.  VEC = 4 (n a
+// multiple of 4): a work-item takes four consecutive vertices at a time, their
+// class bytes in one 4-byte load each and their words and roots in one 16-byte
+// store each.
+template <int VEC>
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
                        const uint8_t *cls_out, const uint8_t *cls_in, const uint8_t *hlive, int32_t *unk,
                        Ctl *ctl) {
+    constexpr int R = INIT_V / VEC;  // groups of VEC vertices per work-item and round
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t shw[16];
     __shared__ uint32_t sbase;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(n, beg, end);  // beg is a multiple of 64, end of 4 when n is
     const uint32_t bs = blockDim.x;
     unsigned long long nt = 0;
     uint32_t g = 0;  // batch: graph of the current vertex
-    if (off && beg + threadIdx.x < end) g = graph_of(off, G, (int32_t)(beg + threadIdx.x));
+    if (off && beg + (int64_t)threadIdx.x * VEC < end) g = graph_of(off, G, (int32_t)(beg + threadIdx.x * VEC));
     for (int64_t base = beg; base < end; base += (int64_t)bs * INIT_V) {  // block-uniform
-        uint32_t co[INIT_V], ci[INIT_V], ho[INIT_V], hi[INIT_V];
+        uint32_t co[R], ci[R], ho[R], hi[R];  // VEC bytes each
 #pragma unroll
-        for (int k = 0; k < INIT_V; k++) {
-            const int64_t v = base + (int64_t)k * bs + threadIdx.x;
-            const bool in = cls_out && v < end;
-            co[k] = in ? cls_out[v] : 0u;
-            ci[k] = in ? cls_in[v] : 0u;
-            ho[k] = in && hlive ? hlive[v] : 0u;
-            hi[k] = in && hlive ? hlive[(int64_t)n + v] : 0u;
+        for (int q = 0; q < R; q++) {
+            const int64_t v0 = base + ((int64_t)q * bs + threadIdx.x) * VEC;
+            const bool in = cls_out && v0 < end;
+            if (VEC == 4) {
+                co[q] = in ? *reinterpret_cast<const uint32_t *>(cls_out + v0) : 0u;
+                ci[q] = in ? *reinterpret_cast<const uint32_t *>(cls_in + v0) : 0u;
+                ho[q] = in && hlive ? *reinterpret_cast<const uint32_t *>(hlive + v0) : 0u;
+                hi[q] = in && hlive ? *reinterpret_cast<const uint32_t *>(hlive + (int64_t)n + v0) : 0u;
+            } else {
+                co[q] = in ? cls_out[v0] : 0u;
+                ci[q] = in ? cls_in[v0] : 0u;
+                ho[q] = in && hlive ? hlive[v0] : 0u;
+                hi[q] = in && hlive ? hlive[(int64_t)n + v0] : 0u;
+            }
         }
         uint32_t um = 0;  // live vertices of this round the first trim level must check
 #pragma unroll
-        for (int k = 0; k < INIT_V; k++) {
-            const int64_t v = base + (int64_t)k * bs + threadIdx.x;
-            if (v >= end) continue;
-            uint32_t c = 1u;
-            if (off) {
-                while (off[g + 1] <= v) g++;
-                c = (g + 1u) & COLOUR_MASK;
+        for (int q = 0; q < R; q++) {
+            const int64_t v0 = base + ((int64_t)q * bs + threadIdx.x) * VEC;
+            if (v0 >= end) continue;
+            uint32_t w[VEC];
+#pragma unroll
+            for (int j = 0; j < VEC; j++) {
+                const int64_t v = v0 + j;
+                uint32_t c = 1u;
+                if (off) {
+                    while (off[g + 1] <= v) g++;
+                    c = (g + 1u) & COLOUR_MASK;
+                }
+                const uint32_t bo = (co[q] >> (8 * j)) & 0xFFu, bi = (ci[q] >> (8 * j)) & 0xFFu;
+                const bool trim = cls_out && !(row_has_other(bo) && row_has_other(bi));
+                w[j] = make_word(c, trim ? 28u : 0u);
+                nt += trim ? 1u : 0u;
+                if (!trim && !(((ho[q] >> (8 * j)) & 0xFFu) && ((hi[q] >> (8 * j)) & 0xFFu)))
+                    um |= 1u << (q * VEC + j);
             }
-            const bool trim = cls_out && !(row_has_other(co[k]) && row_has_other(ci[k]));
-            word[v] = make_word(c, trim ? 28u : 0u);
-            root[v] = (int32_t)v;
-            nt += trim ? 1u : 0u;
-            if (!trim && !(ho[k] && hi[k])) um |= 1u << k;
+            if (VEC == 4) {
+                *reinterpret_cast<uint4 *>(word + v0) = make_uint4(w[0], w[1 % VEC], w[2 % VEC], w[3 % VEC]);
+                *reinterpret_cast<int4 *>(root + v0) =
+                    make_int4((int32_t)v0, (int32_t)v0 + 1, (int32_t)v0 + 2, (int32_t)v0 + 3);
+            } else {
+                word[v0] = w[0];
+                root[v0] = (int32_t)v0;
+            }
         }
         if (hlive) {  // one prefix sum and one atomicAdd per round and workgroup
             uint32_t total;
@@ -91,8 +117,10 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
             if (threadIdx.x == 0) sbase = atomicAdd(&ctl->qsize[0][0], total);
             __syncthreads();
             uint32_t o = sbase + pre;
-            for (uint32_t m = um; m; m &= m - 1)
-                unk[o++] = (int32_t)(base + (int64_t)(__ffs(m) - 1) * bs + threadIdx.x);
+            for (uint32_t m = um; m; m &= m - 1) {
+                const int b = __ffs(m) - 1;
+                unk[o++] = (int32_t)(base + ((int64_t)(b / VEC) * bs + threadIdx.x) * VEC + b % VEC);
+            }
             __syncthreads();  // sbase is rewritten by the next round
         }
     }
@@ -650,8 +678,12 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl) {
-    hipLaunchKernelGGL(k_init, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr, cls ? hlive : nullptr,
-                       unk, ctl);
+    if ((n & 3) == 0)
+        hipLaunchKernelGGL(k_init<4>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
+                           cls ? hlive : nullptr, unk, ctl);
+    else
+        hipLaunchKernelGGL(k_init<1>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
+                           cls ? hlive : nullptr, unk, ctl);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
