@@ -42,6 +42,9 @@ struct scc_ctx {
     int32_t *d_head = nullptr;    // first entry of each row: out-rows [0, n), in-rows [n, 2n)
     int4 *d_ell = nullptr;        // first four entries of each row, same halves
     uint8_t *d_hlive = nullptr;   // first entry of each row left live by k_init, same halves
+    int32_t *d_live[2] = {nullptr, nullptr};  // live-vertex lists (written by elections, ctl->lcnt)
+    bool use_lists = false;       // scc_run with the plain rule: elections keep a live-vertex list
+    int lsel = -1, lsel_prev = -1;  // list holding the live vertices (-1: none yet), before the last election
     uint32_t trim_tag = 0;
     unsigned long long *d_slots[2] = {nullptr, nullptr};
     int slot_bits = 10;
@@ -296,6 +299,9 @@ struct TrimChain {
         a.iter = it;
         a.tag = c->trim_tag;
         a.init_list = after_init && it == 0 ? 1 : 0;
+        // sweeps go over the live-vertex list of the last election, when there is one
+        a.live = c->use_lists && c->lsel >= 0 ? c->d_live[c->lsel] : nullptr;
+        a.live_cnt = a.live ? &c->d_ctl->lcnt[c->lsel] : nullptr;
         a.inq[0] = c->d_q[(it - 1) & 1];
         a.outq[0] = c->d_q[it & 1];
         a.inq[1] = a.inq[0];
@@ -447,7 +453,9 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     int64_t nslots = (int64_t)1 << c->slot_bits;
     if ((c->epoch & 0xFFFFu) < 2u) launch_fill_u64(l, cur, nslots, ~0ull);  // tag wrap
-    ctl_op(c, 4);
+    const bool lists = c->use_lists && !learned;
+    const int lin_sel = c->lsel, lout_sel = lin_sel < 0 ? 0 : lin_sel ^ 1;
+    ctl_op(c, 4 | (lists ? (lout_sel ? 4096 : 2048) : 0));  // with the output list's count zeroed
     if (learned) {
         if (recolour)
             launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits,
@@ -455,11 +463,25 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         CHK(do_features(c, true, nullptr));
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       0, 1, c->d_score, c->d_ctl);
+    } else if (lists) {
+        // the update reads the list of the election before (all n the first time)
+        // and lists the vertices still live; the confirm pass reads that list
+        const int32_t *lin = lin_sel >= 0 ? c->d_live[lin_sel] : nullptr;
+        const uint32_t *lin_cnt = lin_sel >= 0 ? &c->d_ctl->lcnt[lin_sel] : nullptr;
+        launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
+                      recolour, 1, nullptr, c->d_ctl, lin, lin_cnt, c->d_live[lout_sel],
+                      &c->d_ctl->lcnt[lout_sel]);
+        c->lsel_prev = lin_sel;
+        c->lsel = lout_sel;
     } else {
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       recolour, 1, nullptr, c->d_ctl);
     }
-    launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
+    if (lists)
+        launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl, c->d_live[lout_sel],
+                       &c->d_ctl->lcnt[lout_sel]);
+    else
+        launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
     ctl_queue(c, 64);  // the pivot list seeds both directions
     c->prev_tab ^= 1;
     if (!sync) return SCC_OK;
@@ -543,6 +565,11 @@ void free_graph(scc_ctx *c) {
     c->d_ell = nullptr;
     if (c->d_hlive) (void)hipFree(c->d_hlive);
     c->d_hlive = nullptr;
+    for (auto &p : c->d_live) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    c->lsel = c->lsel_prev = -1;
     for (auto &t : c->d_slots) {
         if (t) (void)hipFree(t);
         t = nullptr;
@@ -774,6 +801,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_head, 2 * vb));
     HIPCHK(c, hipMalloc(&c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMalloc(&c->d_hlive, 2 * std::max<size_t>((size_t)n, 1)));
+    for (auto &p : c->d_live) HIPCHK(c, hipMalloc(&p, vb));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -839,6 +867,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     const int32_t n = c->g.n;
     c->syncs = 0;
     c->cur_step = 0;
+    c->use_lists = !learned;  // elections of this run keep the live-vertex list (none yet)
+    c->lsel = c->lsel_prev = -1;
     if (c->label_in_root) c->labels_valid = false;  // the run rewrites the root array
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t_start = std::chrono::steady_clock::now();
@@ -943,6 +973,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 } else {  // undo the election's bookkeeping and finish the step with waits
                     c->epoch--;
                     c->prev_tab ^= 1;
+                    c->lsel = c->lsel_prev;  // its update did not run: the list before it holds
                     ctl_op(c, 512);
                     if (h.abort == 1) CHK(do_reach(c, 3, sched, c->d_piv, &dr, Lr));
                     st.ms_fwd += ms_since(tp);
@@ -1025,6 +1056,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     };
     s = body();
     c->one_colour = false;
+    c->use_lists = false;  // phase entry points work on states of their own
+    c->lsel = c->lsel_prev = -1;
     if (s != SCC_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_total = ms_since(t_start);

@@ -58,6 +58,7 @@ struct Ctl {
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
     uint32_t st_maxscan[4];  // instrumented: longest serial row scan (arcs) of one work-item,
                              // ring: level iter writes [iter & 3]
+    uint32_t lcnt[2];        // lengths of the two live-vertex lists (an update reads one, writes the other)
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

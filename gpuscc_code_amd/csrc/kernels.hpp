@@ -64,6 +64,8 @@ struct ReachArgs {
     int skip_fb;        // right after a complete reach from pivots: F and B without P means an SCC of
                         // two or more vertices, which no trim removes, so those vertices are not checked
     int init_list;      // the first level of trim1: check only the vertices k_init listed in inq[0]
+    const int32_t *live;        // trim sweeps: only the vertices live[0 .. *live_cnt) (null: all n)
+    const uint32_t *live_cnt;
 };
 
 // Local rounds start at this level in the first FB step: its first levels are
@@ -107,12 +109,16 @@ void launch_head_live(const Launch &l, const uint8_t *cls, const int32_t *head, 
 // last + 1 levels launched without a wait: its depth, and abort = stage unless it converged);
 // with publish, the block is then copied to that (pinned host) mirror
 void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int last = 0, int stage = 0, Ctl *publish = nullptr);
+// ctl (may be null): FB eliminations leave ctl->live.  lin (may be null): process only the
+// vertices lin[0 .. *lin_cnt); lout (may be null): append the vertices live afterwards
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,
-                   Ctl *ctl);  // ctl (may be null): FB eliminations leave ctl->live
+                   Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr,
+                   int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr);
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
-                    int slot_bits, int32_t *piv, Ctl *ctl);
+                    int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin = nullptr,
+                    const uint32_t *lin_cnt = nullptr);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl);
 void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl);

@@ -258,6 +258,8 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
         ctl->npiv = 0;
         ctl->first_piv = 0xFFFFFFFFu;
     }
+    if (what & 2048) ctl->lcnt[0] = 0;  // live-vertex list 0 is about to be rebuilt
+    if (what & 4096) ctl->lcnt[1] = 0;
     if (what & 8) ctl->count = 0;
     if (what & 16) ctl->maxc = 0;
     if (what & 32) {
@@ -287,16 +289,39 @@ __global__ void k_ctl_reset(Ctl *ctl, int what, int guard_last, int guard_stage,
 // ---------------------------------------------------------------------------
 constexpr int UPD_V = 8;  // vertices per work-item per round: their loads are issued together
 
+// Live-vertex lists: with lin, the vertices processed are lin[0 .. *lin_cnt)
+// instead of all of [0, n); with lout, the vertices still live after the
+// update are appended to lout (count *lout_cnt, zeroed before).  After the
+// first FB step few vertices stay live (a mesh keeps 0.4 %), and each later
+// step's update, confirm and first trim level then cost that many, not n.
+template <int K>
+__device__ inline void list_append(uint32_t keep, const int32_t (&vid)[K], int32_t *out, uint32_t *out_cnt,
+                                   uint32_t *shw, uint32_t &sbase) {
+    uint32_t total;  // all work-items call this (two barriers)
+    const uint32_t pre = block_excl_scan((uint32_t)__popc(keep), shw, total);
+    if (total == 0) return;  // block-uniform
+    if (threadIdx.x == 0) sbase = atomicAdd(out_cnt, total);
+    __syncthreads();
+    uint32_t o = sbase + pre;
+#pragma unroll
+    for (int k = 0; k < K; k++)  // static indices keep vid in registers
+        if ((keep >> k) & 1u) out[o++] = vid[k];
+    __syncthreads();  // sbase is rewritten by the next call
+}
+
 __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          const unsigned long long *__restrict__ prev, unsigned long long *cur,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
-                         const uint16_t *__restrict__ score, Ctl *ctl) {
+                         const uint16_t *__restrict__ score, Ctl *ctl, const int32_t *lin,
+                         const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt) {
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t wslot[16];
     __shared__ unsigned long long wmin[16];
+    __shared__ uint32_t shw[16];
+    __shared__ uint32_t sbase;
     if (ctl && __atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
     const int lane = threadIdx.x & (WAVE - 1);
     unsigned long long gone = 0;  // F and B: eliminated into the pivot's SCC
     // wave-uniform pending minimum of one slot, carried across rounds
@@ -304,19 +329,22 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
     unsigned long long pmin = ~0ull;
     for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * UPD_V) {  // block-uniform
         uint32_t w[UPD_V];
+        int32_t vid[UPD_V];
 #pragma unroll
         for (int k = 0; k < UPD_V; k++) {
-            int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
-            w[k] = vv < end ? word[vv] : ST_E;
+            int64_t i = base + (int64_t)k * blockDim.x + threadIdx.x;
+            vid[k] = i < end ? (lin ? lin[i] : (int32_t)i) : -1;
         }
-        uint32_t cmask = 0, sl[UPD_V];
+#pragma unroll
+        for (int k = 0; k < UPD_V; k++) w[k] = vid[k] >= 0 ? word[vid[k]] : ST_E;
+        uint32_t cmask = 0, keep = 0, sl[UPD_V];
         unsigned long long key[UPD_V];
 #pragma unroll
         for (int k = 0; k < UPD_V; k++) {
             sl[k] = 0;
             key[k] = ~0ull;
             if (w[k] & ST_E) continue;
-            int32_t v = (int32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+            int32_t v = vid[k];
             uint32_t c = colour_of(w[k]);
             if (recolour) {
                 uint32_t f = ((w[k] & ST_F) ? 1u : 0u) + ((w[k] & ST_B) ? 2u : 0u);
@@ -330,6 +358,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                 c = (3u * c + f) & COLOUR_MASK;
                 word[v] = make_word(c, 0);
             }
+            keep |= 1u << k;
             if (elect) {
                 cmask |= 1u << k;
                 sl[k] = slot_of(c, slot_bits);
@@ -337,6 +366,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                 key[k] = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
             }
         }
+        if (lout) list_append(keep, vid, lout, lout_cnt, shw, sbase);
         // When every candidate of the wave targets one slot (the first election
         // sends all live vertices there), their minimum joins the wave's
         // pending minimum (combined over the workgroup at the end: one
@@ -406,24 +436,27 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
 constexpr int CONF_V = 16;  // vertices per work-item per round
 
 __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *__restrict__ cur,
-                          int slot_bits, int32_t *piv, Ctl *ctl) {
+                          int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
     __shared__ uint32_t shw[16];
     __shared__ uint32_t sbase;
     if (__atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
     for (int64_t base = beg; base < end; base += (int64_t)blockDim.x * CONF_V) {  // block-uniform
         uint32_t w[CONF_V];
+        int32_t vid[CONF_V];
 #pragma unroll
         for (int k = 0; k < CONF_V; k++) {
-            int64_t vv = base + (int64_t)k * blockDim.x + threadIdx.x;
-            w[k] = vv < end ? word[vv] : ST_E;
+            int64_t i = base + (int64_t)k * blockDim.x + threadIdx.x;
+            vid[k] = i < end ? (lin ? lin[i] : (int32_t)i) : -1;
         }
+#pragma unroll
+        for (int k = 0; k < CONF_V; k++) w[k] = vid[k] >= 0 ? word[vid[k]] : ST_E;
         uint32_t won = 0;
 #pragma unroll
         for (int k = 0; k < CONF_V; k++) {
             if (w[k] & ST_E) continue;
-            uint32_t v = (uint32_t)(base + (int64_t)k * blockDim.x + threadIdx.x);
+            uint32_t v = (uint32_t)vid[k];
             uint32_t c = colour_of(w[k]);
             if ((uint32_t)cur[slot_of(c, slot_bits)] == v) {
                 word[v] = make_word(c, 19);
@@ -441,8 +474,9 @@ __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *_
         }
         __syncthreads();
         uint32_t o = sbase + pre;
-        for (uint32_t m = won; m; m &= m - 1)
-            piv[o++] = (int32_t)(base + (int64_t)(__ffs(m) - 1) * blockDim.x + threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < CONF_V; k++)
+            if ((won >> k) & 1u) piv[o++] = vid[k];
         __syncthreads();  // sbase is rewritten by the next round
     }
 }
@@ -709,13 +743,14 @@ void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int gua
 }
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
-                   unsigned long long tag, int recolour, int elect, const uint16_t *score, Ctl *ctl) {
+                   unsigned long long tag, int recolour, int elect, const uint16_t *score, Ctl *ctl,
+                   const int32_t *lin, const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt) {
     hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, elect,
-                       score, ctl);
+                       score, ctl, lin, lin_cnt, lout, lout_cnt);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
-                    int slot_bits, int32_t *piv, Ctl *ctl) {
-    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl);
+                    int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
+    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl, lin, lin_cnt);
 }
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl) {
     hipLaunchKernelGGL(k_count_state, GRID(l), word, n, what, ctl);

@@ -806,7 +806,8 @@ constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
 // round.  All work-items call this.
 __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint32_t (&claimed)[2],
                                    const unsigned long long (&arcs)[2], uint32_t *shw, uint32_t *sbase,
-                                   unsigned long long *sh, bool trims = false, uint32_t newp = 0) {
+                                   unsigned long long *sh, bool trims = false, uint32_t newp = 0,
+                                   const int32_t *lst = nullptr) {
     const uint32_t c0 = __popc(claimed[0]), c1 = __popc(claimed[1]);
     uint32_t total;  // packed counts: a workgroup claims at most 16 * 1024 < 2^16 per direction
     const uint32_t pre = block_excl_scan(c0 | (c1 << 16), shw, total, a.nt);
@@ -827,8 +828,8 @@ __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint
 #pragma unroll
     for (int d = 0; d < 2; d++) {
         for (uint32_t m = claimed[d]; m; m &= m - 1) {
-            int k = __ffs(m) - 1;
-            a.outq[d][o[d]++] = (int32_t)(base + (int64_t)k * a.nt + threadIdx.x);
+            const int64_t i = base + (int64_t)(__ffs(m) - 1) * a.nt + threadIdx.x;
+            a.outq[d][o[d]++] = lst ? lst[i] : (int32_t)i;  // lst: bits index a vertex list
         }
     }
     __syncthreads();  // sbase is rewritten by the next round
@@ -1007,8 +1008,8 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
     uint32_t cand = 0;
 #pragma unroll
     for (int k = 0; k < TRIM_V; k++) {
-        int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
-        uint32_t w = vv < end ? a.word[vv] : ST_E;
+        int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;  // list index when a.live is given
+        uint32_t w = vv < end ? a.word[a.live ? a.live[vv] : vv] : ST_E;
         if (!(w & ST_E) && !(a.skip_fb && (w & (ST_P | ST_F | ST_B)) == (ST_F | ST_B))) cand |= 1u << k;
     }
     constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, out-row (d = 0) or in-row
@@ -1020,7 +1021,8 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
         for (int j = 0; j < SCAN_ILP; j++) {
             kk[j] = cand ? __ffs(cand) - 1 : -1;
             cand &= cand ? cand - 1 : 0u;
-            v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * a.nt + threadIdx.x) : -1;
+            const int64_t i = base + (int64_t)kk[j] * a.nt + threadIdx.x;
+            v[j] = kk[j] >= 0 ? (a.live ? a.live[i] : (int32_t)i) : -1;
         }
         // the words (colours) load together with the heads and row bounds: only
         // this work-item writes a candidate's word in a sweep level
@@ -1100,9 +1102,9 @@ __global__ void k_trim_level(ReachArgs a) {
         }
     } else if (a.iter == 0 || qs >= (uint32_t)(a.n / TRIM_SWEEP_DIV)) {
         // sweep: rounds of TRIM_V vertices per work-item, no barrier until the
-        // round's trims are queued
+        // round's trims are queued; over the live-vertex list when there is one
         int64_t beg, end;
-        my_range(a.n, beg, end);
+        my_range(a.live ? (int64_t)*a.live_cnt : (int64_t)a.n, beg, end);
         unsigned long long ce = 0;
         const unsigned long long none[2] = {0ull, 0ull};
         for (int64_t base = beg; base < end; base += (int64_t)a.nt * TRIM_V) {  // block-uniform
@@ -1110,7 +1112,7 @@ __global__ void k_trim_level(ReachArgs a) {
             const uint32_t m = trim_round_scan(a, base, end, ce, newp, smax);
             if (m) found = true;
             const uint32_t claimed[2] = {m, 0u};
-            commit_round(a, base, claimed, none, it.shw, st.base, sh, true, newp);
+            commit_round(a, base, claimed, none, it.shw, st.base, sh, true, newp, a.live);
         }
     } else {
         // data-driven: re-check the live same-colour neighbours of the vertices
