@@ -39,9 +39,10 @@ KNOBS = {
     "reach_graphs": lambda n, cus: [0, 1],
     "reach_block": lambda n, cus: [0, 512],
     "hybrid_policy": lambda n, cus: [0, 2],
+    "tail_live": lambda n, cus: [0, 8192, -1],
 }
 ORDER = ["hybrid_policy", "grid_blocks", "reach_block", "hybrid_arc_div", "hybrid_threshold", "local_rounds",
-         "huge_row", "block_size", "reach_graphs"]
+         "huge_row", "block_size", "reach_graphs", "tail_live"]
 
 
 def log(*a):

@@ -97,6 +97,9 @@ struct scc_ctx {
 
 namespace {
 
+// live vertices at or below which the remaining FB steps run in one workgroup (k_fb_tail)
+constexpr uint32_t kTailLive = 1u << 16;
+
 scc_status fail(scc_ctx *c, scc_status s, const std::string &what) {
     if (c) c->err = what;
     return s;
@@ -664,7 +667,8 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
     if (opts) c->opts = *opts;
     if ((c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
                                      (c->opts.block_size & 63) != 0)) ||
-        (c->opts.reach_block != 0 && c->opts.reach_block != 256 && c->opts.reach_block != 512)) {
+        (c->opts.reach_block != 0 && c->opts.reach_block != 256 && c->opts.reach_block != 512) ||
+        c->opts.tail_live < -1) {
         delete c;
         return SCC_E_INPUT;
     }
@@ -704,6 +708,7 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
         return fail(c, SCC_E_INPUT, "block_size must be a multiple of 64 in [64, 1024]");
     if (opts->reach_block != 0 && opts->reach_block != 256 && opts->reach_block != 512)
         return fail(c, SCC_E_INPUT, "reach_block must be 0, 256 or 512");
+    if (opts->tail_live < -1) return fail(c, SCC_E_INPUT, "tail_live must be -1 (off), 0 (default) or a count");
     if (opts->grid_blocks < 0) return fail(c, SCC_E_INPUT, "grid_blocks < 0");
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
@@ -914,6 +919,55 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !learned &&
                              !by_elim && c->prof_flags == 0;
         while (unsynced || npiv > 0) {
+            // The tail: once few vertices are live, the remaining steps run in one
+            // workgroup and one launch (k_fb_tail); it stops early only before an
+            // election tag would wrap, and the loop then goes on as before.
+            if (spec_ok && !unsynced && c->lsel >= 0 && c->opts.tail_live >= 0 &&
+                c->h_ctl->live <= (c->opts.tail_live ? (uint32_t)c->opts.tail_live : kTailLive)) {
+                tp = std::chrono::steady_clock::now();
+                TailArgs ta{};
+                ta.row = c->g.row;
+                ta.col = c->g.col;
+                ta.rowT = c->g.rowT;
+                ta.colT = c->g.colT;
+                ta.word = c->d_word;
+                ta.root = c->d_root;
+                ta.n = n;
+                ta.slots[0] = c->d_slots[0];
+                ta.slots[1] = c->d_slots[1];
+                ta.slot_bits = c->slot_bits;
+                ta.prev_tab = c->prev_tab;
+                ta.epoch = c->epoch;
+                ta.piv = c->d_piv;
+                ta.list[0] = c->d_live[0];
+                ta.list[1] = c->d_live[1];
+                ta.lsel = c->lsel;
+                for (int i = 0; i < 4; i++) ta.q[i] = c->d_q[i];
+                ta.ctl = c->d_ctl;
+                ta.max_steps = c->h_ctl->live + 2;  // each step eliminates its pivots
+                ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
+                st.tail_live = c->h_ctl->live;
+                if (c->pend_stage || c->pend_bits) ctl_op(c, 0);  // queued resets go out first
+                launch_fb_tail(c->stream, ta);
+                CHK(read_ctl(c));
+                const Ctl &h = *c->h_ctl;
+                if (h.tail[1] == 2) return fail(c, SCC_E_STATE, "tail steps did not converge");
+                const uint32_t ts = h.tail[0];
+                st.steps += ts;
+                st.tail_steps += ts;
+                st.pivots_total += (int64_t)h.tail_pivots;
+                st.fwd_depths += h.tail[2];
+                st.bwd_depths += h.tail[2];
+                st.trim_iters += h.tail[3];
+                st.ms_fwd += ms_since(tp);
+                c->epoch += ts;
+                if (ts & 1u) {
+                    c->prev_tab ^= 1;
+                    c->lsel ^= 1;
+                }
+                npiv = h.npiv;
+                if (npiv == 0) break;
+            }
             st.steps++;
             c->one_colour = st.steps == 1;  // no recolouring yet: every arc stays inside a colour
             int sched = schedule == SCC_SCHED_B ? SCC_SCHED_H : schedule;

@@ -59,6 +59,9 @@ struct Ctl {
     uint32_t st_maxscan[4];  // instrumented: longest serial row scan (arcs) of one work-item,
                              // ring: level iter writes [iter & 3]
     uint32_t lcnt[2];        // lengths of the two live-vertex lists (an update reads one, writes the other)
+    uint32_t tail[4];        // k_fb_tail: steps run, status (0 done, 1 host continues, 2 error),
+                             // reach levels and trim rounds
+    unsigned long long tail_pivots;  // k_fb_tail: pivots elected
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

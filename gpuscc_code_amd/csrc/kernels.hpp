@@ -132,6 +132,28 @@ void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *l
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl);
 
+// kernels_tail.hip -- the remaining FB steps of a run in one workgroup, over the live list
+// list[lsel] (ctl->lcnt[lsel]) and the pivot list piv (ctl->npiv) of the last election,
+// whose table is slots[prev_tab] with tag elect_tag(epoch).  Stops before max_steps steps or
+// an epoch whose tag wraps (ctl->tail[1] = 1); ctl->tail and ctl->tail_pivots report the work.
+struct TailArgs {
+    const int32_t *row, *col, *rowT, *colT;
+    uint32_t *word;
+    int32_t *root;
+    int32_t n;
+    unsigned long long *slots[2];
+    int slot_bits, prev_tab;
+    uint32_t epoch;
+    int32_t *piv;
+    int32_t *list[2];
+    int lsel;
+    int32_t *q[4];  // queues, n entries each
+    Ctl *ctl;
+    uint32_t max_steps;  // live vertices + 2: bounds the steps, and each step's levels and trim rounds
+    int xbits;           // the run's schedule marks expanded vertices (q, t: FExt / BExt)
+};
+void launch_fb_tail(hipStream_t s, const TailArgs &t);
+
 // kernels_mlp.hip -- learned pivot scorer
 void launch_static_features(const Launch &l, const Graph &g, float4 *out);
 void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits);

@@ -1,0 +1,436 @@
+// The tail of an FB run in one workgroup.  Once few vertices are live (after
+// the first FB steps of a high-diameter graph only a fraction of a percent
+// stay live, spread over many small colour sets), an FB step is about a dozen
+// kernel launches that have almost nothing to do, and their launch and
+// inter-kernel costs, not their work, set the run time: the plain-variant mesh
+// takes 387 such steps.  k_fb_tail runs the remaining steps in one launch of
+// one workgroup, with barriers in place of kernel boundaries.  Each step does
+// what the multi-kernel path does, so the final colours, states and roots are
+// the same (the reach is a closure, the trim fixpoint is unique, and the
+// election keys are the same):
+//   reach F and B from the pivot list inside colours (reach_q.cu:12-79),
+//   trim to the fixpoint, skipping F-and-B non-pivots (trim_kernel,
+//   scc_routines.cu:336-394), update + election + confirm
+//   (update_kernel, scc_routines.cu:187-224, lowest key per slot).
+// The steps first run over global memory and the live list of the last
+// election (GlobalSpace).  As soon as the live subgraph fits in LDS -- the
+// words and global ids of the live vertices, and the arcs that join two live
+// vertices of one colour, the only arcs any later step follows, since colours
+// only split and eliminated vertices stay eliminated -- it is copied there once
+// and the remaining steps touch global memory only for roots and the election
+// slots (LocalSpace); the words, the live list and the pivot list are written
+// back at the end.
+#include "fb_device.hpp"
+#include "kernels.hpp"
+
+namespace gpuscc {
+
+constexpr int TAIL_BLOCK = 1024;
+// LDS for the local copy: words and ids (8 B per vertex), out and in row
+// offsets (8 B), four uint16 queues (8 B) and uint16 arcs
+constexpr uint32_t TAIL_ARENA = 150u * 1024u;
+
+__host__ __device__ inline uint32_t tail_fixed_bytes(uint32_t L) { return 24u * L + 8u; }
+
+// The local copy lives here; its parts are addressed by offsets from this
+// array, so every access compiles to an LDS instruction.
+__shared__ __align__(16) uint8_t tail_arena[TAIL_ARENA];
+
+// Every access of the tail comes from this one workgroup, whose waves share
+// one CU and its L1, so a workgroup-scope barrier orders them (the release /
+// acquire fences of __syncthreads).  Agent-scope fences would write back and
+// invalidate the XCD's L2 at every barrier (the L2s of the XCDs are not
+// coherent with each other), several microseconds each; the kernel boundaries
+// order the tail with the kernels before and after it.
+__device__ inline void tail_sync() { __syncthreads(); }
+
+struct TailShared {
+    uint32_t cnt[4];  // queue sizes: F in, B in, F out, B out
+    uint32_t npiv, lcnt, first, flag;
+    unsigned long long red[2];  // this step: vertices leaving the live set, new P by trims
+    uint32_t scan[TAIL_BLOCK / WAVE];
+};
+
+// Vertices are global ids; the trim and update go over the live list of the
+// last election, the confirm over the list the update writes.
+struct GlobalSpace {
+    const TailArgs *t;
+    const int32_t *lst;
+    uint32_t lc;
+    int32_t *out;
+    __device__ uint32_t size() const { return lc; }
+    __device__ int32_t item(uint32_t i) const { return lst[i]; }
+    __device__ uint32_t csize(uint32_t ln) const { return ln; }
+    __device__ int32_t citem(uint32_t i) const { return out[i]; }
+    __device__ void keep(uint32_t k, int32_t v) const { out[k] = v; }
+    __device__ uint32_t ld(int32_t v) const { return t->word[v]; }
+    __device__ void st(int32_t v, uint32_t w) const { t->word[v] = w; }
+    __device__ uint32_t orw(int32_t v, uint32_t b) const { return atomicOr(&t->word[v], b); }
+    __device__ int32_t gid(int32_t v) const { return v; }
+    __device__ int32_t *qptr(int b) const {  // select, not an indexed load of the pointer
+        int32_t *q0 = t->q[0], *q1 = t->q[1], *q2 = t->q[2], *q3 = t->q[3];
+        return b == 0 ? q0 : b == 1 ? q1 : b == 2 ? q2 : q3;
+    }
+    __device__ int32_t qget(int b, uint32_t k) const { return qptr(b)[k]; }
+    __device__ void qput(int b, uint32_t k, int32_t v) const { qptr(b)[k] = v; }
+    __device__ void start(uint32_t np, uint32_t tid) const {
+        for (uint32_t i = tid; i < np; i += TAIL_BLOCK) t->q[0][i] = t->q[1][i] = t->piv[i];
+    }
+    __device__ void pivot(uint32_t k, int32_t v) const { t->piv[k] = v; }
+    template <class F>
+    __device__ bool each_nb(int d, int32_t v, F f) const {  // true once f is
+        const int32_t *row = d == 0 ? t->row : t->rowT, *col = d == 0 ? t->col : t->colT;
+        for (int32_t e = row[v]; e < row[v + 1]; e++)
+            if (f(col[e])) return true;
+        return false;
+    }
+};
+
+// Vertices are local indexes 0..L-1 into the LDS copy; the pivot list is the
+// first two queues (a step's reach starts from them).  tail_arena holds, for L
+// live vertices and `to` out-arcs: words[L], global ids[L], out offsets[L+1],
+// in offsets[L+1], four uint16 queues[L], out arcs[to], in arcs (uint16).
+struct LocalSpace {
+    uint32_t L, to;
+    __device__ uint32_t *w() const { return (uint32_t *)tail_arena; }
+    __device__ int32_t *gidv() const { return (int32_t *)(tail_arena + 4u * L); }
+    __device__ uint32_t *off(int d) const { return (uint32_t *)(tail_arena + 8u * L) + (d ? L + 1 : 0); }
+    __device__ uint16_t *q(int b) const { return (uint16_t *)(tail_arena + 16u * L + 8u) + b * L; }
+    __device__ uint16_t *adj(int d) const { return (uint16_t *)(tail_arena + tail_fixed_bytes(L)) + (d ? to : 0); }
+    __device__ uint32_t size() const { return L; }
+    __device__ int32_t item(uint32_t i) const { return (int32_t)i; }
+    __device__ uint32_t csize(uint32_t) const { return L; }
+    __device__ int32_t citem(uint32_t i) const { return (int32_t)i; }
+    __device__ void keep(uint32_t, int32_t) const {}
+    __device__ uint32_t ld(int32_t v) const { return w()[v]; }
+    __device__ void st(int32_t v, uint32_t x) const { w()[v] = x; }
+    __device__ uint32_t orw(int32_t v, uint32_t b) const { return atomicOr(&w()[v], b); }
+    __device__ int32_t gid(int32_t v) const { return gidv()[v]; }
+    __device__ int32_t qget(int b, uint32_t k) const { return q(b)[k]; }
+    __device__ void qput(int b, uint32_t k, int32_t v) const { q(b)[k] = (uint16_t)v; }
+    __device__ void start(uint32_t, uint32_t) const {}
+    __device__ void pivot(uint32_t k, int32_t v) const { q(0)[k] = q(1)[k] = (uint16_t)v; }
+    template <class F>
+    __device__ bool each_nb(int d, int32_t v, F f) const {
+        const uint32_t *o = off(d);
+        const uint16_t *a = adj(d);
+        for (uint32_t e = o[v]; e < o[v + 1]; e++)
+            if (f((int32_t)a[e])) return true;
+        return false;
+    }
+};
+
+struct TailRun {
+    int lsel, pt;
+    uint32_t epoch, npiv;
+    unsigned long long depths, trims;
+};
+
+// One FB step; false when a bound that every correct run stays within is passed.
+template <class Sp>
+__device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailRun &r) {
+    const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
+    Ctl *ctl = t.ctl;
+    // ---- reach: F over out-arcs, B over in-arcs, both from the pivot list
+    if (tid == 0) {
+        sh.cnt[0] = sh.cnt[1] = r.npiv;
+        sh.cnt[2] = sh.cnt[3] = 0;
+    }
+    sp.start(r.npiv, tid);
+    tail_sync();
+    int qi = 0;  // input queues: 2 * qi + d
+    for (uint32_t level = 0;; level++) {
+        const uint32_t n0 = sh.cnt[0], n1 = sh.cnt[1];
+        if (n0 + n1 == 0) break;                     // block-uniform (read after the sync)
+        if (level > 2u * t.max_steps) return false;  // every level queues a new vertex
+        r.depths++;
+        for (uint32_t i = tid; i < n0 + n1; i += nt) {
+            const int d = i < n0 ? 0 : 1;
+            const int32_t v = sp.qget(2 * qi + d, d == 0 ? i : i - n0);
+            const uint32_t c = colour_of(sp.ld(v)), bit = d == 0 ? ST_F : ST_B;
+            // the sweep schedules mark every vertex they expand (reach_q.cu:44-79), and an
+            // FB-eliminated word keeps the mark: the same words as the multi-kernel path
+            if (t.xbits) sp.orw(v, d == 0 ? ST_FX : ST_BX);
+            const int ob = 2 * (qi ^ 1) + d;
+            sp.each_nb(d, v, [&](int32_t u) {
+                const uint32_t wu = sp.ld(u);
+                if ((wu & (bit | ST_E)) || colour_of(wu) != c) return false;
+                if (sp.orw(u, bit) & bit) return false;
+                sp.qput(ob, atomicAdd(&sh.cnt[2 + d], 1u), u);  // each vertex queued once
+                return false;
+            });
+        }
+        tail_sync();
+        if (tid == 0) {
+            sh.cnt[0] = sh.cnt[2];
+            sh.cnt[1] = sh.cnt[3];
+            sh.cnt[2] = sh.cnt[3] = 0;
+        }
+        qi ^= 1;
+        tail_sync();
+    }
+    // ---- trim to the fixpoint (F and B without P: an SCC of two or more); a
+    // round that trims nothing ends it (the count is read between two barriers,
+    // so every work-item takes the same decision)
+    const uint32_t lc = sp.size();
+    if (tid < 2) sh.red[tid] = 0;
+    tail_sync();
+    for (uint32_t it = 0, seen = 0;; it++) {
+        for (uint32_t i = tid; i < lc; i += nt) {
+            const int32_t v = sp.item(i);
+            const uint32_t w = sp.ld(v);
+            if ((w & ST_E) || (w & (ST_P | ST_F | ST_B)) == (ST_F | ST_B)) continue;
+            const uint32_t c = colour_of(w);
+            auto live = [&](int32_t u) {
+                if (u == v) return false;
+                const uint32_t wu = sp.ld(u);
+                return !(wu & ST_E) && colour_of(wu) == c;
+            };
+            if (sp.each_nb(0, v, live) && sp.each_nb(1, v, live)) continue;
+            sp.st(v, make_word(c, 28));  // P|T|E (scc_routines.cu:381)
+            const int32_t g = sp.gid(v);
+            t.root[g] = g;
+            atomicAdd(&sh.red[0], 1ull);
+            if (!(w & ST_P)) atomicAdd(&sh.red[1], 1ull);  // a trimmed pivot holds P already
+        }
+        tail_sync();
+        const uint32_t now = (uint32_t)sh.red[0];
+        tail_sync();
+        r.trims++;
+        if (now == seen) break;  // block-uniform
+        seen = now;
+        if (it > t.max_steps) return false;  // every round but the last trims a vertex
+    }
+    // ---- update: eliminate F and B into the pivot's SCC, split the rest, elect
+    r.epoch++;
+    const unsigned long long tag = elect_tag(r.epoch);
+    const unsigned long long *prev = t.slots[r.pt];
+    unsigned long long *cur = t.slots[r.pt ^ 1];
+    if (tid == 0) {
+        sh.lcnt = 0;
+        sh.npiv = 0;
+        sh.first = 0xFFFFFFFFu;
+    }
+    tail_sync();
+    for (uint32_t i = tid; i < lc; i += nt) {
+        const int32_t v = sp.item(i);
+        const uint32_t w = sp.ld(v);
+        if (w & ST_E) continue;
+        uint32_t c = colour_of(w);
+        const uint32_t f = ((w & ST_F) ? 1u : 0u) + ((w & ST_B) ? 2u : 0u);
+        const int32_t g = sp.gid(v);
+        if (f == 3) {
+            sp.st(v, w | ST_E);
+            const uint32_t pv = (uint32_t)prev[slot_of(c, t.slot_bits)];
+            t.root[g] = pv < (uint32_t)t.n ? (int32_t)pv : g;
+            atomicAdd(&sh.red[0], 1ull);
+            continue;
+        }
+        c = (3u * c + f) & COLOUR_MASK;
+        sp.st(v, make_word(c, 0));
+        sp.keep(atomicAdd(&sh.lcnt, 1u), v);
+        // read the slot first: most vertices of a colour lose, and their atomics
+        // would queue at one address
+        const unsigned long long key = tag | (uint32_t)g;
+        unsigned long long *slot = &cur[slot_of(c, t.slot_bits)];
+        if (key < __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(slot, key);
+    }
+    tail_sync();
+    // ---- confirm: the lowest key of each slot becomes its colour's pivot (state 19)
+    const uint32_t ln = sh.lcnt, cn = sp.csize(ln);
+    for (uint32_t i = tid; i < cn; i += nt) {
+        const int32_t v = sp.citem(i);
+        const uint32_t w = sp.ld(v);
+        if (w & ST_E) continue;
+        const uint32_t c = colour_of(w);
+        const int32_t g = sp.gid(v);
+        const uint32_t win =
+            (uint32_t)__hip_atomic_load(&cur[slot_of(c, t.slot_bits)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (win != (uint32_t)g) continue;
+        sp.st(v, make_word(c, 19));
+        sp.pivot(atomicAdd(&sh.npiv, 1u), v);
+        atomicMin(&sh.first, (uint32_t)g);
+    }
+    tail_sync();
+    r.npiv = sh.npiv;
+    if (tid == 0) {  // the device counts the host reads at the end
+        ctl->live -= (uint32_t)sh.red[0];
+        ctl->pcount += (uint32_t)sh.red[1] + r.npiv;
+        ctl->lcnt[r.lsel ^ 1] = ln;
+        ctl->npiv_prev = ctl->npiv;
+        ctl->first_piv_prev = ctl->first_piv;
+        ctl->npiv = r.npiv;
+        ctl->first_piv = sh.first;
+    }
+    r.pt ^= 1;
+    r.lsel ^= 1;
+    tail_sync();
+    return true;
+}
+
+__device__ inline bool tail_arc(const TailArgs &t, int32_t v, int32_t u, uint32_t c) {
+    if (u == v) return false;
+    const uint32_t wu = t.word[u];
+    return !(wu & ST_E) && colour_of(wu) == c;
+}
+
+// Copies the L live vertices of list lst, and the arcs between live vertices
+// of one colour, into LDS.  1: copied; 0: the arcs do not fit; -1: a live
+// vertex outside the list, which the elections never leave.
+__device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uint32_t L, const int32_t *lst,
+                          uint32_t npiv) {
+    const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
+    ls.L = L;
+    ls.to = 0;
+    uint32_t *lw = ls.w(), *off0 = ls.off(0), *off1 = ls.off(1);
+    int32_t *gidv = ls.gidv();
+    uint32_t *map = (uint32_t *)t.q[3];  // global id -> local index of the live vertices
+    for (uint32_t i = tid; i < L; i += nt) {
+        const int32_t v = lst[i];
+        gidv[i] = v;
+        lw[i] = t.word[v];
+        map[v] = i;
+    }
+    if (tid == 0) sh.flag = 0;
+    tail_sync();
+    // each work-item counts a run of consecutive vertices, so one scan of the
+    // per-work-item sums places every row
+    const uint32_t per = (L + nt - 1) / nt, b = min(tid * per, L), e = min(b + per, L);
+    uint32_t so = 0, si = 0;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t w = lw[i], c = colour_of(w);
+        const int32_t v = gidv[i];
+        uint32_t k[2] = {0, 0};
+        if (!(w & ST_E)) {
+            for (int d = 0; d < 2; d++) {
+                const int32_t *row = d == 0 ? t.row : t.rowT, *col = d == 0 ? t.col : t.colT;
+                for (int32_t x = row[v]; x < row[v + 1]; x++) {
+                    const int32_t u = col[x];
+                    if (!tail_arc(t, v, u, c)) continue;
+                    const uint32_t j = map[u];
+                    if (j >= L || gidv[j] != u) sh.flag = 1;
+                    k[d]++;
+                }
+            }
+        }
+        off0[i] = k[0];
+        off1[i] = k[1];
+        so += k[0];
+        si += k[1];
+    }
+    uint32_t to = 0, ti = 0;
+    const uint32_t bo = block_excl_scan(so, sh.scan, to, nt);
+    const uint32_t bi = block_excl_scan(si, sh.scan, ti, nt);
+    tail_sync();
+    if (sh.flag) return -1;                                             // block-uniform
+    if (tail_fixed_bytes(L) + 2ull * (to + ti) > TAIL_ARENA) return 0;  // block-uniform
+    ls.to = to;
+    uint16_t *adj0 = ls.adj(0), *adj1 = ls.adj(1), *q0 = ls.q(0), *q1 = ls.q(1);
+    uint32_t po = bo, pi = bi;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t ko = off0[i], ki = off1[i];
+        off0[i] = po;
+        off1[i] = pi;
+        const uint32_t w = lw[i], c = colour_of(w);
+        const int32_t v = gidv[i];
+        if (ko + ki) {
+            for (int d = 0; d < 2; d++) {
+                const int32_t *row = d == 0 ? t.row : t.rowT, *col = d == 0 ? t.col : t.colT;
+                uint16_t *a = d == 0 ? adj0 : adj1;
+                uint32_t p = d == 0 ? po : pi;
+                for (int32_t x = row[v]; x < row[v + 1]; x++) {
+                    const int32_t u = col[x];
+                    if (tail_arc(t, v, u, c)) a[p++] = (uint16_t)map[u];
+                }
+            }
+        }
+        po += ko;
+        pi += ki;
+    }
+    if (tid == 0) {
+        off0[L] = to;
+        off1[L] = ti;
+    }
+    for (uint32_t k = tid; k < npiv; k += nt) {
+        const uint32_t j = map[t.piv[k]];
+        q0[k] = q1[k] = (uint16_t)j;
+        if (j >= L || gidv[j] != t.piv[k]) sh.flag = 1;
+    }
+    tail_sync();
+    return sh.flag ? -1 : 1;
+}
+
+// Words, the live list (lsel) and the pivot list back to global memory.
+__device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShared &sh, int lsel, uint32_t npiv) {
+    const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
+    if (tid == 0) sh.lcnt = 0;
+    tail_sync();
+    int32_t *out = t.list[lsel];
+    const uint32_t *lw = ls.w();
+    const int32_t *gidv = ls.gidv();
+    const uint16_t *q0 = ls.q(0);
+    for (uint32_t i = tid; i < ls.L; i += nt) {
+        const uint32_t w = lw[i];
+        const int32_t v = gidv[i];
+        t.word[v] = w;
+        if (!(w & ST_E)) out[atomicAdd(&sh.lcnt, 1u)] = v;
+    }
+    for (uint32_t k = tid; k < npiv; k += nt) t.piv[k] = gidv[q0[k]];
+    tail_sync();
+    if (tid == 0) t.ctl->lcnt[lsel] = sh.lcnt;
+}
+
+__global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
+    __shared__ TailShared sh;
+    Ctl *ctl = t.ctl;
+    TailRun r{t.lsel, t.prev_tab, t.epoch, ctl->npiv, 0, 0};
+    uint32_t steps = 0, status = 0, try_below = 0xFFFFFFFFu;
+    unsigned long long pivots = 0;
+    bool local = false;
+    LocalSpace ls{};
+    while (r.npiv > 0) {
+        if (steps >= t.max_steps || ((r.epoch + 1u) & 0xFFFFu) < 2u) {  // the host takes over
+            status = 1;
+            break;
+        }
+        if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
+            const uint32_t L = ctl->lcnt[r.lsel];
+            if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {
+                const int b = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv);
+                if (b < 0) {
+                    status = 2;
+                    break;
+                }
+                local = b > 0;
+                try_below = L / 2;
+            }
+        }
+        steps++;
+        pivots += r.npiv;  // the pivots this step reaches from
+        bool ok;
+        if (local) {
+            ok = tail_step(t, ls, sh, r);
+        } else {
+            const GlobalSpace gs{&t, t.list[r.lsel], ctl->lcnt[r.lsel], t.list[r.lsel ^ 1]};
+            ok = tail_step(t, gs, sh, r);
+        }
+        if (!ok) {
+            status = 2;
+            break;
+        }
+    }
+    if (local) tail_writeback(t, ls, sh, r.lsel, r.npiv);
+    if (threadIdx.x == 0) {
+        ctl->tail[0] = steps;
+        ctl->tail[1] = status;
+        ctl->tail[2] = (uint32_t)r.depths;
+        ctl->tail[3] = (uint32_t)r.trims;
+        ctl->tail_pivots = pivots;
+    }
+}
+
+void launch_fb_tail(hipStream_t s, const TailArgs &t) {
+    hipLaunchKernelGGL(k_fb_tail, dim3(1), dim3(TAIL_BLOCK), 0, s, t);
+}
+
+}  // namespace gpuscc

@@ -42,6 +42,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     hybrid_arc_div: int = 0
     reach_graphs: int = 0
     reach_block: int = 0
+    tail_live: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -56,6 +57,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.hybrid_arc_div = self.hybrid_arc_div
         o.reach_graphs = self.reach_graphs
         o.reach_block = self.reach_block
+        o.tail_live = self.tail_live
         return o
 
 

@@ -82,6 +82,8 @@ typedef struct {
                                   graphs and replayed (one launch per chain); 0 = launched level by level */
     int32_t reach_block;       /* workgroup size of the level kernels (reach t/d/l/h, trim): 256 or 512
                                   (0 = 256) */
+    int32_t tail_live;         /* once at most this many vertices are live, the remaining FB steps run in
+                                  one workgroup and one launch (0 = default 65536, -1 = off) */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -98,6 +100,8 @@ typedef struct {
     int64_t biggest;        /* largest SCC size */
     int64_t host_syncs;     /* device->host waits the run made */
     int64_t first_pivot;    /* first pivot (lowest live id after trim-1, scc.cu:235-245), -1 if none */
+    int64_t tail_steps;     /* of steps, those run in the one-workgroup tail (scc_opts.tail_live) */
+    int64_t tail_live;      /* live vertices when the tail started, 0 if it did not run */
     double ms_total;        /* upload-excluded wall time, labels ready (scc.cu:217->1011) */
     double ms_fwd, ms_bwd, ms_trim, ms_update; /* per phase, host clock (STOPWATCH, timing.h:14) */
 } scc_stats;

@@ -60,3 +60,23 @@ def test_create_without_gpu_fails_cleanly():
     h = ctypes.c_void_p()
     st = _lib.lib().scc_create(0, None, ctypes.byref(h))
     assert st == _lib.SCC_E_HIP and not h.value
+
+
+def header_struct_fields(name):
+    """Field names, in order, of `typedef struct { ... } name;` in gpuscc.h."""
+    text = open(os.path.join(ROOT, "include", "gpuscc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + ";", text).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if decl:
+            fields += [re.sub(r"[\s*]", "", f) for f in re.sub(r"^[A-Za-z0-9_ ]+?\s+(?=[*A-Za-z_])", "", decl,
+                                                                   count=1).split(",")]
+    return fields
+
+
+@pytest.mark.parametrize("name,cls", [("scc_opts", _lib.SccOpts), ("scc_stats", _lib.SccStats),
+                                      ("scc_profile_data", _lib.SccProfile), ("scc_host_graph", _lib.SccHostGraph)])
+def test_binding_structs_match_the_header(name, cls):
+    assert header_struct_fields(name) == [f[0] for f in cls._fields_]

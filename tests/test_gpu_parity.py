@@ -451,6 +451,64 @@ def test_wide_level_workgroups(ctx, sched):
         ctx.set_opts(scc.alg_opt())
 
 
+@pytest.mark.parametrize("sched", ["q", "d", "h"])
+def test_tail_settings(ctx, sched):
+    """The one-workgroup tail (tail_live) ends where the multi-kernel steps end:
+    the same words, roots, steps and pivots as with it off, and the oracle's
+    full state, whether it starts at once, late or never.  The planted graph's
+    live arcs first exceed the tail's LDS copy (steps over global memory until
+    the live set has split), the larger mesh starts it with more live vertices
+    than the copy holds."""
+    keys = ("steps", "num_scc", "pivots_total", "first_pivot", "trivial", "biggest")
+    try:
+        for g in (demo_graph(), scc.gen_mesh(96, 0.6, seed=5), planted(1500, 2, 75000, seed=7)[0],
+                  scc.gen_mesh(256, 0.6, seed=8), random_digraph(3000, 12000, seed=12, self_loops=True),
+                  cycle(3000)):
+            ref, _ = oracle.tarjan(g.row, g.col)
+            ctx.upload(g)
+            ctx.set_opts(scc.alg_opt(tail_live=-1))
+            lab0, st0 = ctx.run(sched)
+            w0, r0 = ctx.get_state()
+            assert np.array_equal(lab0, ref) and st0["tail_steps"] == 0
+            for tl in (0, 1, 64, 1 << 30):
+                ctx.set_opts(scc.alg_opt(tail_live=tl))
+                lab, st = ctx.run(sched)
+                w, r = ctx.get_state()
+                assert np.array_equal(lab, ref)
+                assert np.array_equal(w, w0) and np.array_equal(r, r0)
+                assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}
+                if tl == 1 << 30 and st0["steps"] > 1:
+                    assert st["tail_steps"] == st["steps"] - 1 and st["tail_live"] > 0
+        ctx.set_opts(scc.alg_opt(tail_live=1 << 30))
+        check_against_oracle(ctx, planted(1500, 2, 75000, seed=7)[0], sched, 0)
+        check_against_oracle(ctx, planted(2000, 2, 2500, seed=3)[0], sched, 0)
+        with pytest.raises(scc.SccError):
+            ctx.set_opts(scc.alg_opt(tail_live=-2))
+    finally:
+        ctx.set_opts(scc.alg_opt())
+
+
+def test_tail_hands_back_at_tag_wrap(ctx):
+    """Election tags are 16 bits and the elections of every run of a context
+    share them: the tail stops before the election whose tag would wrap, and
+    the host refills the table and goes on.  Runs until more than 2^16
+    elections have passed; every run ends with the first run's state."""
+    g = planted(2000, 2, 2500, seed=3)[0]  # 810 FB steps
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    lab0, st0 = ctx.run("h")
+    w0, r0 = ctx.get_state()
+    assert np.array_equal(lab0, ref) and st0["steps"] >= 100 and st0["tail_steps"] > 0
+    elections, runs = st0["steps"] + 1, 1
+    while elections < (1 << 16) + 3 * st0["steps"]:
+        lab, st = ctx.run("h")
+        w, r = ctx.get_state()
+        assert np.array_equal(lab, ref) and np.array_equal(w, w0) and np.array_equal(r, r0), f"run {runs}"
+        assert st["steps"] == st0["steps"] and st["pivots_total"] == st0["pivots_total"]
+        elections += st["steps"] + 1
+        runs += 1
+
+
 def test_level_log(ctx):
     """Profile flag 4 logs one row per level launched: each chain is a run of
     levels 0, 1, 2, ...; one reach chain per FB step; a level's longest serial
