@@ -51,9 +51,25 @@ struct TailShared {
     uint32_t scan[TAIL_BLOCK / WAVE];
 };
 
+// The election through the global slot tables (the multi-kernel path's):
+// prev = slots[pt] holds the last election's keys, cur = slots[pt ^ 1] this one's.
+struct GlobalSlots {
+    __device__ void clear_cur(const TailArgs &, int, uint32_t) const {}
+    __device__ uint32_t prev_winner(const TailArgs &t, int pt, uint32_t s) const { return (uint32_t)t.slots[pt][s]; }
+    __device__ void offer(const TailArgs &t, int pt, uint32_t s, unsigned long long key) const {
+        // read the slot first: most vertices of a colour lose, and their atomics
+        // would queue at one address
+        unsigned long long *slot = &t.slots[pt ^ 1][s];
+        if (key < __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(slot, key);
+    }
+    __device__ uint32_t winner(const TailArgs &t, int pt, uint32_t s) const {
+        return (uint32_t)__hip_atomic_load(&t.slots[pt ^ 1][s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+};
+
 // Vertices are global ids; the trim and update go over the live list of the
 // last election, the confirm over the list the update writes.
-struct GlobalSpace {
+struct GlobalSpace : GlobalSlots {
     const TailArgs *t;
     const int32_t *lst;
     uint32_t lc;
@@ -86,12 +102,56 @@ struct GlobalSpace {
     }
 };
 
+constexpr uint32_t TAB_EMPTY = 0xFFFFFFFFu;  // no slot id (slot ids are below 2^24)
+
 // Vertices are local indexes 0..L-1 into the LDS copy; the pivot list is the
 // first two queues (a step's reach starts from them).  tail_arena holds, for L
 // live vertices and `to` out-arcs: words[L], global ids[L], out offsets[L+1],
-// in offsets[L+1], four uint16 queues[L], out arcs[to], in arcs (uint16).
-struct LocalSpace {
-    uint32_t L, to;
+// in offsets[L+1], four uint16 queues[L], out arcs[to], in arcs (uint16), and,
+// when there is room (H > 0, from byte t0), two election tables of H entries
+// (slot id, lowest global id), open addressing: the slot tables of the steps
+// in LDS, cleared every step instead of tagged.  Without the room, the global
+// slot tables serve.
+struct LocalSpace : GlobalSlots {
+    uint32_t L, to, H, t0;
+    __device__ uint32_t *tab(int k) const { return (uint32_t *)(tail_arena + t0) + (k & 1) * 2u * H; }
+    __device__ void clear_cur(const TailArgs &t, int pt, uint32_t tid) const {
+        if (!H) return;
+        uint32_t *e = tab(pt ^ 1);
+        for (uint32_t i = tid; i < 2u * H; i += TAIL_BLOCK) e[i] = TAB_EMPTY;
+    }
+    __device__ uint32_t find(int k, uint32_t s) const {
+        const uint32_t *e = tab(k);
+        for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {
+            const uint32_t x = e[h];
+            if (x == s) return e[H + h];
+            if (x == TAB_EMPTY) break;
+        }
+        return TAB_EMPTY;
+    }
+    __device__ void insert(int k, uint32_t s, uint32_t g) const {
+        uint32_t *e = tab(k);
+        for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {  // H > distinct slots
+            uint32_t x = e[h];
+            if (x == TAB_EMPTY) x = atomicCAS(&e[h], TAB_EMPTY, s);
+            if (x == TAB_EMPTY || x == s) {
+                if (g < e[H + h]) atomicMin(&e[H + h], g);
+                return;
+            }
+        }
+    }
+    __device__ uint32_t prev_winner(const TailArgs &t, int pt, uint32_t s) const {
+        return H ? find(pt, s) : GlobalSlots::prev_winner(t, pt, s);
+    }
+    __device__ void offer(const TailArgs &t, int pt, uint32_t s, unsigned long long key) const {
+        if (H)
+            insert(pt ^ 1, s, (uint32_t)key);
+        else
+            GlobalSlots::offer(t, pt, s, key);
+    }
+    __device__ uint32_t winner(const TailArgs &t, int pt, uint32_t s) const {
+        return H ? find(pt ^ 1, s) : GlobalSlots::winner(t, pt, s);
+    }
     __device__ uint32_t *w() const { return (uint32_t *)tail_arena; }
     __device__ int32_t *gidv() const { return (int32_t *)(tail_arena + 4u * L); }
     __device__ uint32_t *off(int d) const { return (uint32_t *)(tail_arena + 8u * L) + (d ? L + 1 : 0); }
@@ -137,6 +197,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         sh.cnt[2] = sh.cnt[3] = 0;
     }
     sp.start(r.npiv, tid);
+    sp.clear_cur(t, r.pt, tid);  // this step's election table (the reach does not touch it)
     tail_sync();
     int qi = 0;  // input queues: 2 * qi + d
     for (uint32_t level = 0;; level++) {
@@ -204,8 +265,6 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
     // ---- update: eliminate F and B into the pivot's SCC, split the rest, elect
     r.epoch++;
     const unsigned long long tag = elect_tag(r.epoch);
-    const unsigned long long *prev = t.slots[r.pt];
-    unsigned long long *cur = t.slots[r.pt ^ 1];
     if (tid == 0) {
         sh.lcnt = 0;
         sh.npiv = 0;
@@ -221,7 +280,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         const int32_t g = sp.gid(v);
         if (f == 3) {
             sp.st(v, w | ST_E);
-            const uint32_t pv = (uint32_t)prev[slot_of(c, t.slot_bits)];
+            const uint32_t pv = sp.prev_winner(t, r.pt, slot_of(c, t.slot_bits));
             t.root[g] = pv < (uint32_t)t.n ? (int32_t)pv : g;
             atomicAdd(&sh.red[0], 1ull);
             continue;
@@ -229,11 +288,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         c = (3u * c + f) & COLOUR_MASK;
         sp.st(v, make_word(c, 0));
         sp.keep(atomicAdd(&sh.lcnt, 1u), v);
-        // read the slot first: most vertices of a colour lose, and their atomics
-        // would queue at one address
-        const unsigned long long key = tag | (uint32_t)g;
-        unsigned long long *slot = &cur[slot_of(c, t.slot_bits)];
-        if (key < __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(slot, key);
+        sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)g);
     }
     tail_sync();
     // ---- confirm: the lowest key of each slot becomes its colour's pivot (state 19)
@@ -244,8 +299,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         if (w & ST_E) continue;
         const uint32_t c = colour_of(w);
         const int32_t g = sp.gid(v);
-        const uint32_t win =
-            (uint32_t)__hip_atomic_load(&cur[slot_of(c, t.slot_bits)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t win = sp.winner(t, r.pt, slot_of(c, t.slot_bits));
         if (win != (uint32_t)g) continue;
         sp.st(v, make_word(c, 19));
         sp.pivot(atomicAdd(&sh.npiv, 1u), v);
@@ -278,10 +332,11 @@ __device__ inline bool tail_arc(const TailArgs &t, int32_t v, int32_t u, uint32_
 // of one colour, into LDS.  1: copied; 0: the arcs do not fit; -1: a live
 // vertex outside the list, which the elections never leave.
 __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uint32_t L, const int32_t *lst,
-                          uint32_t npiv) {
+                          uint32_t npiv, int pt) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
     ls.L = L;
     ls.to = 0;
+    ls.H = 0;
     uint32_t *lw = ls.w(), *off0 = ls.off(0), *off1 = ls.off(1);
     int32_t *gidv = ls.gidv();
     uint32_t *map = (uint32_t *)t.q[3];  // global id -> local index of the live vertices
@@ -325,6 +380,11 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uin
     if (sh.flag) return -1;                                             // block-uniform
     if (tail_fixed_bytes(L) + 2ull * (to + ti) > TAIL_ARENA) return 0;  // block-uniform
     ls.to = to;
+    // election tables when they fit: H a power of two above 1.5 L (at most L distinct slots)
+    ls.t0 = (tail_fixed_bytes(L) + 2u * (to + ti) + 15u) & ~15u;
+    uint32_t H = 64;
+    while (H < L + L / 2 + 1) H <<= 1;
+    if (ls.t0 + 16ull * H <= TAIL_ARENA) ls.H = H;
     uint16_t *adj0 = ls.adj(0), *adj1 = ls.adj(1), *q0 = ls.q(0), *q1 = ls.q(1);
     uint32_t po = bo, pi = bi;
     for (uint32_t i = b; i < e; i++) {
@@ -356,12 +416,21 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uin
         q0[k] = q1[k] = (uint16_t)j;
         if (j >= L || gidv[j] != t.piv[k]) sh.flag = 1;
     }
+    ls.clear_cur(t, pt ^ 1, tid);  // the table of the last election: its winners are the pivots
     tail_sync();
-    return sh.flag ? -1 : 1;
+    if (sh.flag) return -1;
+    if (ls.H)
+        for (uint32_t k = tid; k < npiv; k += nt)
+            ls.insert(pt, slot_of(colour_of(lw[q0[k]]), t.slot_bits), (uint32_t)gidv[q0[k]]);
+    tail_sync();
+    return 1;
 }
 
-// Words, the live list (lsel) and the pivot list back to global memory.
-__device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShared &sh, int lsel, uint32_t npiv) {
+// Words, the live list (lsel) and the pivot list back to global memory, and,
+// when the election tables were in LDS, the keys of the last election's
+// winners into its global table (the host's next election reads them there).
+__device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShared &sh, int lsel, uint32_t npiv, int pt,
+                               uint32_t epoch) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
     if (tid == 0) sh.lcnt = 0;
     tail_sync();
@@ -375,7 +444,11 @@ __device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShar
         t.word[v] = w;
         if (!(w & ST_E)) out[atomicAdd(&sh.lcnt, 1u)] = v;
     }
-    for (uint32_t k = tid; k < npiv; k += nt) t.piv[k] = gidv[q0[k]];
+    for (uint32_t k = tid; k < npiv; k += nt) {
+        const int32_t g = gidv[q0[k]];
+        t.piv[k] = g;
+        if (ls.H) t.slots[pt][slot_of(colour_of(lw[q0[k]]), t.slot_bits)] = elect_tag(epoch) | (uint32_t)g;
+    }
     tail_sync();
     if (tid == 0) t.ctl->lcnt[lsel] = sh.lcnt;
 }
@@ -396,7 +469,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
             const uint32_t L = ctl->lcnt[r.lsel];
             if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {
-                const int b = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv);
+                const int b = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv, r.pt);
                 if (b < 0) {
                     status = 2;
                     break;
@@ -411,7 +484,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         if (local) {
             ok = tail_step(t, ls, sh, r);
         } else {
-            const GlobalSpace gs{&t, t.list[r.lsel], ctl->lcnt[r.lsel], t.list[r.lsel ^ 1]};
+            const GlobalSpace gs{{}, &t, t.list[r.lsel], ctl->lcnt[r.lsel], t.list[r.lsel ^ 1]};
             ok = tail_step(t, gs, sh, r);
         }
         if (!ok) {
@@ -419,7 +492,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
             break;
         }
     }
-    if (local) tail_writeback(t, ls, sh, r.lsel, r.npiv);
+    if (local) tail_writeback(t, ls, sh, r.lsel, r.npiv, r.pt, r.epoch);
     if (threadIdx.x == 0) {
         ctl->tail[0] = steps;
         ctl->tail[1] = status;
