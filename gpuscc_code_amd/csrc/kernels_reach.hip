@@ -441,9 +441,13 @@ __global__ void k_reach_topo(ReachArgs a) {
 // its BFS level changes only the number of levels.  What is left (and any
 // staging overflow) goes to the next level's queue.
 // ---------------------------------------------------------------------------
+constexpr uint32_t LOCAL_CAP = 512;   // items of one local round (the widest workgroup: one per work-item)
+constexpr int32_t HEAD_UNKNOWN = -2;  // hd[.].x of an item whose row head is not at hand (ELL entries are >= -1)
+
 struct LocalQ {
-    int32_t v[2 * SQ_CAP];
-    uint32_t rc[3][2];  // staging counts of the local rounds, rotating (see local_rounds)
+    int32_t v[2][LOCAL_CAP];  // the items of a round and of the next: vertex | direction << 31
+    int4 hd[2][LOCAL_CAP];    // their row heads, loaded with the claim that found them (complete rows only)
+    uint32_t rc[3];           // item counts, rotating (see local_rounds)
 };
 
 // The local rounds start after the last use of the items of a level, so the
@@ -548,14 +552,126 @@ __device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, 
 constexpr int32_t LOCAL_MAX_DEG = 64;       // longer rows leave the local rounds for the next level
 constexpr uint32_t LOCAL_ITEMS_PER_WG = 8;  // local rounds only when a level's frontier is this thin
 
-// Workgroup-local rounds on the visits staged by this workgroup: each round
-// moves the staged vertices into lq and expands them one per lane; rows longer
-// than LOCAL_MAX_DEG go to the next level's queue instead.  Two barriers per
-// round: the staging counts rotate over three slots -- round r reads slot r,
-// pushes into slot r + 1 and clears slot r + 2, which round r - 1 read before
-// the round's first barrier -- so no barrier is needed to clear a count before
-// the round's pushes.  A push past the buffers goes to the queue (st_push_at),
-// so the rounds need no flushes either.  All work-items call this.
+// a vertex found in a local round: into the next round's items, with its row
+// head when the row fits it, else (past the round's capacity) to the level's staging
+__device__ inline void lq_push(const ReachArgs &a, Stage &st, LocalQ &lq, uint32_t *out, int pn, int d, int32_t u,
+                               int4 h) {
+    const uint32_t i = atomicAdd(out, 1u);
+    if (i < LOCAL_CAP) {
+        lq.v[pn][i] = (int32_t)((uint32_t)u | ((uint32_t)d << 31));
+        lq.hd[pn][i] = h;
+    } else {
+        st_push(st, d, u, a);
+    }
+}
+
+// Expands a local item whose row fits its head nb.  The claims go out with the
+// loads of the targets' own heads and class bytes, so the next round starts
+// from them without another round trip: one dependent access per round when
+// every arc stays inside a colour (step one), two otherwise (target words, then
+// claims -- see expand_fit).
+__device__ inline void expand_fit_local(const ReachArgs &a, Stage &st, LocalQ &lq, uint32_t *out, int pn, int d,
+                                        int32_t v, uint32_t c, int4 nb, bool &found, unsigned long long &cx,
+                                        unsigned long long &ce, unsigned long long &cv) {
+    const uint32_t bit = dbit(d);
+    const int32_t u[4] = {nb.x, nb.y, nb.z, nb.w};
+    const uint32_t deg = (nb.x >= 0 ? 1u : 0u) + (nb.y >= 0 ? 1u : 0u) + (nb.z >= 0 ? 1u : 0u) +
+                         (nb.w >= 0 ? 1u : 0u);
+    if (deg >= a.huge_thr) {
+        push_huge(a, d, v);
+        found = true;
+        return;
+    }
+    cx++;
+    ce += deg;
+    const int4 *ell = dell(a, d);
+    const uint8_t *cls = ddcls(a, d);
+    // every access is issued for all four entries, an absent one at v itself
+    // (and its claim with an empty mask), with no branch between them: a branch
+    // would make the compiler wait for each access before the next
+    int32_t s[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) s[j] = u[j] >= 0 ? u[j] : v;
+    int4 hn[4];
+    uint32_t cn[4], old[4];
+    if (a.one_colour) {  // the claims first, then the heads: one round trip
+#pragma unroll
+        for (int j = 0; j < 4; j++) old[j] = atomicOr(&a.word[s[j]], u[j] >= 0 ? bit : 0u);
+#pragma unroll
+        for (int j = 0; j < 4; j++) hn[j] = ell[s[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) cn[j] = cls[s[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (u[j] < 0) old[j] = bit;
+    } else {  // heads and target words, then the claims: two
+        uint32_t wu[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) hn[j] = ell[s[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) cn[j] = cls[s[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) wu[j] = a.word[s[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool claim = u[j] >= 0 && !(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c;
+            old[j] = atomicOr(&a.word[s[j]], claim ? bit : 0u);
+            if (!claim) old[j] = bit;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (old[j] & (bit | ST_E)) continue;
+        found = true;
+        cv++;
+        int4 h = hn[j];
+        if (!(cn[j] & CLS_FITS)) h.x = HEAD_UNKNOWN;
+        lq_push(a, st, lq, out, pn, d, u[j], h);
+    }
+}
+
+// Serial expansion of a local item whose row does not fit its head (at most
+// LOCAL_MAX_DEG arcs); its finds go to the next round without heads.
+__device__ inline void expand_serial_local(const ReachArgs &a, Stage &st, LocalQ &lq, uint32_t *out, int pn, int d,
+                                           int32_t v, uint32_t c, bool &found, unsigned long long &cx,
+                                           unsigned long long &ce, unsigned long long &cv) {
+    const uint32_t bit = dbit(d);
+    const int32_t *row = drow(a, d), *col = dcol(a, d);
+    const int32_t e0 = row[v], e1 = row[v + 1];
+    cx++;
+    ce += (unsigned long long)(e1 - e0);
+    constexpr int A = 4;
+    for (int32_t e = e0; e < e1; e += A) {
+        int32_t u[A];
+        uint32_t wu[A], old[A];
+#pragma unroll
+        for (int j = 0; j < A; j++) u[j] = e + j < e1 ? col[e + j] : -1;
+#pragma unroll
+        for (int j = 0; j < A; j++) wu[j] = u[j] >= 0 ? a.word[u[j]] : ST_E;
+#pragma unroll
+        for (int j = 0; j < A; j++) {
+            old[j] = bit;
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+        }
+#pragma unroll
+        for (int j = 0; j < A; j++) {
+            if (old[j] & bit) continue;
+            found = true;
+            cv++;
+            lq_push(a, st, lq, out, pn, d, u[j], make_int4(HEAD_UNKNOWN, 0, 0, 0));
+        }
+    }
+}
+
+// Workgroup-local rounds on the visits staged by this workgroup: round 0
+// takes them from the staging buffers, every later round the vertices the
+// round before found (with their row heads, loaded alongside the claims that
+// found them); rows longer than LOCAL_MAX_DEG go to the next level's queue
+// instead.  One barrier per round: the items alternate between two buffers
+// and the counts rotate over three slots -- round r reads slot r, pushes into
+// slot r + 1 and clears slot r + 2, which round r - 1 read before round r's
+// barrier.  A find past the round's capacity is staged for the next level, as
+// are the items left when the rounds stop.  All work-items call this.
 __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, Stage &st, LocalQ &lq,
                                     bool &found, unsigned long long &cx, unsigned long long &ce,
                                     unsigned long long &cv) {
@@ -563,59 +679,74 @@ __device__ inline void local_rounds(const ReachArgs &a, uint32_t level_items, St
     // (high-diameter graphs) would leave most workgroups idle per level.
     if (a.iter < a.local_first || level_items > LOCAL_ITEMS_PER_WG * gridDim.x) return;
     __syncthreads();  // the level's pushes are staged; the items sharing lq's memory are done with
-    if (threadIdx.x < 2) {
-        lq.rc[0][threadIdx.x] = st.cnt[threadIdx.x];
-        lq.rc[1][threadIdx.x] = 0;
-        lq.rc[2][threadIdx.x] = 0;
+    const uint32_t s0 = st.cnt[0], s1 = st.cnt[1];
+    // block-uniform: nothing to expand, or the workgroup's own frontier is wider
+    // than one item per work-item -- a wide frontier is the next level's work
+    // (balanced over all workgroups)
+    if (s0 + s1 == 0 || s0 + s1 > a.nt) return;
+    for (uint32_t k = threadIdx.x; k < s0 + s1; k += a.nt) {
+        const int d = k < s0 ? 0 : 1;
+        lq.v[0][k] = (int32_t)((uint32_t)st.buf[d][d == 0 ? k : k - s0] | ((uint32_t)d << 31));
+        lq.hd[0][k].x = HEAD_UNKNOWN;
     }
-    int ci = 0;  // slot of this round's input counts
+    __syncthreads();  // st.cnt and st.buf are read: the rounds may stage again
+    if (threadIdx.x == 0) {
+        st.cnt[0] = st.cnt[1] = 0;
+        lq.rc[0] = s0 + s1;
+        lq.rc[1] = lq.rc[2] = 0;
+    }
+    int ci = 0, p = 0;  // slot of this round's count, buffer of its items
     for (int r = 0; r < a.local_rounds; r++) {
-        __syncthreads();  // the pushes of the round before are complete
-        uint32_t c0 = lq.rc[ci][0] < SQ_CAP ? lq.rc[ci][0] : SQ_CAP;
-        uint32_t c1 = lq.rc[ci][1] < SQ_CAP ? lq.rc[ci][1] : SQ_CAP;
-        // block-uniform (read after the barrier): stop when nothing is staged, or when
-        // the workgroup's own frontier outgrew one item per work-item -- a wide
-        // frontier is the next level's work (balanced over all workgroups), and its
-        // pushes would overflow the staging buffers one global atomic at a time
-        if (c0 + c1 == 0 || lq.rc[ci][0] + lq.rc[ci][1] > a.nt) break;
+        __syncthreads();  // the finds of the round before are complete
+        const uint32_t cnt = lq.rc[ci];
+        if (cnt == 0 || cnt > a.nt) break;  // block-uniform (read after the barrier)
         const int co = ci == 2 ? 0 : ci + 1, cz = co == 2 ? 0 : co + 1;
-        for (uint32_t k = threadIdx.x; k < c0; k += a.nt) lq.v[k] = st.buf[0][k];
-        for (uint32_t k = threadIdx.x; k < c1; k += a.nt) lq.v[SQ_CAP + k] = st.buf[1][k];
-        if (threadIdx.x < 2) lq.rc[cz][threadIdx.x] = 0;
-        __syncthreads();  // the buffers are copied: this round's pushes may overwrite them
-        uint32_t *out = lq.rc[co];
-        for (uint32_t k0 = 0; k0 < c0 + c1; k0 += a.nt) {
-            uint32_t k = k0 + threadIdx.x;
-            if (k < c0 + c1) {
-                int d = k < c0 ? 0 : 1;
-                int32_t v = d == 0 ? lq.v[k] : lq.v[SQ_CAP + (k - c0)];
+        if (threadIdx.x == 0) lq.rc[cz] = 0;
+        uint32_t *out = &lq.rc[co];
+        const uint32_t k = threadIdx.x;
+        if (k < cnt) {
+            const int32_t x = lq.v[p][k];
+            const int d = x < 0 ? 1 : 0;
+            const int32_t v = x & 0x7FFFFFFF;
+            int4 nb = lq.hd[p][k];
+            uint32_t c = 0;
+            bool fits = true;
+            if (nb.x == HEAD_UNKNOWN) {
                 // class byte, own word and row head in one round trip (the empty
                 // asm needs the head before the branch, so its load is not sunk
                 // into the branch behind the other two)
                 const uint32_t cls = ddcls(a, d)[v];
-                const uint32_t c = colour_of(a.word[v]);
-                const int4 nb = dell(a, d)[v];
+                c = colour_of(a.word[v]);
+                nb = dell(a, d)[v];
                 asm volatile("" ::"v"(nb.x));
-                if (cls & CLS_FITS) {
-                    if (a.one_colour) expand_fit<false, false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
-                    else expand_fit<false>(a, st, out, d, v, c, nb, found, cx, ce, cv);
+                fits = (cls & CLS_FITS) != 0;
+            } else if (!a.one_colour) {
+                c = colour_of(a.word[v]);
+            }
+            if (fits) {
+                expand_fit_local(a, st, lq, out, p ^ 1, d, v, c, nb, found, cx, ce, cv);
+            } else {
+                const int32_t *row = drow(a, d);
+                if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
+                    atomicAdd(&st.arcs[d], class_arcs(ddcls(a, d)[v]));
+                    uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
+                    doutq(a, d)[g] = v;
                 } else {
-                    const int32_t *row = drow(a, d);
-                    if (row[v + 1] - row[v] > LOCAL_MAX_DEG) {
-                        atomicAdd(&st.arcs[d], class_arcs(cls));
-                        uint32_t g = atomicAdd(&a.ctl->qsize[d][(a.iter + 1) & 3], 1u);
-                        doutq(a, d)[g] = v;
-                    } else {
-                        expand_serial<false>(a, st, out, d, v, c, found, cx, ce, cv);
-                    }
+                    expand_serial_local(a, st, lq, out, p ^ 1, d, v, c, found, cx, ce, cv);
                 }
             }
         }
         ci = co;
+        p ^= 1;
     }
-    // what the rounds leave staged is counted in slot ci; st_flush takes it from st.cnt
+    // the items of the last round read (or the finds of the last round run)
+    // are claimed but not expanded: the next level's work
     __syncthreads();
-    if (threadIdx.x < 2) st.cnt[threadIdx.x] = lq.rc[ci][threadIdx.x];
+    const uint32_t left = lq.rc[ci] < LOCAL_CAP ? lq.rc[ci] : LOCAL_CAP;
+    for (uint32_t k = threadIdx.x; k < left; k += a.nt) {
+        const int32_t x = lq.v[p][k];
+        st_push(st, x < 0 ? 1 : 0, x & 0x7FFFFFFF, a);
+    }
 }
 
 __global__ void k_reach_queue(ReachArgs a) {
