@@ -198,7 +198,7 @@ struct Items {
 
 template <bool PUSH>
 __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nitems, uint32_t total,
-                                    Stage *st, bool &found, unsigned long long &cv) {
+                                    Stage *st, bool &found, unsigned long long &cv, bool keep_last = false) {
     // EXP_ILP arcs per work-item and pass: their column ids, then their words,
     // then the claims of the candidates, each group of loads in flight together
     for (uint32_t base = 0; base < total; base += a.nt * EXP_ILP) {  // block-uniform trip count
@@ -232,7 +232,9 @@ __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nite
             cv++;
             if (PUSH) st_push_wave(*st, (int)(cd[j] >> 31), u[j], a);
         }
-        if (PUSH) st_maybe_flush(*st, a, EXP_ILP);
+        // room for the next pass (with 512 work-items one pass can fill the whole
+        // buffer); keep_last leaves the last pass's finds staged for the local rounds
+        if (PUSH && !(keep_last && base + a.nt * EXP_ILP >= total)) st_maybe_flush(*st, a, EXP_ILP);
     }
 }
 
@@ -793,6 +795,8 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
                                       unsigned long long &ce, unsigned long long &cv) {
     uint32_t chunk = (tot + gridDim.x - 1) / gridDim.x;
     uint32_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < tot ? b0 + chunk : tot;
+    // a level thin enough for the local rounds keeps its finds staged for them
+    const bool keep = a.local_rounds > 0 && a.iter >= a.local_first && tot <= LOCAL_ITEMS_PER_WG * gridDim.x;
     for (uint32_t base = b0; base < b1; base += a.nt) {
         uint32_t i = base + threadIdx.x;
         uint32_t deg = 0, cd = 0;
@@ -819,7 +823,7 @@ __device__ inline void queue_balanced(const ReachArgs &a, uint32_t qs0, uint32_t
         it.start[threadIdx.x] = start;
         it.cd[threadIdx.x] = cd;
         __syncthreads();
-        if (total > 0) expand_items<true>(a, it, a.nt, total, &st, found, cv);
+        if (total > 0) expand_items<true>(a, it, a.nt, total, &st, found, cv, keep);
         st_maybe_flush(st, a);
     }
 }
