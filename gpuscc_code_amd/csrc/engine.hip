@@ -66,6 +66,10 @@ struct scc_ctx {
     int adam_t = 0;
     int cur_step = 0;
     bool one_colour = false;  // scc_run's first FB step: every arc joins two vertices of one colour
+    uint32_t *d_vis = nullptr;  // visited bitmaps of the first step's reach, F then B (vis_words each)
+    int64_t vis_words = 0;
+    bool vis_mark = false;   // the next confirm marks its winners in the bitmaps (a run's first election)
+    bool vis_valid = false;  // this run cleared the bitmaps and marked its first pivots
     bool uploaded = false, labels_valid = false;
     bool label_in_root = false;  // the last run's labels are the root array itself (plain rule)
     int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
@@ -353,6 +357,15 @@ struct ReachChain {
         a.ell[0] = c->d_ell;
         a.ell[1] = c->d_ell + c->g.n;
         a.pull_cost = c->opts.hybrid_policy == 2 ? 1 : 0;
+        a.vis[0] = a.vis[1] = nullptr;
+        // the pulls test bitmap bits: batches only -- their pull levels scan many rows that
+        // find nothing (4,096 graphs at different depths), and the bitmap gathers stay in L2
+        // (batch 8.87 -> 8.62 ms); on single graphs the marks cost more than the pulls save
+        // (R-MAT s22 0.486 -> 0.51 ms, mesh 6.4 -> 6.9 ms)
+        if (sched == SCC_SCHED_H && c->one_colour && c->vis_valid && c->num_graphs > 0) {
+            a.vis[0] = c->d_vis;
+            a.vis[1] = c->d_vis + c->vis_words;
+        }
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
@@ -480,11 +493,14 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       recolour, 1, nullptr, c->d_ctl);
     }
+    uint32_t *vis = c->vis_mark ? c->d_vis : nullptr;
     if (lists)
         launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl, c->d_live[lout_sel],
-                       &c->d_ctl->lcnt[lout_sel]);
+                       &c->d_ctl->lcnt[lout_sel], vis, c->vis_words);
     else
-        launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl);
+        launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl, nullptr, nullptr, vis,
+                       c->vis_words);
+    c->vis_mark = false;
     ctl_queue(c, 64);  // the pivot list seeds both directions
     c->prev_tab ^= 1;
     if (!sync) return SCC_OK;
@@ -572,6 +588,9 @@ void free_graph(scc_ctx *c) {
         if (p) (void)hipFree(p);
         p = nullptr;
     }
+    if (c->d_vis) (void)hipFree(c->d_vis);
+    c->d_vis = nullptr;
+    c->vis_words = 0;
     c->lsel = c->lsel_prev = -1;
     for (auto &t : c->d_slots) {
         if (t) (void)hipFree(t);
@@ -807,6 +826,8 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipMalloc(&c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, hipMalloc(&c->d_hlive, 2 * std::max<size_t>((size_t)n, 1)));
     for (auto &p : c->d_live) HIPCHK(c, hipMalloc(&p, vb));
+    c->vis_words = ((int64_t)n + 63) / 64 * 2 + 2;
+    HIPCHK(c, hipMalloc(&c->d_vis, sizeof(uint32_t) * 2 * (size_t)c->vis_words));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
     HIPCHK(c, hipMalloc(&c->d_seedB, vb));
@@ -888,7 +909,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         c->pend_bits = c->pend_stage = 0;
         ctl_op(c, 512 | 1024 | 128 | 2 | 1);
         launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
-                    c->d_hlive, c->d_q[1], c->d_ctl);
+                    c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         auto tp = std::chrono::steady_clock::now();
@@ -902,7 +923,9 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         bool unsynced = c->h_ctl->live > 0;
         if (unsynced) {
             tp = std::chrono::steady_clock::now();
+            c->vis_mark = true;  // the first pivots seed the visited bitmaps k_init cleared
             CHK(do_elect(c, 0, &npiv, learned, false));  // first pivot: lowest live id (scc.cu:235-245)
+            c->vis_valid = true;
             st.ms_update += ms_since(tp);
         }
         bool wcc_done = false;
@@ -1110,6 +1133,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     };
     s = body();
     c->one_colour = false;
+    c->vis_mark = c->vis_valid = false;
     c->use_lists = false;  // phase entry points work on states of their own
     c->lsel = c->lsel_prev = -1;
     if (s != SCC_OK) return s;

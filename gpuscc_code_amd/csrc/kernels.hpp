@@ -57,6 +57,10 @@ struct ReachArgs {
     int one_colour;     // every arc joins two vertices of one colour (the first FB step of a run): the
                         // local rounds claim without reading the target's word first
     int instr;          // count work (expanded vertices, scanned arcs, visits)
+    uint32_t *vis[2];   // the first FB step of the hybrid schedule (else null): bit u of vis[d] is set
+                        // once u is reached in direction d (seeds by the first election, visits by every
+                        // claim), so the pull scans test a bit in an L2-resident bitmap instead of
+                        // gathering whole vertex words
     // trim levels
     int32_t *root;      // trimmed vertices become their own SCC
     uint32_t *stamp;    // per-vertex tag of the last trim level that checked it
@@ -88,8 +92,10 @@ struct Graph {  // device CSR + CSR^T
 // With hlive (k_head_live) the live vertices whose first out- or in-entry is not known to
 // stay live are appended to unk, their count added to ctl->qsize[0][0] (zeroed before):
 // the first level of trim1 checks only them.
+// vis (may be null): the two visited bitmaps (vis_words words each) are cleared.
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl);
+                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl,
+                 uint32_t *vis = nullptr, int64_t vis_words = 0);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
@@ -116,9 +122,11 @@ void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,
                    Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr,
                    int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr);
+// vis (may be null): the winners are marked in both visited bitmaps (the first election
+// of a run: the seeds of the first reach)
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin = nullptr,
-                    const uint32_t *lin_cnt = nullptr);
+                    const uint32_t *lin_cnt = nullptr, uint32_t *vis = nullptr, int64_t vis_words = 0);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl);
 void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl);

@@ -51,7 +51,7 @@ constexpr int INIT_V = 8;
 template <int VEC>
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
                        const uint8_t *cls_out, const uint8_t *cls_in, const uint8_t *hlive, int32_t *unk,
-                       Ctl *ctl) {
+                       Ctl *ctl, uint32_t *vis, int64_t vis_words) {
     constexpr int R = INIT_V / VEC;  // groups of VEC vertices per work-item and round
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t shw[16];
@@ -100,6 +100,10 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
                 nt += trim ? 1u : 0u;
                 if (!trim && !(((ho[q] >> (8 * j)) & 0xFFu) && ((hi[q] >> (8 * j)) & 0xFFu)))
                     um |= 1u << (q * VEC + j);
+            }
+            if (vis && (v0 & 31) == 0) {  // the visited bitmaps: one word per 32 vertices
+                vis[v0 >> 5] = 0u;
+                vis[vis_words + (v0 >> 5)] = 0u;
             }
             if (VEC == 4) {
                 *reinterpret_cast<uint4 *>(word + v0) = make_uint4(w[0], w[1 % VEC], w[2 % VEC], w[3 % VEC]);
@@ -436,7 +440,8 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
 constexpr int CONF_V = 16;  // vertices per work-item per round
 
 __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *__restrict__ cur,
-                          int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
+                          int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt,
+                          uint32_t *vis, int64_t vis_words) {
     __shared__ uint32_t shw[16];
     __shared__ uint32_t sbase;
     if (__atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
@@ -462,6 +467,10 @@ __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *_
                 word[v] = make_word(c, 19);
                 won |= 1u << k;
                 atomicMin(&ctl->first_piv, v);
+                if (vis) {  // a pivot is reached both ways
+                    atomicOr(&vis[v >> 5], 1u << (v & 31));
+                    atomicOr(&vis[vis_words + (v >> 5)], 1u << (v & 31));
+                }
             }
         }
         // the round's winners join the pivot list: one prefix sum, one atomicAdd
@@ -711,13 +720,14 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 #define GRID(l) dim3((l).grid), dim3((l).block), 0, (l).stream
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
-                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl) {
+                 int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl, uint32_t *vis,
+                 int64_t vis_words) {
     if ((n & 3) == 0)
         hipLaunchKernelGGL(k_init<4>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words);
     else
         hipLaunchKernelGGL(k_init<1>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
@@ -749,8 +759,9 @@ void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                        score, ctl, lin, lin_cnt, lout, lout_cnt);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
-                    int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
-    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl, lin, lin_cnt);
+                    int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt,
+                    uint32_t *vis, int64_t vis_words) {
+    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl, lin, lin_cnt, vis, vis_words);
 }
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl) {
     hipLaunchKernelGGL(k_count_state, GRID(l), word, n, what, ctl);

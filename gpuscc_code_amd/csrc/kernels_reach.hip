@@ -47,6 +47,20 @@ __device__ inline int32_t *doutq(const ReachArgs &a, int d) { return dpick(d, a.
 __device__ inline int32_t *dhout(const ReachArgs &a, int d) { return dpick(d, a.hout[0], a.hout[1]); }
 __device__ inline const int32_t *dhin(const ReachArgs &a, int d) { return dpick(d, a.hin[0], a.hin[1]); }
 
+__device__ inline uint32_t *dvis(const ReachArgs &a, int d) { return dpick(d, a.vis[0], a.vis[1]); }
+
+// u is being claimed in direction d from a reached vertex: its bit in the
+// visited bitmap (first FB step of the hybrid schedule only), set with the claim
+// itself (no wait for the claim's result).  A claim without a look at the
+// target's word may mark an eliminated vertex; in the first step those are
+// trim-1's, and a trimmed vertex with a live in-neighbour has no live
+// out-neighbour (else trim-1 would have kept it), so no pull ever meets such a
+// mark: the marks are the reached live vertices as far as the pulls can tell.
+__device__ inline void mark_vis(const ReachArgs &a, int d, int32_t u) {
+    uint32_t *bm = dvis(a, d);
+    if (bm) atomicOr(&bm[u >> 5], 1u << (u & 31));
+}
+
 __device__ inline bool try_claim(uint32_t *word, int32_t u, uint32_t c, uint32_t bit) {
     uint32_t wu = word[u];
     if ((wu & (bit | ST_E)) != 0 || colour_of(wu) != c) return false;
@@ -222,8 +236,10 @@ __device__ inline void expand_items(const ReachArgs &a, Items &it, uint32_t nite
         for (int j = 0; j < EXP_ILP; j++) {
             uint32_t bit = dbit((int)(cd[j] >> 31));
             old[j] = bit;
-            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == (cd[j] & 0x7FFFFFFFu))
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == (cd[j] & 0x7FFFFFFFu)) {
                 old[j] = atomicOr(&a.word[u[j]], bit);
+                mark_vis(a, (int)(cd[j] >> 31), u[j]);  // with the claim: u is reached by it or another
+            }
         }
 #pragma unroll
         for (int j = 0; j < EXP_ILP; j++) {
@@ -323,6 +339,7 @@ __device__ inline void huge_phase(const ReachArgs &a, Items &it, Stage *st, bool
                     if (e < cend && try_claim(a.word, col[e], cc, bit)) {
                         found = true;
                         cv++;
+                        mark_vis(a, d, col[e]);
                         if (PUSH) st_push_wave(*st, d, col[e], a);
                     }
                     if (PUSH) st_maybe_flush(*st, a);
@@ -489,7 +506,10 @@ __device__ inline void expand_serial(const ReachArgs &a, Stage &st, uint32_t *cn
 #pragma unroll
         for (int j = 0; j < A; j++) {
             old[j] = bit;
-            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) {
+                old[j] = atomicOr(&a.word[u[j]], bit);
+                mark_vis(a, d, u[j]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < A; j++) {
@@ -535,11 +555,17 @@ __device__ inline void expand_fit(const ReachArgs &a, Stage &st, uint32_t *cnt, 
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             old[j] = bit;
-            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) {
+                old[j] = atomicOr(&a.word[u[j]], bit);
+                mark_vis(a, d, u[j]);
+            }
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) old[j] = u[j] >= 0 ? atomicOr(&a.word[u[j]], bit) : bit;
+        for (int j = 0; j < 4; j++) {
+            old[j] = u[j] >= 0 ? atomicOr(&a.word[u[j]], bit) : bit;
+            if (u[j] >= 0) mark_vis(a, d, u[j]);
+        }
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -596,9 +622,14 @@ __device__ inline void expand_fit_local(const ReachArgs &a, Stage &st, LocalQ &l
     for (int j = 0; j < 4; j++) s[j] = u[j] >= 0 ? u[j] : v;
     int4 hn[4];
     uint32_t cn[4], old[4];
-    if (a.one_colour) {  // the claims first, then the heads: one round trip
+    if (a.one_colour) {  // the claims (and their marks) first, then the heads: one round trip
 #pragma unroll
         for (int j = 0; j < 4; j++) old[j] = atomicOr(&a.word[s[j]], u[j] >= 0 ? bit : 0u);
+        uint32_t *bm = dvis(a, d);
+        if (bm) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) atomicOr(&bm[s[j] >> 5], u[j] >= 0 ? 1u << (s[j] & 31) : 0u);
+        }
 #pragma unroll
         for (int j = 0; j < 4; j++) hn[j] = ell[s[j]];
 #pragma unroll
@@ -618,6 +649,7 @@ __device__ inline void expand_fit_local(const ReachArgs &a, Stage &st, LocalQ &l
         for (int j = 0; j < 4; j++) {
             const bool claim = u[j] >= 0 && !(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c;
             old[j] = atomicOr(&a.word[s[j]], claim ? bit : 0u);
+            if (claim) mark_vis(a, d, u[j]);
             if (!claim) old[j] = bit;
         }
     }
@@ -653,7 +685,10 @@ __device__ inline void expand_serial_local(const ReachArgs &a, Stage &st, LocalQ
 #pragma unroll
         for (int j = 0; j < A; j++) {
             old[j] = bit;
-            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) old[j] = atomicOr(&a.word[u[j]], bit);
+            if (!(wu[j] & (bit | ST_E)) && colour_of(wu[j]) == c) {
+                old[j] = atomicOr(&a.word[u[j]], bit);
+                mark_vis(a, d, u[j]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < A; j++) {
@@ -933,6 +968,40 @@ __device__ inline uint32_t scan_rows(const uint32_t *word, const int32_t *const 
     return hit;
 }
 
+// The same scans against the visited bitmaps (bm[t]: the bitmap of row t's
+// direction): a neighbour hits when its bit is set.  Valid in the first FB step,
+// where every arc joins two vertices of one colour and an eliminated vertex is
+// never marked, so the bit is exactly "reached in that direction and live".
+template <int T>
+__device__ inline uint32_t scan_rows_vis(const uint32_t *const (&bm)[T], const int32_t *const (&col)[T],
+                                         int32_t (&e)[T], const int32_t (&e1)[T], const int32_t (&u0)[T],
+                                         const int32_t (&self)[T], uint32_t act, unsigned long long &ce,
+                                         uint32_t &steps) {
+    uint32_t hit = 0;
+    int s = 0;
+    for (; act; s++) {
+        uint32_t bw[T];
+        int32_t u[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) u[t] = ((act >> t) & 1u) ? (s == 0 ? u0[t] : col[t][e[t]]) : -1;
+#pragma unroll
+        for (int t = 0; t < T; t++) bw[t] = u[t] >= 0 ? bm[t][u[t] >> 5] : 0u;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            if (!((act >> t) & 1u)) continue;
+            ce++;
+            if (u[t] != self[t] && ((bw[t] >> (u[t] & 31)) & 1u)) {
+                hit |= 1u << t;
+                act &= ~(1u << t);
+            } else if (s > 0 ? ++e[t] >= e1[t] : e[t] >= e1[t]) {
+                act &= ~(1u << t);
+            }
+        }
+    }
+    steps = (uint32_t)s > steps ? (uint32_t)s : steps;
+    return hit;
+}
+
 constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
 
 // Queues the vertices one sweep round claimed (pull visits, trims): bit k of
@@ -1058,7 +1127,15 @@ __global__ void k_reach_hybrid(ReachArgs a) {
                 }
 #pragma unroll
                 for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
-                uint32_t hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
+                uint32_t hit;
+                if (a.vis[0]) {
+                    const uint32_t *bm[T];
+#pragma unroll
+                    for (int t = 0; t < T; t++) bm[t] = dvis(a, t / SCAN_ILP);
+                    hit = scan_rows_vis<T>(bm, col, e, e1, u0, self, act, ce, smax);
+                } else {
+                    hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
+                }
                 // claims: the class bytes load with the atomics
                 uint32_t cls[T];
 #pragma unroll
@@ -1078,6 +1155,25 @@ __global__ void k_reach_hybrid(ReachArgs a) {
             if (claimed[0] | claimed[1]) {
                 found = true;
                 cv += (unsigned long long)(__popc(claimed[0]) + __popc(claimed[1]));
+            }
+            if (a.vis[0]) {
+                // the round's claims into the visited bitmaps: vertex base + k * nt + lane
+                // of a wave is one of 64 consecutive vertices starting at a multiple of 64
+                // (beg and nt are), two bitmap words, one atomic each
+                const uint32_t lane = threadIdx.x & (WAVE - 1);
+                const int64_t w0 = base + (int64_t)(threadIdx.x - lane);
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    uint32_t *bm = dvis(a, d);
+#pragma unroll
+                    for (int k = 0; k < PULL_V; k++) {
+                        const unsigned long long m = __ballot((claimed[d] >> k) & 1u);
+                        if (m == 0) continue;  // wave-uniform
+                        const int64_t v0 = w0 + (int64_t)k * a.nt;
+                        if (lane == 0 && (uint32_t)m) atomicOr(&bm[v0 >> 5], (uint32_t)m);
+                        if (lane == 32 && (uint32_t)(m >> 32)) atomicOr(&bm[(v0 >> 5) + 1], (uint32_t)(m >> 32));
+                    }
+                }
             }
             commit_round(a, base, claimed, arcs, it.shw, st.base, sh);
         }
