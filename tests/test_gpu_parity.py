@@ -371,6 +371,42 @@ def test_batched_graphs(ctx, sched):
     ctx.set_batch(None)
 
 
+def test_batch_pull_bitmaps(ctx):
+    """The first FB step of a batch pulls against visited bitmaps (bits set by
+    the first election and by every claim): forced pull levels (small
+    thresholds) on batches with trimmed vertices, full state against the oracle
+    for every variant, and the same with pulls off."""
+    try:
+        for g in (scc.gen_batch(48, 64, 2048, seed=4), scc.gen_batch(200, 16, 600, 1.0, 3.0, 0.1, 0.3, seed=9)):
+            for opts in (scc.alg_opt(hybrid_threshold=16, hybrid_arc_div=1000),
+                         scc.alg_opt(hybrid_threshold=16, hybrid_arc_div=1000, local_rounds=-1),
+                         scc.alg_opt(hybrid_threshold=1 << 30, hybrid_arc_div=-1)):
+                ctx.set_opts(opts)
+                ctx.upload(g)
+                ctx.set_batch(g.graph_offsets)
+                for variant in (0, 1, 2):
+                    ctx.profile(4)  # level log: the pulls happen where they are forced
+                    lab, st = ctx.run("h", variant)
+                    lv = ctx.profile_levels(reset=True)
+                    ctx.profile(0)
+                    pulled = lv[:, scc.LEVEL_FIELDS.index("pulled")][lv[:, 0] == 0]
+                    assert (pulled.sum() > 0) == (opts.hybrid_threshold == 16)
+                    ref, _ = oracle.tarjan(g.row, g.col)
+                    assert np.array_equal(lab, ref)
+                    off = g.graph_offsets
+                    gid = np.repeat(np.arange(len(off) - 1), np.diff(off)).astype(np.uint32)
+                    o = oracle.fb(g.row, g.col, g.rowT, g.colT, variant=variant, init_colour=gid + 1,
+                                  slot_bits=default_slot_bits(g.n), hash_kind=1)
+                    words, roots = ctx.get_state()
+                    colour, state = scc.split_word(words)
+                    assert st["steps"] == o["steps"]
+                    assert np.array_equal(colour, o["colour"]) and np.array_equal(roots, o["root"])
+                    assert np.array_equal(state & 0x1C, o["state"] & 0x1C)
+                ctx.set_batch(None)
+    finally:
+        ctx.set_opts(scc.alg_opt())
+
+
 def test_batch_rejects_crossing_arcs(ctx):
     g = graph_from_arcs(6, [(0, 1), (1, 0), (2, 4), (4, 3)])
     ctx.upload(g)
