@@ -83,7 +83,7 @@ struct scc_ctx {
     std::vector<double> level_log;  // flag 4: SCC_LEVEL_FIELDS per level
     // speculative steps: reach and trim chain lengths of step k in the last run
     std::vector<int> spec_reach, spec_trim;
-    hipEvent_t spec_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t spec_ev[6] = {};  // a speculative step's phase bounds; [4], [5]: trim-1's chain
     // reach chains captured as HIP graphs: key (schedule, levels, thresholds) -> executable
     struct GraphKey {
         int sched, levels;
@@ -912,22 +912,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
-        auto tp = std::chrono::steady_clock::now();
-        CHK(do_trim(c, &it, true));  // the rest of trim1 (scc.cu:232)
-        st.trim_iters += it + 1;
-        st.ms_trim += ms_since(tp);
-        // The device counts live vertices: after trim-1 every live vertex still
-        // has its initial colour, so live vertices mean at least one pivot and
-        // the first election needs no wait of its own (its count arrives with
-        // the first reach's).
-        bool unsynced = c->h_ctl->live > 0;
-        if (unsynced) {
-            tp = std::chrono::steady_clock::now();
-            c->vis_mark = true;  // the first pivots seed the visited bitmaps k_init cleared
-            CHK(do_elect(c, 0, &npiv, learned, false));  // first pivot: lowest live id (scc.cu:235-245)
-            c->vis_valid = true;
-            st.ms_update += ms_since(tp);
-        }
         bool wcc_done = false;
         const int64_t one_pct = n / 100;
         // schedules that switch once n/100 vertices are FB-eliminated: the
@@ -941,6 +925,45 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // waits.  The results are those of the waiting path.
         const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !learned &&
                              !by_elim && c->prof_flags == 0;
+        auto tp = std::chrono::steady_clock::now();
+        // With speculative steps the rest of trim-1 goes out without a wait as well:
+        // its chain (as long as trim-1's in the last run), a guard (stage 3), the first
+        // election and the first step, whose wait tells whether trim-1 converged.
+        // The first step must then take the speculative path: no tag wrap in the
+        // first two elections.
+        bool spec_t1 = spec_ok && ((c->epoch + 2) & 0xFFFFu) >= 2u;
+        const int Lt1 = std::max(1, std::min(c->guess[3], 1024));
+        bool unsynced = false;
+        if (spec_t1) {
+            if (!c->spec_ev[0])
+                for (auto &e : c->spec_ev) HIPCHK(c, hipEventCreate(&e));
+            HIPCHK(c, hipEventRecord(c->spec_ev[4], c->stream));
+            TrimChain t1(c, false);
+            t1.after_init = true;  // k_init's reset started the chain
+            for (int i = 0; i < Lt1; i++) t1(i, nullptr, nullptr);
+            HIPCHK(c, hipEventRecord(c->spec_ev[5], c->stream));
+            ctl_guard(c, Lt1 - 1, 3);  // goes out with the election's first reset
+            c->vis_mark = true;        // the first pivots seed the visited bitmaps k_init cleared
+            CHK(do_elect(c, 0, &npiv, learned, false));
+            c->vis_valid = true;
+            unsynced = true;  // live vertices are not known yet: the first step's wait tells
+        } else {
+            CHK(do_trim(c, &it, true));  // the rest of trim1 (scc.cu:232)
+            st.trim_iters += it + 1;
+            st.ms_trim += ms_since(tp);
+            // The device counts live vertices: after trim-1 every live vertex still
+            // has its initial colour, so live vertices mean at least one pivot and
+            // the first election needs no wait of its own (its count arrives with
+            // the first reach's).
+            unsynced = c->h_ctl->live > 0;
+            if (unsynced) {
+                tp = std::chrono::steady_clock::now();
+                c->vis_mark = true;  // the first pivots seed the visited bitmaps k_init cleared
+                CHK(do_elect(c, 0, &npiv, learned, false));  // first pivot: lowest live id (scc.cu:235-245)
+                c->vis_valid = true;
+                st.ms_update += ms_since(tp);
+            }
+        }
         while (unsynced || npiv > 0) {
             // The tail: once few vertices are live, the remaining steps run in one
             // workgroup and one launch (k_fb_tail); it stops early only before an
@@ -1026,6 +1049,40 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 HIPCHK(c, hipEventRecord(c->spec_ev[3], c->stream));
                 CHK(read_ctl(c));
                 const Ctl h = *c->h_ctl;
+                if (spec_t1) {  // the first step also tells how trim-1 ended
+                    spec_t1 = false;
+                    if (h.abort == 3) {
+                        // trim-1 needs more levels, and nothing after its chain ran: undo the
+                        // two elections' bookkeeping, finish trim-1 with waits, start over
+                        c->epoch -= 2;  // prev_tab flipped twice: as before
+                        c->lsel = c->lsel_prev = -1;
+                        c->cur_step = 0;  // the first step's reach starts its local rounds late again
+                        st.steps--;
+                        ctl_op(c, 512);
+                        CHK(do_trim(c, &it, true, Lt1));  // sets the next guess
+                        st.trim_iters += it + 1;
+                        unsynced = c->h_ctl->live > 0;
+                        if (unsynced) {
+                            c->vis_mark = true;
+                            CHK(do_elect(c, 0, &npiv, learned, false));
+                            c->vis_valid = true;
+                        }
+                        continue;
+                    }
+                    st.trim_iters += (int64_t)h.depth[2] + 1;
+                    c->guess[3] = std::max(1, (int)h.depth[2]);
+                    float t1ms = 0;
+                    HIPCHK(c, hipEventElapsedTime(&t1ms, c->spec_ev[4], c->spec_ev[5]));
+                    st.ms_trim += t1ms;
+                    if ((h.abort == 0 ? h.npiv_prev : h.npiv) == 0) {
+                        // trim-1 left no live vertex: no pivot, and the step found nothing
+                        ctl_op(c, 512);
+                        st.steps--;
+                        unsynced = false;
+                        npiv = 0;
+                        continue;
+                    }
+                }
                 const bool elected = h.abort == 0;
                 // the election before this step's reach (the step's own election,
                 // when it ran, moved those counts to the _prev fields)

@@ -51,7 +51,7 @@ struct Ctl {
     uint32_t pcount;       // vertices holding P (class representatives: #SCC at the end)
     uint32_t abort;        // speculative step: chain stage that did not converge (0 = none);
                            // every gated kernel exits at once while it is set
-    uint32_t depth[2];     // speculative step: levels of its reach chain and trim chain
+    uint32_t depth[3];     // speculative step: levels of its reach chain and trim chain; of trim-1's chain
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     uint32_t creach[2][4];  // vertices reached so far by each direction (ring: level iter writes [iter & 3])

@@ -431,7 +431,7 @@ def test_speculative_steps_match_waiting_path(ctx, sched):
     for graphs in (0, 1):  # reach chains launched level by level / replayed as captured HIP graphs
         ctx.set_opts(scc.alg_opt(reach_graphs=graphs))
         for g in (demo_graph(), scc.gen_rmat(13, 8, seed=21), scc.gen_mesh(96, 0.6, seed=5), cycle(3000),
-                  path_dag(500)):
+                  path_dag(500), cycle_with_tails()):
             ref, _ = oracle.tarjan(g.row, g.col)
             ctx.upload(g)
             ctx.profile(2)  # event timing: every chain waits
@@ -446,6 +446,32 @@ def test_speculative_steps_match_waiting_path(ctx, sched):
                 assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}
             ctx.profile_read(reset=True)
     ctx.set_opts(scc.alg_opt())
+
+
+def cycle_with_tails(k=50, t=300):
+    """A k-cycle with a t-vertex path into it and one out of it: trim-1 peels the
+    paths one vertex per level (about t levels) and leaves the cycle live."""
+    arcs = [(i, (i + 1) % k) for i in range(k)]
+    arcs += [(k + i, k + i + 1) for i in range(t - 1)] + [(k + t - 1, 0)]
+    arcs += [(1, k + t)] + [(k + t + i, k + t + i + 1) for i in range(t - 1)]
+    return graph_from_arcs(k + 2 * t, arcs)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_trim1_resumed_after_short_guess(ctx, variant):
+    """With speculative steps the rest of trim-1 goes out with the first step, as
+    long as the last run's trim-1 (a guess after an upload); a deeper trim-1
+    stops everything behind its guard and the host resumes it with waits.  Full
+    state against the oracle when stopped, when the depth is known, and when
+    trim-1 leaves nothing live."""
+    ctx.upload(cycle(100))
+    ctx.run("h")  # a shallow trim-1: the next guess is short
+    _, st1 = check_against_oracle(ctx, cycle_with_tails(), "h", variant)  # stopped and resumed
+    _, st2 = check_against_oracle(ctx, cycle_with_tails(), "h", variant)  # depth known from the run before
+    if variant == 0:  # the speculative path: the resumed trim-1 costs waits of its own
+        assert st1["host_syncs"] > st2["host_syncs"]
+    lab, st = check_against_oracle(ctx, path_dag(400), "h", variant)  # no live vertex after trim-1
+    assert st["steps"] == 0 and st["first_pivot"] == -1
 
 
 def test_pull_cost_policy(ctx):
