@@ -513,7 +513,7 @@ def test_wide_level_workgroups(ctx, sched):
         ctx.set_opts(scc.alg_opt())
 
 
-@pytest.mark.parametrize("sched", ["q", "d", "h"])
+@pytest.mark.parametrize("sched", SCHEDS)
 def test_tail_settings(ctx, sched):
     """The one-workgroup tail (tail_live) ends where the multi-kernel steps end:
     the same words, roots, steps and pivots as with it off, and the oracle's
