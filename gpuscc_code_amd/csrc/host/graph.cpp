@@ -266,10 +266,13 @@ static scc_status load_binary(const std::string &buf, scc_host_graph *g, std::st
     const int32_t *w = (const int32_t *)(buf.data() + kBinHeader);
     std::vector<int32_t> row(w, w + n + 1), col(w + n + 1, w + n + 1 + m64);
     if (row[0] != 0 || row[n] != m64) return bad("binary graph: row offsets do not span the arcs");
+    // every offset in [0, m] and never decreasing, checked in full before any
+    // column is read through an offset
+    for (int32_t v = 0; v < n; v++)
+        if (row[v + 1] < row[v] || row[v + 1] > m64) return bad("binary graph: row offsets out of range or decreasing");
     std::atomic<bool> fine{true};
     parallel_for(n, [&](int64_t b, int64_t e, int) {
         for (int64_t v = b; v < e && fine.load(std::memory_order_relaxed); v++) {
-            if (row[v + 1] < row[v]) { fine = false; break; }
             for (int32_t i = row[v]; i < row[v + 1]; i++)
                 if (col[i] < 0 || col[i] >= n || (i > row[v] && col[i] <= col[i - 1])) { fine = false; break; }
         }

@@ -307,6 +307,12 @@ def test_binary_files_are_checked(tmp_path):
     bad = bytearray(raw)
     bad[64:68] = (1).to_bytes(4, "little")                 # row[0] != 0
     corrupt("row0.bin", bad)
+    # interior offsets outside [0, m]: far past the arcs, and negative at the
+    # start of a later thread's share of the rows
+    for name, v, val in (("far.bin", 1, 10 ** 6), ("neg.bin", g.n // 2, -5), ("past.bin", g.n - 1, g.m + 1)):
+        bad = bytearray(raw)
+        bad[64 + 4 * v:68 + 4 * v] = val.to_bytes(4, "little", signed=True)
+        corrupt(name, bad)
 
 
 def test_cli_writes_a_binary_graph(tmp_path):
