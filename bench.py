@@ -4,13 +4,21 @@
 A "step" is one whole SCC decomposition (scc_run: trim, pivots, forward and
 backward reachability, colour updates, canonical labels) of the workload graph,
 which is already resident in HBM when the timed region starts.  Default
-workload: R-MAT scale 22 (4,194,304 vertices, 33,554,432 arcs drawn, edge
-factor 8, a=0.57 b=0.19 c=0.19, seed 2, duplicates and self-loops removed).
+workload: R-MAT scale 24, edge factor 16, a=0.57 b=0.19 c=0.19, seed 5
+(16,777,216 vertices, 268,435,456 arcs drawn, duplicates and self-loops
+removed), the graph the learned-versus-plain pivot comparison is quoted on.
 
 value = MTEPS = arcs after dedup / seconds per decomposition.
-roofline: algorithmic bytes of the reachability level kernels (from device
-work counters in an untimed counting run) over their summed duration, timed
-with HIP events on the library's stream during the timed region.
+cold_ms: the first decomposition after the upload (no chain lengths
+remembered from an earlier run on the graph).
+roofline: algorithmic bytes of the reachability levels per decomposition
+(per unit: one state byte per vertex per sweep level, 13 bytes per expanded
+vertex, 9 per scanned arc, 9 per new visit on the queue schedules, 1 on the
+sweeps; counted by device counters in an untimed counting run) over the
+reach time of the timed decompositions, taken with HIP events on the
+library's stream around each reach chain.
+traffic: HBM bytes per launch of the level kernel from rocprofv3
+FETCH_SIZE / WRITE_SIZE runs of this workload (profiles/), when measured.
 cpu_baseline: the oracle's single-threaded Tarjan on the same graph.
 learned_pivot: the same decompositions with the trained MLP pivot scorer
 (time, FB steps, plain time / learned time, labels bit-exact).
@@ -38,11 +46,24 @@ def log(*a):
 
 
 def algorithmic_bytes(sched, n, prof):
-    """Bytes the reachability levels must move, per the data layout:
-    a sweep level reads every 4-byte vertex word; an expanded vertex reads its
-    row-offset pair (8) and writes its word (4); a scanned arc reads its column
-    id (4) and gathers the target's word (4); a new visit is one 4-byte atomic,
-    plus a 4-byte queue push and (on queue levels) a 4-byte queue read."""
+    """Bytes the reachability levels must move per decomposition, per unit of
+    work as the reference lays its data out (a u8 state and a u32 colour per
+    vertex): a sweep level reads one state byte per vertex (n + 1); an expanded
+    vertex reads its row-offset pair, its colour and writes its state (13); a
+    scanned arc reads its column id, the target's state and colour (9); a new
+    visit writes one state byte, plus a queue push and pop (8) on the schedules
+    with frontier queues."""
+    S, X, E, V = prof["sweep_levels"], prof["expanded"], prof["edges"], prof["visited"]
+    per_visit = 1 if sched in ("q", "t") else 9
+    return S * (n + 1) + X * 13 + E * 9 + V * per_visit
+
+
+def layout_bytes(sched, n, prof):
+    """The same work counted in this build's layout (colour and state in one
+    4-byte word per vertex): a sweep reads 4 bytes per vertex, an expanded
+    vertex 12 (offsets 8, word 4), a scanned arc 8 (column id, target word), a
+    new visit one 4-byte atomic plus 4 for its queue push (and 4 for the pop on
+    the queue schedules)."""
     S, X, E, V = prof["sweep_levels"], prof["expanded"], prof["edges"], prof["visited"]
     if sched in ("q", "t"):
         return S * 4 * n + X * 12 + E * 8 + V * 4
@@ -52,14 +73,19 @@ def algorithmic_bytes(sched, n, prof):
 
 
 def traffic_for(args):
-    """HBM bytes per launch of the level kernel measured by rocprofv3 counters
-    (profiles/), when they were measured for this exact configuration."""
+    """HBM bytes per decomposition moved by the level kernel, from rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this exact configuration (profiles/), or
+    None when it was not measured."""
     if not args.traffic or not os.path.exists(args.traffic):
         return None
-    t = json.load(open(args.traffic))
-    if (args.workload, args.schedule, args.scale, args.edge_factor, args.seed) != ("rmat", "h", 22, 8, 2):
-        return None
-    return t["bytes_per_launch"]
+    key = {"workload": args.workload, "schedule": args.schedule, "variant": args.variant,
+           "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed}
+    if args.workload != "rmat":
+        key = {k: key[k] for k in ("workload", "schedule", "variant")}
+    for e in json.load(open(args.traffic)).get("entries", []):
+        if all(e.get(k) == v for k, v in key.items()):
+            return e
+    return None
 
 
 def write_levels(path, workload, rows):
@@ -73,7 +99,20 @@ def write_levels(path, workload, rows):
                               for i, x in enumerate([r[0], r[1], r[2] * 1e3] + list(r[3:]))) + "\n")
 
 
-def cpu_tarjan_baseline(g, reps=3):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_tarjan(g, reps):
+    """The oracle's Tarjan on the graph, pinned to one core: (labels, stats,
+    DFS seconds of each repetition; the timed region is the DFS only)."""
     import oracle
     try:
         old = os.sched_getaffinity(0)
@@ -82,41 +121,40 @@ def cpu_tarjan_baseline(g, reps=3):
         old = None
     times = []
     try:
-        for _ in range(reps):
-            t, _cnt = oracle.tarjan_time(g.row, g.col)
-            times.append(t)
+        for _ in range(max(reps, 1)):
+            ref, tst = oracle.tarjan(g.row, g.col)
+            times.append(tst["seconds"])
     finally:
         if old is not None:
             os.sched_setaffinity(0, old)
-    t = statistics.median(times)
-    return {"value": round(g.m / t / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
-            "sample": f"oracle Tarjan (single thread, pinned) on the whole graph, median of {reps} "
-                      f"runs, {t * 1e3:.1f} ms each; host has {os.cpu_count()} CPUs"}
+    return ref, tst, times
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--schedule", default="h", choices=list("qtdlhcb"),
                     help="q t d l h as the reference's -a; c = COMBINE, b = BEST")
     ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2])
     ap.add_argument("--workload", default="rmat", choices=["rmat", "mesh", "batch"],
-                    help="rmat: R-MAT (default, scale 22); mesh: 2048x2048 grid; "
+                    help="rmat: R-MAT (default: scale 24, edge factor 16, seed 5); mesh: 2048x2048 grid; "
                          "batch: 4096 uneven ER graphs with planted cycles")
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--edge-factor", type=int, default=8)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-permute", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3,
+                    help="Tarjan repetitions of the CPU baseline (median reported)")
     ap.add_argument("--sequential", action="store_true",
                     help="forward reach to its fixpoint, then backward (the reference's order)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the calibration copy kernel over MIB MiB (for traffic counters)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_reach_hybrid_rmat22_traffic.json"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_reach_hybrid_traffic.json"),
                     metavar="JSON",
-                    help="measured HBM bytes per launch of the level kernel (from rocprofv3 counters)")
+                    help="measured HBM bytes of the level kernel (from rocprofv3 counters)")
     ap.add_argument("--levels", default=None, metavar="FILE",
                     help="also write the per-level log of one decomposition (kernel time, frontier, "
                          "push/pull, work) to FILE")
@@ -131,6 +169,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-pivot leg (same workload and options, pivots elected by the "
                          "trained MLP scorer, gpuscc_code_amd/pivot_mlp.npy)")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the labels check against the oracle's Tarjan (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -159,7 +199,9 @@ def main():
                                      hybrid_threshold=args.pull_threshold,
                                      hybrid_arc_div=args.pull_arc_div,
                                      grid_blocks=args.grid_blocks))
+    t0 = time.time()
     ctx.upload(g)
+    log(f"uploaded in {time.time() - t0:.1f}s")
     num_graphs = 1
     if args.workload == "batch":
         ctx.set_batch(g.graph_offsets)
@@ -177,51 +219,78 @@ def main():
 
     if args.calibrate_copy:
         ctx.profile_copy(args.calibrate_copy << 20)
+    runs = 0  # decompositions this process runs (for per-decomposition counter totals)
 
-    # untimed counting run: algorithmic work of the reachability levels
+    # cold: the first decomposition after the upload (and option change), with no
+    # chain lengths remembered from an earlier run on this graph
+    t0 = time.perf_counter()
+    _, st_cold = ctx.run(args.schedule, args.variant, labels=False)
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    runs += 1
+
+    # untimed counting run: algorithmic work of the reachability levels (the
+    # counters are cleared first: the cold run's resumed chains add to them)
     ctx.profile(1)
+    ctx.profile_read(reset=True)
     ctx.run(args.schedule, args.variant, labels=False)
     work = ctx.profile_read(reset=True)
     ctx.profile(0)
+    runs += 1
     for _ in range(args.warmup):
         ctx.run(args.schedule, args.variant, labels=False)
+        runs += 1
 
-    # timed region: K decompositions, nothing instrumented
+    # timed region: K decompositions, nothing instrumented (the reach chains of
+    # the speculative steps are bracketed by HIP events on the library's stream,
+    # which give ms_fwd)
     stats = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         _, st = ctx.run(args.schedule, args.variant, labels=False)
         stats.append(st)
     t_total = time.perf_counter() - t_start
+    runs += args.steps
 
-    # kernel durations for the roofline: K more decompositions with every level
-    # kernel bracketed by HIP events on its stream
+    # cross-check of the level-kernel time: K more decompositions with every level
+    # kernel bracketed by its own pair of HIP events
     ctx.profile(2)
     ctx.profile_read(reset=True)
     for _ in range(args.steps):
         ctx.run(args.schedule, args.variant, labels=False)
     prof = ctx.profile_read(reset=True)
     ctx.profile(0)
+    runs += args.steps
 
     ms_per_step = t_total * 1e3 / args.steps
     mteps = g.m / (ms_per_step * 1e3)
 
     # learned-pivot leg: the same decomposition with pivots elected by the trained
-    # scorer (its static features are computed by the warm-up run, once per graph)
+    # scorer
     learned = None
     if not args.no_learned:
         from gpuscc_code_amd import learned as learned_mod
         ctx.mlp_set(learned_mod.load_params())
         for _ in range(max(args.warmup, 1)):
             _, st_l = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED, labels=False)
+            runs += 1
+        st_ls = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             _, st_l = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED, labels=False)
+            st_ls.append(st_l)
         ms_l = (time.perf_counter() - t0) * 1e3 / args.steps
-        lab_l, _ = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED)
+        runs += args.steps
+        lab_l = None
+        if not args.no_check:
+            lab_l, _ = ctx.run(args.schedule, args.variant, pivot=scc.PIVOT_LEARNED)
+            runs += 1
         learned = {"ms_per_step": round(ms_l, 3), "value": round(g.m / (ms_l * 1e3), 2), "unit": "MTEPS",
                    "ms_per_graph": round(ms_l / num_graphs, 6), "fb_steps": st_l["steps"],
                    "plain_fb_steps": stats[-1]["steps"],
+                   "fwd_depths": st_l["fwd_depths"], "plain_fwd_depths": stats[-1]["fwd_depths"],
+                   "host_syncs": st_l["host_syncs"],
+                   "phase_ms": {k: round(statistics.mean(s[k] for s in st_ls), 3)
+                                for k in ("ms_fwd", "ms_bwd", "ms_trim", "ms_update")},
                    "plain_over_learned": round(ms_per_step / ms_l, 3), "labels": lab_l}
 
     if args.levels:  # one more decomposition, levels launched one at a time
@@ -229,48 +298,76 @@ def main():
         ctx.run(args.schedule, args.variant, labels=False)
         write_levels(args.levels, wl, ctx.profile_levels())
         ctx.profile(0)
+        runs += 1
 
-    # correctness of the measured configuration
-    labels, st_check = ctx.run(args.schedule, args.variant)
-    import oracle
-    ref, tst = oracle.tarjan(g.row, g.col)
-    bit_exact = bool(np.array_equal(labels, ref))
-    if learned is not None:
-        learned["bit_exact"] = bool(np.array_equal(learned.pop("labels"), ref))
+    # correctness of the measured configuration, and the CPU baseline
+    bit_exact, st_check, tst, cpu = None, None, None, None
+    if not args.no_check:
+        labels, st_check = ctx.run(args.schedule, args.variant)
+        runs += 1
+        reps = 0 if args.no_cpu else args.cpu_reps
+        ref, tst, cpu_times = cpu_tarjan(g, reps)
+        bit_exact = bool(np.array_equal(labels, ref))
+        if learned is not None:
+            learned["bit_exact"] = bool(np.array_equal(learned.pop("labels"), ref))
+        if not args.no_cpu:
+            t = statistics.median(cpu_times)
+            cpu = {"value": round(g.m / t / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
+                   "ms": round(t * 1e3, 1),
+                   "sample": f"the oracle's iterative Tarjan (single thread, pinned to one core) on the "
+                             f"whole graph, DFS only, median of {len(cpu_times)} runs of "
+                             f"{', '.join(f'{x * 1e3:.0f}' for x in cpu_times)} ms",
+                   "host_cpu": cpu_model(), "host_threads": os.cpu_count()}
+    elif learned is not None:
+        learned.pop("labels")
 
+    # roofline of the reachability level kernel
     per_run_bytes = algorithmic_bytes(args.schedule, g.n, work)
-    launches = prof["reach_launches"] / args.steps
-    kernel_ms = prof["reach_kernel_ms"] / args.steps
-    achieved = per_run_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    reach_ms = statistics.mean(s["ms_fwd"] + s["ms_bwd"] for s in stats)  # timed region, HIP events
+    last = stats[-1]
+    launches_live = max(last["fwd_depths"], 1)  # level kernels of the timed decompositions' reach chains
+    achieved = per_run_bytes / (reach_ms * 1e-3) / 1e9 if reach_ms > 0 else 0.0
+    launches_ev = prof["reach_launches"] / args.steps
+    kernel_ms_ev = prof["reach_kernel_ms"] / args.steps
+    tr = traffic_for(args)
+    traffic = None
+    if tr is not None:
+        traffic = int(tr["bytes_per_decomposition"] / launches_live)
     roofline = {
         "bound": "hbm", "kernel": KERNEL_OF[args.schedule],
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_for(args),
-        "bytes_per_launch": int(per_run_bytes / max(launches, 1)),
-        "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 2),
-        "launches_per_step": launches, "kernel_ms_per_step": round(kernel_ms, 3),
+        "traffic": traffic,
+        "bytes_per_launch": int(per_run_bytes / launches_live),
+        "avg_launch_us": round(reach_ms * 1e3 / launches_live, 2),
+        "launches_per_step": launches_live, "reach_ms_per_step": round(reach_ms, 4),
+        "bytes_per_step": int(per_run_bytes),
+        "layout_bytes_per_step": int(layout_bytes(args.schedule, g.n, work)),
+        "traffic_over_algorithmic": None if traffic is None else round(traffic * launches_live / per_run_bytes, 2),
+        "events_per_kernel": {"launches_per_step": launches_ev, "kernel_ms_per_step": round(kernel_ms_ev, 4),
+                              "avg_launch_us": round(kernel_ms_ev * 1e3 / max(launches_ev, 1), 2)},
         "work_per_step": {k: work[k] for k in ("sweep_levels", "queue_levels", "expanded", "edges",
                                                "visited")},
     }
-    cpu = None if args.no_cpu else cpu_tarjan_baseline(g)
-    last = stats[-1]
     out = {
         "metric": METRIC, "value": round(mteps, 2), "unit": "MTEPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic",
         "ms_per_graph": round(ms_per_step / num_graphs, 6), "graphs_per_s": round(num_graphs * 1e3 / ms_per_step, 1),
+        "cold_ms": round(cold_ms, 3), "cold_host_syncs": st_cold["host_syncs"],
         "config": {"workload": wl, "graphs": num_graphs,
                    "n": g.n, "m": g.m, "schedule": args.schedule,
                    "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
                    "reach": "sequential" if args.sequential else "fused",
                    "tuned": tuned_key, "opts": opts},
-        "bit_exact": bit_exact, "num_scc": st_check["num_scc"], "biggest_scc": tst["biggest"],
+        "bit_exact": bit_exact, "num_scc": None if st_check is None else st_check["num_scc"],
+        "biggest_scc": None if tst is None else tst["biggest"],
         "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],
         "trim_iters": last["trim_iters"], "host_syncs": last["host_syncs"],
         "phase_ms": {k: round(statistics.mean(s[k] for s in stats), 3)
                      for k in ("ms_fwd", "ms_bwd", "ms_trim", "ms_update")},
+        "decompositions_run": runs,
         "learned_pivot": learned, "roofline": roofline, "cpu_baseline": cpu,
     }
     ctx.close()

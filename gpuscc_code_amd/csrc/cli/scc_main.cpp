@@ -33,6 +33,8 @@ static void usage() {
                 "\t-f input directed graph (edge list, MatrixMarket, or a binary graph written by -o)\n"
                 "\t-o also write the loaded graph as a binary graph file (loads without parsing text)\n"
                 "\t-j also print one JSON line per run\n"
+                "\t-H 1: -a h switches as the reference's HYBRID (t until n/100 vertices are FB-eliminated,\n"
+                "\t   then q); default 0: each level pushes or pulls by its frontier's size\n"
                 "\t-V write trivial.txt, scc_dist.txt, parallel_scc.txt, state_range_root.txt\n");
 }
 
@@ -141,7 +143,7 @@ int main(int argc, char **argv) {
     scc_opts opts;
     std::memset(&opts, 0, sizeof(opts));
     int c;
-    while ((c = getopt(argc, argv, "htjVf:a:w:b:v:o:")) != -1) {
+    while ((c = getopt(argc, argv, "htjVf:a:w:b:v:o:H:")) != -1) {
         switch (c) {
         case 'h': break;  // header flag: accepted, unused (as in load.cpp)
         case 't': tarjan = true; break;
@@ -167,6 +169,7 @@ int main(int argc, char **argv) {
         case 'b': opts.block_size = (int32_t)std::strtol(optarg, nullptr, 0); break;
         case 'v': variant = std::atoi(optarg); break;
         case 'o': save_to = optarg; break;  // binary CSR file for later runs (no reference counterpart)
+        case 'H': opts.hybrid_policy = (int32_t)std::atoi(optarg); break;  // 1: the reference's HYBRID rule
         default: usage(); return 1;
         }
     }

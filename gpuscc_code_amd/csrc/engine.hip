@@ -66,6 +66,7 @@ struct scc_ctx {
     int adam_t = 0;
     int cur_step = 0;
     bool one_colour = false;  // scc_run's first FB step: every arc joins two vertices of one colour
+    bool best_step = false;   // BEST's first FB step: direction-optimising reach (alpha 15, beta 18)
     uint32_t *d_vis = nullptr;  // visited bitmaps of the first step's reach, F then B (vis_words each)
     int64_t vis_words = 0;
     bool vis_mark = false;   // the next confirm marks its winners in the bitmaps (a run's first election)
@@ -74,6 +75,8 @@ struct scc_ctx {
     bool label_in_root = false;  // the last run's labels are the root array itself (plain rule)
     int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
     int64_t syncs = 0;
+    int fail_alloc_at = 0;   // fault injection: the nth device allocation of the next upload fails
+    int alloc_count = 0;
     // control-block work queued for the next k_ctl_reset launch (see ctl_op)
     int pend_bits = 0, pend_last = 0, pend_stage = 0;
     int instr = 0;                  // profile flag 1: device work counters
@@ -112,6 +115,17 @@ scc_status fail(scc_ctx *c, scc_status s, const std::string &what) {
 scc_status hip_fail(scc_ctx *c, hipError_t e, const char *what) {
     scc_status s = (e == hipErrorOutOfMemory) ? SCC_E_OOM : SCC_E_HIP;
     return fail(c, s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A device allocation of scc_upload; with fault injection armed (scc_test_fail_alloc)
+// the chosen one reports out of memory as hipMalloc would.
+template <class T>
+hipError_t dev_alloc(scc_ctx *c, T **p, size_t bytes) {
+    if (c->fail_alloc_at > 0 && ++c->alloc_count == c->fail_alloc_at) {
+        c->fail_alloc_at = 0;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc((void **)p, bytes);
 }
 
 #define HIPCHK(ctx, x)                                  \
@@ -357,6 +371,12 @@ struct ReachChain {
         a.ell[0] = c->d_ell;
         a.ell[1] = c->d_ell + c->g.n;
         a.pull_cost = c->opts.hybrid_policy == 2 ? 1 : 0;
+        if (c->best_step) {  // reach_b.cu:540,645: alpha = 15 against the unexplored arcs, beta = 18
+            a.pull_cost = 2;
+            a.alpha = 15;
+            a.pull_arcs = (unsigned long long)c->g.m;
+            a.pull_thr = (uint32_t)std::max<int64_t>(1, c->g.n / 18);
+        }
         a.vis[0] = a.vis[1] = nullptr;
         // the pulls test bitmap bits: batches only -- their pull levels scan many rows that
         // find nothing (4,096 graphs at different depths), and the bitmap gathers stay in L2
@@ -775,6 +795,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_graph(c);
+    c->alloc_count = 0;
     // validate rows and build the transpose on the host when it is not given
     std::vector<int32_t> hrT, hcT;
     for (int32_t v = 0; v < n; v++)
@@ -810,40 +831,40 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     colT = hub_colT.data();
     size_t nb = sizeof(int32_t) * ((size_t)n + 1), mb = sizeof(int32_t) * std::max<size_t>((size_t)m, 1);
     size_t vb = sizeof(int32_t) * std::max<size_t>((size_t)n, 1);
-    HIPCHK(c, hipMalloc(&c->d_row, nb));
-    HIPCHK(c, hipMalloc(&c->d_col, mb));
-    HIPCHK(c, hipMalloc(&c->d_rowT, nb));
-    HIPCHK(c, hipMalloc(&c->d_colT, mb));
-    HIPCHK(c, hipMalloc(&c->d_word, vb));
-    HIPCHK(c, hipMalloc(&c->d_root, vb));
-    HIPCHK(c, hipMalloc(&c->d_label, vb));
-    HIPCHK(c, hipMalloc(&c->d_tmp, vb));
-    for (auto &q : c->d_q) HIPCHK(c, hipMalloc(&q, vb));
-    for (auto &q : c->d_huge) HIPCHK(c, hipMalloc(&q, vb));
-    HIPCHK(c, hipMalloc(&c->d_stamp, vb));
-    HIPCHK(c, hipMalloc(&c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
-    HIPCHK(c, hipMalloc(&c->d_head, 2 * vb));
-    HIPCHK(c, hipMalloc(&c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
-    HIPCHK(c, hipMalloc(&c->d_hlive, 2 * std::max<size_t>((size_t)n, 1)));
-    for (auto &p : c->d_live) HIPCHK(c, hipMalloc(&p, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_row, nb));
+    HIPCHK(c, dev_alloc(c, &c->d_col, mb));
+    HIPCHK(c, dev_alloc(c, &c->d_rowT, nb));
+    HIPCHK(c, dev_alloc(c, &c->d_colT, mb));
+    HIPCHK(c, dev_alloc(c, &c->d_word, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_root, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_label, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_tmp, vb));
+    for (auto &q : c->d_q) HIPCHK(c, dev_alloc(c, &q, vb));
+    for (auto &q : c->d_huge) HIPCHK(c, dev_alloc(c, &q, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_stamp, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_dcls, 2 * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_head, 2 * vb));
+    HIPCHK(c, dev_alloc(c, &c->d_ell, 2 * sizeof(int4) * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_hlive, 2 * std::max<size_t>((size_t)n, 1)));
+    for (auto &p : c->d_live) HIPCHK(c, dev_alloc(c, &p, vb));
     c->vis_words = ((int64_t)n + 63) / 64 * 2 + 2;
-    HIPCHK(c, hipMalloc(&c->d_vis, sizeof(uint32_t) * 2 * (size_t)c->vis_words));
+    HIPCHK(c, dev_alloc(c, &c->d_vis, sizeof(uint32_t) * 2 * (size_t)c->vis_words));
     HIPCHK(c, hipMemsetAsync(c->d_stamp, 0, vb, c->stream));
     c->trim_tag = 0;
-    HIPCHK(c, hipMalloc(&c->d_seedB, vb));
-    HIPCHK(c, hipMalloc(&c->d_piv, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_seedB, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_piv, vb));
     // election table: about n slots (at least 2^10, at most 2^22), or the option
     int sb = 10;
     while (sb < 22 && ((int64_t)1 << sb) < (int64_t)n) sb++;
     if (c->opts.slot_bits > 0) sb = std::min(24, std::max(1, (int)c->opts.slot_bits));
     c->slot_bits = sb;
     size_t sbytes = sizeof(unsigned long long) << sb;
-    HIPCHK(c, hipMalloc(&c->d_slots[0], sbytes));
-    HIPCHK(c, hipMalloc(&c->d_slots[1], sbytes));
+    HIPCHK(c, dev_alloc(c, &c->d_slots[0], sbytes));
+    HIPCHK(c, dev_alloc(c, &c->d_slots[1], sbytes));
     // scorer buffers (used by the learned pivot rule; allocated here so runs never allocate)
-    HIPCHK(c, hipMalloc(&c->d_stat, sizeof(float4) * std::max<size_t>((size_t)n, 1)));
-    HIPCHK(c, hipMalloc(&c->d_cnt, sizeof(uint32_t) << sb));
-    HIPCHK(c, hipMalloc(&c->d_score, sizeof(uint16_t) * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_stat, sizeof(float4) * std::max<size_t>((size_t)n, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_cnt, sizeof(uint32_t) << sb));
+    HIPCHK(c, dev_alloc(c, &c->d_score, sizeof(uint16_t) * std::max<size_t>((size_t)n, 1)));
     c->stat_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->d_row, row, nb, hipMemcpyHostToDevice, c->stream));
     if (m > 0) HIPCHK(c, hipMemcpyAsync(c->d_col, col, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
@@ -878,15 +899,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return fail(c, SCC_E_STATE, "learned pivot rule needs parameters (scc_mlp_set_params or scc_mlp_train)");
     const bool learned = pivot_rule == SCC_PIVOT_LEARNED;
     HIPCHK(c, hipSetDevice(c->device));
-    struct Restore {  // BEST changes the hybrid thresholds for this run only
-        scc_ctx *c;
-        scc_opts o;
-        ~Restore() { c->opts = o; }
-    } restore{c, c->opts};
-    if (schedule == SCC_SCHED_B) {  // alpha = 15, beta = 18 (reach_b.cu:540,645) unless set
-        if (c->opts.hybrid_arc_div == 0) c->opts.hybrid_arc_div = 15;
-        if (c->opts.hybrid_threshold == 0) c->opts.hybrid_threshold = (int32_t)std::max<int64_t>(1024, c->g.n / 18);
-    }
     scc_stats st;
     std::memset(&st, 0, sizeof(st));
     st.first_pivot = -1;
@@ -915,8 +927,11 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         bool wcc_done = false;
         const int64_t one_pct = n / 100;
         // schedules that switch once n/100 vertices are FB-eliminated: the
-        // reference's HYBRID (t, then q; scc.cu:493-563) and COMBINE (l, then q)
-        const bool by_elim = (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1) || schedule == SCC_SCHED_C;
+        // reference's HYBRID (t, then q; scc.cu:493-563), COMBINE (l, then q;
+        // scc.cu:565-638) and BEST (the direction-optimising reach in step 1, t in
+        // later steps, then q; scc.cu:640-700, reach_b.cu:374-725)
+        const bool by_elim = (schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1) || schedule == SCC_SCHED_C ||
+                             schedule == SCC_SCHED_B;
         // Speculative steps: reach chain, trim chain and election go out without
         // host waits between them, each chain as long as the same step's chain
         // was in the last run.  A guard after each chain sets ctl->abort when the
@@ -1016,8 +1031,10 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             }
             st.steps++;
             c->one_colour = st.steps == 1;  // no recolouring yet: every arc stays inside a colour
-            int sched = schedule == SCC_SCHED_B ? SCC_SCHED_H : schedule;
+            int sched = schedule;
             if (by_elim) sched = ne < one_pct ? (schedule == SCC_SCHED_C ? SCC_SCHED_L : SCC_SCHED_T) : SCC_SCHED_Q;
+            c->best_step = schedule == SCC_SCHED_B && st.steps == 1;
+            if (c->best_step) sched = SCC_SCHED_H;
             int64_t d = 0;
             tp = std::chrono::steady_clock::now();
             const size_t k = (size_t)st.steps - 1;
@@ -1190,6 +1207,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     };
     s = body();
     c->one_colour = false;
+    c->best_step = false;
     c->vis_mark = c->vis_valid = false;
     c->use_lists = false;  // phase entry points work on states of their own
     c->lsel = c->lsel_prev = -1;
@@ -1522,6 +1540,24 @@ scc_status scc_phase_wcc(scc_ctx *c, int64_t *iterations, int64_t *pivots) {
     if (c->g.n > 0) CHK(do_wcc(c, &it, &np));
     if (iterations) *iterations = it;
     if (pivots) *pivots = np;
+    return SCC_OK;
+}
+
+scc_status scc_phase_unset_pivots(scc_ctx *c) {
+    if (!c) return SCC_E_INPUT;
+    CHK(detach_labels(c));
+    if (!c->uploaded) return fail(c, SCC_E_STATE, "no graph");
+    if (c->g.n == 0) return SCC_OK;
+    launch_unset_pivots(launch_for(c), c->d_word, c->g.n, c->d_ctl);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SCC_OK;
+}
+
+scc_status scc_test_fail_alloc(scc_ctx *c, int32_t nth) {
+    if (!c || nth < 0) return SCC_E_INPUT;
+    c->fail_alloc_at = nth;
+    c->alloc_count = 0;
     return SCC_OK;
 }
 

@@ -55,6 +55,9 @@ struct Ctl {
     unsigned long long count;     // count reductions (FB-eliminated, P bits, pairs)
     unsigned long long qarcs[2][4];  // estimated arcs of the queued vertices, same rings as qsize
     uint32_t creach[2][4];  // vertices reached so far by each direction (ring: level iter writes [iter & 3])
+    unsigned long long carcs[2][4];  // direction-optimising policy: estimated arcs of the frontiers each
+                                     // direction has pushed so far (same ring)
+    uint32_t pmask[4];      // directions each level pulled (bit 0 F, bit 1 B; ring: level iter writes [iter & 3])
     unsigned long long st_expanded, st_edges, st_visited;  // instrumented counters
     uint32_t st_maxscan[4];  // instrumented: longest serial row scan (arcs) of one work-item,
                              // ring: level iter writes [iter & 3]

@@ -187,8 +187,9 @@ static void zero_graph(scc_host_graph *g) { std::memset(g, 0, sizeof(*g)); }
 
 scc_status csr_from_arcs(int32_t n, int64_t m, const int32_t *src, const int32_t *dst,
                          int drop_self, scc_host_graph *g) {
-    if (!g || n < 0 || m < 0 || m >= (int64_t)INT32_MAX || (m > 0 && (!src || !dst)))
-        return SCC_E_INPUT;
+    if (!g || n < 0 || m < 0) return SCC_E_INPUT;
+    if (m >= (int64_t)INT32_MAX) return SCC_E_OVERFLOW;  // beyond int32 CSR offsets
+    if (m > 0 && (!src || !dst)) return SCC_E_INPUT;
     zero_graph(g);
     std::vector<int32_t> row, col;
     scc_status s = bucket_rows(n, m, src, dst, drop_self, row, col);
@@ -258,8 +259,12 @@ static scc_status load_binary(const std::string &buf, scc_host_graph *g, std::st
     std::memcpy(&ng, h + 36, 4);
     std::memcpy(st4, h + 40, 16);
     std::memcpy(&avg, h + 56, 8);
-    if (n64 < 0 || n64 >= INT32_MAX - 1 || m64 < 0 || m64 >= INT32_MAX || ng < 0 || ng > n64 + 1)
-        return bad("binary graph: sizes out of range");
+    if (n64 < 0 || m64 < 0 || ng < 0) return bad("binary graph: negative sizes");
+    if (n64 >= INT32_MAX - 1 || m64 >= INT32_MAX) {
+        if (err) *err = "binary graph: sizes beyond int32 CSR";
+        return SCC_E_OVERFLOW;
+    }
+    if (ng > n64 + 1) return bad("binary graph: more graphs than vertices");
     const int32_t n = (int32_t)n64;
     const size_t words = (size_t)n + 1 + (size_t)m64 + (ng > 0 ? (size_t)ng + 1 : 0);
     if (buf.size() != kBinHeader + 4 * words) return bad("binary graph: file size does not match its header");
@@ -488,8 +493,9 @@ scc_status load_file(const char *path, scc_host_graph *g, std::string *err) {
     } else if (std::sscanf(line.c_str(), "%lld %lld", &m_rows, &nnz) != 2) {
         return bad("bad header line", SCC_E_INPUT);
     }
-    if (m_rows < 0 || m_rows >= INT32_MAX || nnz < 0 || nnz >= INT32_MAX / 2) {
-        return bad("sizes out of range", SCC_E_OVERFLOW);
+    if (m_rows < 0 || nnz < 0) { return bad("negative sizes", SCC_E_INPUT); }
+    if (m_rows >= INT32_MAX || nnz >= INT32_MAX / 2) {
+        return bad("sizes beyond int32 CSR", SCC_E_OVERFLOW);
     }
     int32_t n = (int32_t)m_rows;
     std::vector<int32_t> src, dst;
@@ -572,7 +578,8 @@ scc_status gen_rmat(int scale, int ef, double a, double b, double c, uint64_t se
 }
 
 scc_status gen_mesh(int side, double p, uint64_t seed, scc_host_graph *g) {
-    if (!g || side < 1 || (int64_t)side * side >= INT32_MAX || p < 0 || p > 1) return SCC_E_INPUT;
+    if (!g || side < 1 || p < 0 || p > 1) return SCC_E_INPUT;
+    if ((int64_t)side * side >= INT32_MAX) return SCC_E_OVERFLOW;
     int64_t n = (int64_t)side * side;
     int64_t horiz = (int64_t)side * (side - 1);
     int64_t L = 2 * horiz;

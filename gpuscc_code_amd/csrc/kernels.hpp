@@ -43,7 +43,12 @@ struct ReachArgs {
     uint32_t dirs;
     uint32_t pull_thr;  // hybrid: a frontier of at least this many vertices pulls
     unsigned long long pull_arcs;  // hybrid: ... or one with at least this many (estimated) arcs
-    int pull_cost;                 // hybrid: pull when frontier arcs >= 2 x (live - reached) instead
+    int pull_cost;                 // hybrid: 1 = pull when frontier arcs >= 2 x (live - reached) instead;
+                                   // 2 = direction-optimising (Beamer): a pushing direction pulls once its
+                                   // frontier's arcs exceed (pull_arcs - arcs it pushed so far) / alpha, a
+                                   // pulling one keeps pulling while its frontier grows or holds more than
+                                   // pull_thr vertices
+    uint32_t alpha;                // pull_cost 2: the divisor of the unexplored arcs
     const uint8_t *dcls[2];  // per vertex and direction: bit length of the degree | CLS_FITS | 0x80 (self-loop)
     const int32_t *head[2];  // per vertex and direction: first entry of the row (-1: empty row), read
                              // in place of col[row[v]] by the scans that stop at the first match

@@ -254,7 +254,9 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
             for (int i = 1; i < 4; i++) ctl->qsize[d][i] = 0;
             for (int i = 0; i < 4; i++) ctl->qarcs[d][i] = 0;
             for (int i = 0; i < 4; i++) ctl->creach[d][i] = 0;
+            for (int i = 0; i < 4; i++) ctl->carcs[d][i] = 0;
         }
+        for (int i = 0; i < 4; i++) ctl->pmask[i] = 0;
     }
     if (what & 4) {
         ctl->npiv_prev = ctl->npiv;

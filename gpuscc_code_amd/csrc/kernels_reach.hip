@@ -907,6 +907,29 @@ __device__ inline void pull_dirs(const ReachArgs &a, uint32_t &qs0, uint32_t &qs
         pull1 = qs1 > 0 && (qs1 >= a.pull_thr || a.ctl->qarcs[1][a.iter & 3] >= a.pull_arcs);
         return;
     }
+    if (a.pull_cost == 2) {
+        // Direction-optimising switch (the BEST schedule's first step, reach_b.cu:540-596
+        // and :645-700 with alpha = 15, beta = 18): top-down until the frontier's arcs
+        // exceed the arcs not yet explored top-down over alpha; bottom-up while the
+        // frontier grows or holds more than n / beta vertices.  Every input below was
+        // written by an earlier level, so all workgroups take the same decision.
+        bool p[2];
+        for (int d = 0; d < 2; d++) {
+            const uint32_t qs = d == 0 ? qs0 : qs1;
+            const int pi = (a.iter - 1) & 3;
+            const bool was_pull = a.iter > 0 && ((a.ctl->pmask[pi] >> d) & 1u);
+            if (was_pull) {
+                p[d] = qs > 0 && (qs >= a.ctl->qsize[d][pi] || qs > a.pull_thr);
+            } else {
+                const unsigned long long done = a.iter > 0 ? a.ctl->carcs[d][pi] : 0ull;
+                const unsigned long long todo = a.pull_arcs > done ? a.pull_arcs - done : 0ull;
+                p[d] = qs > 0 && a.ctl->qarcs[d][a.iter & 3] * a.alpha > todo;
+            }
+        }
+        pull0 = p[0];
+        pull1 = p[1];
+        return;
+    }
     // Cost model: a push level costs about the arcs of its frontier, a pull
     // level about the live vertices the direction has not reached (most of
     // them scan a short row; in a batch of many graphs most never find a
@@ -1055,10 +1078,17 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     unsigned long long cx = 0, ce = 0, cv = 0;
     uint32_t smax = 0;  // longest row scan of this work-item (instrumented)
     if (blockIdx.x == 0 && threadIdx.x < 2) {
-        if (threadIdx.x == 0) a.ctl->pull_mask = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
+        const uint32_t pm = (pull0 ? 1u : 0u) | (pull1 ? 2u : 0u);
+        if (threadIdx.x == 0) {
+            a.ctl->pull_mask = pm;
+            a.ctl->pmask[a.iter & 3] = pm;
+        }
         const int d = (int)threadIdx.x;  // vertices reached through this level's input
-        a.ctl->creach[d][a.iter & 3] =
-            (a.iter > 0 ? a.ctl->creach[d][(a.iter - 1) & 3] : 0u) + (d == 0 ? qs0 : qs1);
+        const int pi = (a.iter - 1) & 3;
+        a.ctl->creach[d][a.iter & 3] = (a.iter > 0 ? a.ctl->creach[d][pi] : 0u) + (d == 0 ? qs0 : qs1);
+        // arcs explored top-down: the pushed frontiers' estimated arcs
+        a.ctl->carcs[d][a.iter & 3] = (a.iter > 0 ? a.ctl->carcs[d][pi] : 0ull) +
+                                      (((pm >> d) & 1u) ? 0ull : a.ctl->qarcs[d][a.iter & 3]);
     }
     huge_phase<true>(a, it, &st, found, cx, ce, cv);
     st_flush(st, a);

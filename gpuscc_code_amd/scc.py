@@ -278,6 +278,14 @@ class Context:
     def recolour(self):
         self._chk(lib().scc_phase_recolour(self._h))
 
+    def unset_pivots(self):
+        """unsetPivots (scc_routines.cu:10-27): state 19 -> 0."""
+        self._chk(lib().scc_phase_unset_pivots(self._h))
+
+    def fail_alloc(self, nth):
+        """Fault injection: the nth device allocation of the next upload fails (0 disarms)."""
+        self._chk(lib().scc_test_fail_alloc(self._h, int(nth)))
+
     def features(self, idx, step=0):
         idx = np.ascontiguousarray(idx, np.int32)
         X = np.empty((max(len(idx), 1), MLP_FEATURES), np.float32)

@@ -40,10 +40,16 @@ typedef enum {
     SCC_SCHED_T = 1,  /* topology sweep with workgroup edge balancing (reach_t.cu) */
     SCC_SCHED_D = 2,  /* data-driven frontier queues (reach_d.cu) */
     SCC_SCHED_L = 3,  /* frontier queues with load-balanced edge expansion (reach_l.cu) */
-    SCC_SCHED_H = 4,  /* hybrid: switches schedule by frontier size (scc.cu:493-563) */
+    SCC_SCHED_H = 4,  /* hybrid: hybrid_policy 0 (default) pushes or pulls each level and direction by
+                         its frontier's size; hybrid_policy 1 is the reference's HYBRID rule
+                         (scc.cu:493-563, switch :877-891): t until n/100 vertices are
+                         FB-eliminated, then q */
     SCC_SCHED_C = 5,  /* COMBINE: l until n/100 vertices are FB-eliminated, then q (scc.cu:565-638) */
-    SCC_SCHED_B = 6   /* BEST: direction-optimising reach (reach_b.cu:374-725): the hybrid level
-                         kernel pulling at frontier arcs >= m/15 or frontier vertices >= n/18 */
+    SCC_SCHED_B = 6   /* BEST (scc.cu:640-700, reach_b.cu:374-725): in the first FB step a
+                         direction-optimising reach (top-down until the frontier's arcs exceed the
+                         unexplored arcs / 15, bottom-up while the frontier grows or holds more
+                         than n/18 vertices); later steps t until n/100 vertices are
+                         FB-eliminated, then q */
 } scc_schedule;
 
 /* The three binaries of Makefile:12-19. */
@@ -186,6 +192,9 @@ scc_status scc_phase_trim2(scc_ctx *ctx, int64_t *pairs);
 scc_status scc_phase_wcc(scc_ctx *ctx, int64_t *iterations, int64_t *pivots);
 /* countElimmed (scc.cu:42-51). */
 scc_status scc_phase_count_elimmed(scc_ctx *ctx, int64_t *count);
+/* unsetPivots (scc_routines.cu:10-27): every vertex in the pivot state 19
+ * (P|F|B) goes back to state 0, as before the WCC pass (scc.cu:796). */
+scc_status scc_phase_unset_pivots(scc_ctx *ctx);
 /* The colour split of update without the election (scc_routines.cu:199-210). */
 scc_status scc_phase_recolour(scc_ctx *ctx);
 /* Scorer features (12 floats per vertex, layout in kernels_mlp.hip) of the
@@ -251,6 +260,13 @@ scc_status scc_profile_copy(scc_ctx *ctx, int64_t bytes);
  * rows logged; reset clears the log. */
 #define SCC_LEVEL_FIELDS 12
 scc_status scc_profile_levels(scc_ctx *ctx, double *out, int64_t cap, int64_t *rows, int reset);
+
+/* Fault injection for tests: the nth device allocation (1-based) of the next
+ * scc_upload reports out of memory, so that upload returns SCC_E_OOM and the
+ * context stays usable for another upload; 0 disarms it.  No reference
+ * counterpart: the reference exits on any failed allocation
+ * (GPUerrchk.h:6-12). */
+scc_status scc_test_fail_alloc(scc_ctx *ctx, int32_t nth);
 
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line

@@ -6,6 +6,8 @@ is deterministic (lowest id per slot), the final colour, state and class
 representative of every vertex must also equal the oracle's sequential FB
 restatement run with the same election table size.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -162,6 +164,49 @@ def test_upload_errors(ctx):
     assert np.all(lab == 0)
 
 
+def test_upload_out_of_memory_leaves_context_usable(ctx):
+    """An allocation that fails inside scc_upload (forced from the host: the nth
+    device allocation reports out of memory) gives SCC_E_OOM, and the context
+    takes the next upload and run as if nothing had happened; the reference
+    exits instead (GPUerrchk.h:6-12)."""
+    g = scc.gen_rmat(10, 8, seed=3)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    for nth in (1, 2, 7, 15, 26):
+        ctx.fail_alloc(nth)
+        with pytest.raises(scc.SccError) as e:
+            ctx.upload(g)
+        assert e.value.status == 2, f"allocation {nth}"
+        with pytest.raises(scc.SccError) as e:  # nothing uploaded: a run is out of order
+            ctx.run("h")
+        assert e.value.status == 5
+        ctx.upload(g)
+        lab, _ = ctx.run("h")
+        assert np.array_equal(lab, ref)
+    ctx.fail_alloc(1000)  # beyond the upload's allocations: never fires
+    ctx.upload(g)
+    ctx.fail_alloc(0)
+    lab, _ = ctx.run("d")
+    assert np.array_equal(lab, ref)
+
+
+def test_upload_sizes_beyond_int32(ctx):
+    """Sizes an int32 CSR cannot hold give SCC_E_OVERFLOW before any array is
+    read (the reference's worklists drop what does not fit, worklistc.h:80-81)."""
+    from gpuscc_code_amd import _lib
+    row = np.zeros(8, np.int32)
+    col = np.zeros(8, np.int32)
+    L = _lib.lib()
+    p = lambda a: a.ctypes.data_as(_lib._i32p)  # noqa: E731
+    assert L.scc_upload(ctx._h, 4, 1 << 31, p(row), p(col), None, None) == 4
+    assert L.scc_upload(ctx._h, 4, (1 << 31) - 1, p(row), p(col), None, None) == 4
+    assert L.scc_upload(ctx._h, (1 << 31) - 1, 4, p(row), p(col), None, None) == 4
+    assert L.scc_upload(ctx._h, -1, 4, p(row), p(col), None, None) == 3
+    g = cycle(5)  # the context still works
+    ctx.upload(g)
+    lab, _ = ctx.run("t")
+    assert np.all(lab == 0)
+
+
 # ---------------------------------------------------------------------------
 # phase-level parity on random states
 # ---------------------------------------------------------------------------
@@ -282,6 +327,65 @@ def test_phase_update_elect():
         assert np.array_equal(gc, ocol)
         assert np.array_equal(gs, ost2)
         assert np.array_equal(groots, oroot)
+
+
+def test_phase_count_elimmed_and_unset_pivots(ctx):
+    """countElimmed (scc.cu:42-51) counts the FB-eliminated vertices, state & 12
+    == 4 (E without T), against the oracle's restatement; unsetPivots
+    (scc_routines.cu:10-27) takes exactly the vertices in state 19 back to 0
+    and leaves every other word alone."""
+    rng = np.random.default_rng(11)
+    g = random_digraph(5000, 20000, seed=12)
+    ctx.upload(g)
+    L = oracle.lib()
+    for trial in range(4):
+        colour = rng.integers(1, 1 << 24, g.n).astype(np.uint32)
+        state = rng.choice(np.array([0, 1, 2, 3, 4, 6, 12, 19, 20, 28, 35, 19, 19], np.uint8), g.n)
+        ctx.set_state(scc.make_word(colour, state))
+        cnt = ctx.count_elimmed()
+        st8 = np.ascontiguousarray(state, np.uint8)
+        want = L.orc_count_elimmed(g.n, st8.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        assert cnt == want == int(np.count_nonzero((state & 12) == 4))
+        ctx.unset_pivots()
+        words, _ = ctx.get_state()
+        c2, s2 = scc.split_word(words)
+        assert np.array_equal(c2, colour)
+        assert np.array_equal(s2, np.where(state == 19, 0, state))
+
+
+def test_hybrid_reference_rule(ctx):
+    """hybrid_policy 1: the reference's HYBRID rule (scc.cu:493-563, switch at
+    scc.cu:877-891: t until n/100 vertices are FB-eliminated, then q), every
+    variant, full state against the oracle, on graphs that cross the switch in
+    their first step (R-MAT, batch) and ones that take many steps (mesh)."""
+    try:
+        ctx.set_opts(scc.alg_opt(hybrid_policy=1))
+        for g in (demo_graph(), scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1), scc.gen_mesh(128, 0.6, seed=3),
+                  planted(400, 5, 3000, seed=2)[0]):
+            for variant in (0, 1, 2):
+                check_against_oracle(ctx, g, "h", variant)
+    finally:
+        ctx.set_opts(scc.alg_opt())
+
+
+def test_best_direction_optimising_first_step(ctx):
+    """BEST (reach_b.cu:374-725): the first FB step of R-MAT switches to
+    bottom-up once its frontier holds more than the unexplored arcs / 15 (its
+    giant SCC makes the frontier explode), so at least one level pulls.  Labels
+    and full state against the oracle; a long cycle (one vertex per frontier)
+    as a closed form."""
+    g = scc.gen_rmat(14, 8, 0.57, 0.19, 0.19, seed=1)
+    try:
+        ctx.profile(1)
+        ctx.profile_read(reset=True)
+        check_against_oracle(ctx, g, "b", 0)
+        prof = ctx.profile_read(reset=True)
+        assert prof["sweep_levels"] >= 1
+        ctx.upload(cycle(3000))
+        lab, st = ctx.run("b")
+        assert np.all(lab == 0) and st["steps"] == 1 and st["num_scc"] == 1
+    finally:
+        ctx.profile(0)
 
 
 def test_phase_trim2_and_wcc(ctx):

@@ -326,3 +326,39 @@ def test_cli_writes_a_binary_graph(tmp_path):
     r2 = subprocess.run([os.path.join(BIN, "scc"), "-t", "-f", str(out)], capture_output=True, text=True,
                         timeout=120)
     assert "nodes=13\tedges=19" in r2.stdout and "    ->count of SCC: 6" in r2.stdout
+
+
+def test_sizes_beyond_int32_are_overflow(tmp_path):
+    """Sizes an int32 CSR cannot hold are SCC_E_OVERFLOW (4), not a silent drop
+    (the reference's worklists drop pushes past their capacity,
+    worklistc.h:80-81) and not a crash; negative sizes are invalid input (3).
+    Every case is refused before an array is read or allocated."""
+    import ctypes
+    from gpuscc_code_amd import _lib
+    L = _lib.lib()
+    src = np.zeros(4, np.int32)
+    g = _lib.SccHostGraph()
+    p = lambda a: a.ctypes.data_as(_lib._i32p)  # noqa: E731
+    assert L.scc_csr_from_arcs(8, 1 << 31, p(src), p(src), 1, ctypes.byref(g)) == 4
+    assert L.scc_csr_from_arcs(8, -1, p(src), p(src), 1, ctypes.byref(g)) == 3
+    assert L.scc_gen_mesh(50000, 0.6, 3, ctypes.byref(g)) == 4   # 2.5e9 vertices
+    assert L.scc_gen_rmat(31, 1, 0.57, 0.19, 0.19, 1, 1, ctypes.byref(g)) == 3  # scale above 30
+    assert L.scc_gen_rmat(28, 8, 0.57, 0.19, 0.19, 1, 1, ctypes.byref(g)) == 4  # 2^31 arcs drawn
+    for name, text, status in (("big.graph", "3000000000 5\n1 2\n", 4),
+                               ("bigm.graph", "5 3000000000\n1 2\n", 4),
+                               ("neg.graph", "-5 3\n1 2\n", 3),
+                               ("big.mtx", "%%MatrixMarket matrix coordinate pattern general\n"
+                                           "3000000000 3000000000 2\n1 2\n2 1\n", 4)):
+        with pytest.raises(scc.SccError) as e:
+            scc.load(write(tmp_path, name, text))
+        assert e.value.status == status, name
+    # a binary graph whose header claims 2^31 vertices
+    gg = scc.gen_rmat(6, 4, seed=1)
+    q = tmp_path / "ok.bin"
+    scc.save(gg, q)
+    raw = bytearray(q.read_bytes())
+    raw[8:16] = (1 << 31).to_bytes(8, "little")
+    q.write_bytes(bytes(raw))
+    with pytest.raises(scc.SccError) as e:
+        scc.load(q)
+    assert e.value.status == 4
