@@ -101,6 +101,8 @@ SIGNATURES = {
     "scc_phase_recolour": (_st, [_vp]),
     "scc_phase_unset_pivots": (_st, [_vp]),
     "scc_test_fail_alloc": (_st, [_vp, ctypes.c_int32]),
+    "scc_run_noalloc": (_st, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                              ctypes.c_int64, ctypes.c_void_p, _i64p, ctypes.c_void_p]),
     "scc_phase_features": (_st, [_vp, ctypes.c_int32, _i32p, ctypes.c_int64, _fp]),
     "scc_mlp_set_params": (_st, [_vp, _fp, ctypes.c_int32]),
     "scc_mlp_get_params": (_st, [_vp, _fp, ctypes.c_int32]),

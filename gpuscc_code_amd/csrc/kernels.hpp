@@ -164,6 +164,9 @@ struct TailArgs {
     Ctl *ctl;
     uint32_t max_steps;  // live vertices + 2: bounds the steps, and each step's levels and trim rounds
     int xbits;           // the run's schedule marks expanded vertices (q, t: FExt / BExt)
+    int first_step;      // no pivots yet: the first step is trim-1 and the first election (lowest live id)
+    int self_refill;     // before an election whose 16-bit tag would wrap, clear its slot table here
+                         // (else hand back to the host, status 1)
 };
 void launch_fb_tail(hipStream_t s, const TailArgs &t);
 

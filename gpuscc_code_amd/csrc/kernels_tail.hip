@@ -461,11 +461,26 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
     unsigned long long pivots = 0;
     bool local = false;
     LocalSpace ls{};
-    while (r.npiv > 0) {
-        if (steps >= t.max_steps || ((r.epoch + 1u) & 0xFFFFu) < 2u) {  // the host takes over
+    // first_step: a step from no pivots is trim-1 (every vertex checked, none holds F
+    // or B) and an election whose colour split 3c + 0 keeps one colour per graph
+    bool first = t.first_step != 0;
+    while (r.npiv > 0 || first) {
+        if (steps >= t.max_steps) {
             status = 1;
             break;
         }
+        if (((r.epoch + 1u) & 0xFFFFu) < 2u) {  // the next election's tag wraps
+            if (!t.self_refill) {                // the host takes over
+                status = 1;
+                break;
+            }
+            // clear the table this step's election writes: no key of an election 2^16
+            // earlier may outrank the new tags
+            unsigned long long *cur = t.slots[r.pt ^ 1];
+            for (uint32_t i = threadIdx.x; i < (1u << t.slot_bits); i += TAIL_BLOCK) cur[i] = ~0ull;
+            tail_sync();
+        }
+        first = false;
         if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
             const uint32_t L = ctl->lcnt[r.lsel];
             if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {

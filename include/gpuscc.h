@@ -268,6 +268,27 @@ scc_status scc_profile_levels(scc_ctx *ctx, double *out, int64_t cap, int64_t *r
  * (GPUerrchk.h:6-12). */
 scc_status scc_test_fail_alloc(scc_ctx *ctx, int32_t nth);
 
+/* ---- a decomposition that allocates nothing ---------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The caller passes a CSR and its transpose already in
+ * device memory (0-based, int32, as scc_upload takes them) and a device buffer
+ * of n labels; the call writes the canonical labels (minimum vertex id of each
+ * SCC) there and the SCC count to *num_scc (host, may be NULL).  It runs the
+ * plain FB rule (trim-1, lowest-id pivots, reach, trim, update) in one
+ * workgroup on `stream` (a hipStream_t; NULL for the null stream) of the
+ * current device and returns after that stream has finished the call.
+ * Every buffer besides the caller's lives in one fixed device array of the
+ * library (about 13.5 MB, held on each device the entry point has run on until
+ * the process ends), so the call makes no allocation; it therefore accepts at
+ * most SCC_NOALLOC_MAX_VERTICES vertices (SCC_E_OVERFLOW beyond) and checks
+ * the offsets and ids of both arrays (SCC_E_INPUT).  Calls that overlap on one
+ * device (other host threads, other streams) share that array and race. */
+#define SCC_NOALLOC_MAX_VERTICES 262144
+scc_status scc_run_noalloc(const int32_t *d_row, const int32_t *d_col, const int32_t *d_rowT,
+                           const int32_t *d_colT, int32_t n, int64_t m, int32_t *d_labels, int64_t *num_scc,
+                           void *stream);
+
 /* ---- host input path (no GPU needed) ------------------------------------- */
 /* load (load.cpp:18-207, load.h:5): edge list or MatrixMarket file; a line
  * "a b" is the arc b->a; self-loops kept, duplicates dropped.  Entry lines are
