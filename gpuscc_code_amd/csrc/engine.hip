@@ -463,42 +463,50 @@ scc_status do_reach(scc_ctx *c, uint32_t dirs, int sched, const int32_t *bseed, 
     return SCC_OK;
 }
 
-// per-vertex features (and, with want_score, MLP scores) for the current words
+// per-vertex features of every vertex (and, with want_score, MLP scores) for the current words;
+// the first election of a run (cur_step 0) takes the degrees as within-colour degrees
 scc_status do_features(scc_ctx *c, bool want_score, float *feat_out) {
     Launch l = launch_for(c);
-    if (!c->stat_valid) {
-        launch_static_features(l, c->g, c->d_stat);
-        c->stat_valid = true;
-    }
     launch_fill_i32(l, (int32_t *)c->d_cnt, (int64_t)1 << c->slot_bits, 0);
     launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits);
     float step_f = 0.25f * std::log1p((float)c->cur_step);
-    launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f,
-                 want_score ? c->d_wb : nullptr, want_score ? c->d_score : nullptr, feat_out);
+    launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0, nullptr,
+                 nullptr, want_score ? c->d_wb : nullptr, want_score ? c->d_score : nullptr, feat_out, nullptr, 0,
+                 nullptr);
     HIPCHK(c, hipGetLastError());
     return SCC_OK;
 }
 
 // one election into the table that is not `prev`; winners -> state 19 + pivot list.
-// learned: the colour split runs first, then the scorer, then the election
-// with the scores in the keys.  sync = false leaves *npiv alone: the count is
-// then read from the control block with the next chain's results.
+// learned: the colour split (and the live list) first, then the colour-set
+// sizes and the MFMA scorer over the live vertices, which offers the keys with
+// the scores in them.  sync = false leaves *npiv alone: the count is then read
+// from the control block with the next chain's results.
 scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = false, bool sync = true) {
     Launch l = launch_for(c);
     c->epoch++;
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     int64_t nslots = (int64_t)1 << c->slot_bits;
     if ((c->epoch & 0xFFFFu) < 2u) launch_fill_u64(l, cur, nslots, ~0ull);  // tag wrap
-    const bool lists = c->use_lists && !learned;
+    const bool lists = c->use_lists;
     const int lin_sel = c->lsel, lout_sel = lin_sel < 0 ? 0 : lin_sel ^ 1;
     ctl_op(c, 4 | (lists ? (lout_sel ? 4096 : 2048) : 0));  // with the output list's count zeroed
     if (learned) {
-        if (recolour)
-            launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits,
-                          elect_tag(c->epoch), 1, 0, nullptr, c->d_ctl);
-        CHK(do_features(c, true, nullptr));
-        launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
-                      0, 1, c->d_score, c->d_ctl);
+        const int32_t *lin = lists && lin_sel >= 0 ? c->d_live[lin_sel] : nullptr;
+        const uint32_t *lin_cnt = lin ? &c->d_ctl->lcnt[lin_sel] : nullptr;
+        int32_t *lout = lists ? c->d_live[lout_sel] : nullptr;
+        uint32_t *lout_cnt = lists ? &c->d_ctl->lcnt[lout_sel] : nullptr;
+        launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), recolour, 0,
+                      nullptr, c->d_ctl, lin, lin_cnt, lout, lout_cnt);
+        if (lists) {
+            c->lsel_prev = lin_sel;
+            c->lsel = lout_sel;
+        }
+        launch_fill_i32(l, (int32_t *)c->d_cnt, nslots, 0);
+        launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits, lout, lout_cnt, c->d_ctl);
+        const float step_f = 0.25f * std::log1p((float)c->cur_step);
+        launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0, lout,
+                     lout_cnt, c->d_wb, nullptr, nullptr, cur, elect_tag(c->epoch), c->d_ctl);
     } else if (lists) {
         // the update reads the list of the election before (all n the first time)
         // and lists the vertices still live; the confirm pass reads that list
@@ -865,7 +873,6 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, dev_alloc(c, &c->d_stat, sizeof(float4) * std::max<size_t>((size_t)n, 1)));
     HIPCHK(c, dev_alloc(c, &c->d_cnt, sizeof(uint32_t) << sb));
     HIPCHK(c, dev_alloc(c, &c->d_score, sizeof(uint16_t) * std::max<size_t>((size_t)n, 1)));
-    c->stat_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->d_row, row, nb, hipMemcpyHostToDevice, c->stream));
     if (m > 0) HIPCHK(c, hipMemcpyAsync(c->d_col, col, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_rowT, rowT, nb, hipMemcpyHostToDevice, c->stream));
@@ -877,6 +884,10 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head, c->d_ell);
     launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n, c->d_ell + n);
     launch_head_live(l, c->d_dcls, c->d_head, n, c->d_hlive);
+    // the scorer's static features (degrees, 2-hop degree sums): a property of the graph,
+    // built with the other row indexes
+    launch_static_features(l, c->g, c->d_stat);
+    c->stat_valid = true;
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());
@@ -905,7 +916,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     const int32_t n = c->g.n;
     c->syncs = 0;
     c->cur_step = 0;
-    c->use_lists = !learned;  // elections of this run keep the live-vertex list (none yet)
+    c->use_lists = true;  // elections of this run keep the live-vertex list (none yet)
     c->lsel = c->lsel_prev = -1;
     if (c->label_in_root) c->labels_valid = false;  // the run rewrites the root array
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -938,8 +949,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // chain may need more levels; every later kernel of the step then exits
         // at once, and the host resumes that chain and finishes the step with
         // waits.  The results are those of the waiting path.
-        const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !learned &&
-                             !by_elim && c->prof_flags == 0;
+        const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !by_elim &&
+                             c->prof_flags == 0;
         auto tp = std::chrono::steady_clock::now();
         // With speculative steps the rest of trim-1 goes out without a wait as well:
         // its chain (as long as trim-1's in the last run), a guard (stage 3), the first
@@ -1062,7 +1073,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 HIPCHK(c, hipEventRecord(c->spec_ev[2], c->stream));
                 c->cur_step = (int)st.steps;
                 int64_t unused = 0;
-                CHK(do_elect(c, 1, &unused, false, false));
+                CHK(do_elect(c, 1, &unused, learned, false));
                 HIPCHK(c, hipEventRecord(c->spec_ev[3], c->stream));
                 CHK(read_ctl(c));
                 const Ctl h = *c->h_ctl;
@@ -1132,7 +1143,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     CHK(do_trim(c, &dt, false, h.abort == 2 ? Lt : 0, true));
                     st.ms_trim += ms_since(tp);
                     tp = std::chrono::steady_clock::now();
-                    CHK(do_elect(c, 1, &npiv, false));
+                    CHK(do_elect(c, 1, &npiv, learned));
                     st.ms_update += ms_since(tp);
                 }
                 st.fwd_depths += dr;

@@ -172,11 +172,16 @@ void launch_fb_tail(hipStream_t s, const TailArgs &t);
 
 // kernels_mlp.hip -- learned pivot scorer
 void launch_static_features(const Launch &l, const Graph &g, float4 *out);
-void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits);
-// features of every vertex into feat_out (n x 12, may be null) and, with params, scores
-void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat,
-                  const uint32_t *cnt, int slot_bits, float step_f, const uint16_t *params,
-                  uint16_t *score, float *feat_out);
+// live vertices per colour slot over the list lin[0 .. *lin_cnt) (lin null: all n)
+void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits,
+                         const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr, const Ctl *ctl = nullptr);
+// features (feat_out, n x 12), and with params the MFMA scorer: scores (score_out) and the
+// learned election (keys offered to cur), over the list lin (null: all n); each output optional;
+// static_deg: the first election of a run (within-colour degrees = degrees)
+void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
+                  int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
+                  const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
+                  unsigned long long tag, const Ctl *ctl);
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit);
 void launch_mlp_fwdbwd(const Launch &l, const float *X, const float *y, const int32_t *idx, int64_t N,
                        const uint16_t *params, MlpWork &w);

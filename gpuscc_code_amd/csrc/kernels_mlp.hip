@@ -85,17 +85,24 @@ __global__ void k_static_features(const int32_t *__restrict__ row, const int32_t
 }
 
 // live vertices per colour slot (the colour-set size, slots shared by
-// colliding colours are summed)
-__global__ void k_colour_count(const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits) {
+// colliding colours are summed), over the vertices lin[0 .. *lin_cnt) or, with
+// lin null, all of [0, n)
+__global__ void k_colour_count(const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits, const int32_t *lin,
+                               const uint32_t *lin_cnt, const Ctl *ctl) {
+    if (ctl && __atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
     int lane = threadIdx.x & (WAVE - 1);
+    // a wave-uniform pending count of one slot, carried across rounds: a colour set
+    // spread over many waves (the first election's one colour per graph) costs one
+    // atomic per wave, not one per round
+    uint32_t pslot = 0xFFFFFFFFu, pcnt = 0;
     for (int64_t base = beg; base < end; base += blockDim.x) {
-        int64_t v = base + threadIdx.x;
+        int64_t i = base + threadIdx.x;
         bool live = false;
         uint32_t sl = 0;
-        if (v < end) {
-            uint32_t w = word[v];
+        if (i < end) {
+            uint32_t w = word[lin ? lin[i] : (int32_t)i];
             live = !(w & ST_E);
             sl = slot_of(colour_of(w), slot_bits);
         }
@@ -104,18 +111,23 @@ __global__ void k_colour_count(const uint32_t *word, int32_t n, uint32_t *cnt, i
         int first = __ffsll((long long)mask) - 1;
         uint32_t sl0 = __shfl(sl, first, WAVE);
         if (__all(!live || sl == sl0)) {
-            if (lane == first) atomicAdd(&cnt[sl0], (uint32_t)__popcll(mask));
+            if (sl0 != pslot) {
+                if (pcnt && lane == 0) atomicAdd(&cnt[pslot], pcnt);
+                pslot = sl0;
+                pcnt = 0;
+            }
+            pcnt += (uint32_t)__popcll(mask);
         } else if (live) {
             atomicAdd(&cnt[sl], 1u);
         }
     }
+    if (pcnt && lane == 0) atomicAdd(&cnt[pslot], pcnt);
 }
 
 __device__ inline uint32_t live_same_colour(const int32_t *__restrict__ row,
                                             const int32_t *__restrict__ col, const uint32_t *word,
-                                            int32_t v, uint32_t c, uint32_t &deg) {
+                                            int32_t v, uint32_t c) {
     int32_t e0 = row[v], e1 = row[v + 1];
-    deg = (uint32_t)(e1 - e0);
     int32_t ee = e1 - e0 > DEG_SAMPLE ? e0 + DEG_SAMPLE : e1;
     uint32_t k = 0;
     for (int32_t e = e0; e < ee; e += FEAT_ILP) {  // FEAT_ILP arcs: column ids, then words, then tests
@@ -132,99 +144,245 @@ __device__ inline uint32_t live_same_colour(const int32_t *__restrict__ row,
     return k;
 }
 
-__device__ inline void vertex_features(const int32_t *row, const int32_t *col, const int32_t *rowT,
-                                       const int32_t *colT, const uint32_t *word, const float4 *stat,
-                                       const uint32_t *cnt, int slot_bits, float step_f, int32_t v,
-                                       uint32_t w, float *f) {
-    uint32_t c = colour_of(w), dout, din;
-    uint32_t oc = live_same_colour(row, col, word, v, c, dout);
-    uint32_t ic = live_same_colour(rowT, colT, word, v, c, din);
-    float4 s = stat[v];
-    float foc = (float)oc, fic = (float)ic;
-    f[0] = log1pf(foc) * LOG_SCALE;
-    f[1] = log1pf(fic) * LOG_SCALE;
-    f[2] = s.x * LOG_SCALE;
-    f[3] = s.y * LOG_SCALE;
-    f[4] = s.z * LOG_SCALE * 0.5f;
-    f[5] = s.w * LOG_SCALE * 0.5f;
-    f[6] = step_f;
-    f[7] = log1pf((float)cnt[slot_of(c, slot_bits)]) * LOG_SCALE;
-    f[8] = foc / ((float)dout + 1.f);
-    f[9] = fic / ((float)din + 1.f);
-    float mx = foc > fic ? foc : fic, mn = foc > fic ? fic : foc;
-    f[10] = mn / (mx + 1.f);
-    f[11] = log1pf(foc * fic) * LOG_SCALE * 0.5f;
-}
-
-// the network on one feature vector; weights in LDS as floats, activations
-// rounded to BF16, FP32 accumulation
-__device__ inline float mlp_logit(const float *p, const float *f, float *h1) {
-    for (int i = 0; i < NH; i++) {
-        float a = p[OFF_B1 + i];
-        for (int k = 0; k < NF; k++) a += p[OFF_W1 + i * NF + k] * f[k];
-        h1[i] = bf16_round(a > 0.f ? a : 0.f);
-    }
-    float s = p[OFF_B3];
-    for (int i = 0; i < NH; i++) {
-        float a = p[OFF_B2 + i];
-        for (int j = 0; j < NH; j++) a += p[OFF_W2 + i * NH + j] * h1[j];
-        float h2 = bf16_round(a > 0.f ? a : 0.f);
-        s += p[OFF_W3 + i] * h2;
-    }
-    return s;
-}
-
 __device__ inline void load_params(const uint16_t *pb, float *sp) {
     for (int i = threadIdx.x; i < MLP_PARAMS; i += blockDim.x) sp[i] = bf16_to_f(pb[i]);
     __syncthreads();
 }
 
-// Features (optional output) and, with parameters, the fused inference: the
-// score of a live vertex is round(sigmoid(logit) * 65535); eliminated vertices
-// score 0 and get zero features.
-__global__ __launch_bounds__(MLP_BLOCK) void k_score(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
-                        const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
-                        const uint32_t *word, const float4 *stat, const uint32_t *cnt, int slot_bits,
-                        float step_f, int32_t n, const uint16_t *params, uint16_t *score,
-                        float *feat_out) {
-    __shared__ float sp[MLP_PARAMS];
-    if (params) load_params(params, sp);
-    int64_t beg, end;
-    my_range(n, beg, end);
-    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
-        int32_t v = (int32_t)vv;
-        uint32_t w = word[v];
-        float f[NF];
-        bool live = !(w & ST_E);
-        if (live) {
-            vertex_features(row, col, rowT, colT, word, stat, cnt, slot_bits, step_f, v, w, f);
-        } else {
-            for (int k = 0; k < NF; k++) f[k] = 0.f;
+// ---------------------------------------------------------------------------
+// The network on MFMA (v_mfma_f32_32x32x8_bf16): one wave scores 32 vertices.
+// The layers are computed transposed, D1 = W1' X^T (hidden x vertices) and
+// D2 = W2 H1^T, so that the layer-1 result is already in the B-operand layout
+// of layer 2.  Lane l = 32h + j holds A[j][8c + 4h .. +3] and B[8c + 4h .. +3][j]
+// of K chunk c, and D[8q + 4h + s][j] in accumulator register 4q + s.  W1' is
+// W1 with the bias as feature 12 (input 1) and features 13-15 zero.  FP32
+// features enter as two BF16 parts (x = hi + lo), so layer 1 sees 16 bits of
+// each feature, not 8; weights and hidden activations are BF16, accumulation
+// FP32, as in the scalar restatement (tests compare them).
+// ---------------------------------------------------------------------------
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+struct MlpRegs {
+    bf16x4_t a1[2];  // W1' chunks
+    bf16x4_t a2[4];  // W2 chunks
+    float b2[16];    // b2 of the accumulator rows this lane holds
+    float w3[16];    // w3 of the same rows
+    float b3;
+};
+
+__device__ inline void mlp_regs_load(const uint16_t *__restrict__ pb, MlpRegs &R) {
+    const int l = threadIdx.x & (WAVE - 1), j = l & 31, h = l >> 5;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 8 * c + 4 * h + t;
+            R.a1[c][t] = (short)(k < NF ? pb[OFF_W1 + j * NF + k] : (k == NF ? pb[OFF_B1 + j] : 0));
         }
-        if (feat_out)
-            for (int k = 0; k < NF; k++) feat_out[(int64_t)v * NF + k] = f[k];
-        if (params) {
-            uint16_t sc = 0;
-            if (live) {
-                float h1[NH];
-                float s = mlp_logit(sp, f, h1);
-                float p = 1.f / (1.f + __expf(-s));
-                sc = (uint16_t)__float2uint_rn(p * 65535.f);
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) R.a2[c][t] = (short)pb[OFF_W2 + j * NH + 8 * c + 4 * h + t];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+        R.b2[r] = bf16_to_f(pb[OFF_B2 + i]);
+        R.w3[r] = bf16_to_f(pb[OFF_W3 + i]);
+    }
+    R.b3 = bf16_to_f(pb[OFF_B3]);
+}
+
+// Logit of vertex j of the tile, returned in lanes j and j + 32.  x[0..3] are
+// this lane's features 4h .. 4h + 3 and x[4..7] its features 8 + 4h .. 11 + 4h
+// (for h = 1: 12 .. 15 = 1, 0, 0, 0).  Every lane of the wave must call it.
+__device__ inline float mlp_logit_mfma(const MlpRegs &R, const float (&x)[8]) {
+    bf16x4_t hi[2], lo[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const float f = x[4 * c + t];
+            const uint16_t a = f_to_bf16(f);
+            hi[c][t] = (short)a;
+            lo[c][t] = (short)f_to_bf16(f - bf16_to_f(a));
+        }
+    f32x16_t d1;
+#pragma unroll
+    for (int r = 0; r < 16; r++) d1[r] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a1[c], hi[c], d1, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a1[c], lo[c], d1, 0, 0, 0);
+    }
+    bf16x4_t h1[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const float v = d1[4 * c + t];
+            h1[c][t] = (short)f_to_bf16(v > 0.f ? v : 0.f);
+        }
+    f32x16_t d2;
+#pragma unroll
+    for (int r = 0; r < 16; r++) d2[r] = R.b2[r];
+#pragma unroll
+    for (int c = 0; c < 4; c++) d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a2[c], h1[c], d2, 0, 0, 0);
+    float part = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const float v = d2[r];
+        part += R.w3[r] * bf16_round(v > 0.f ? v : 0.f);
+    }
+    part += __shfl_xor(part, 32, WAVE);
+    return part + R.b3;
+}
+
+// Features, scores and the learned election, one wave per 32 vertices of the
+// list lin[0 .. *lin_cnt) (all of [0, n) with lin null).  Lanes j and j + 32
+// share vertex j: the first half computes the degree features (0-3, 8-11; the
+// within-colour degrees by row scans, or, in the first election of a run,
+// where every arc joins two vertices of one colour, the degrees themselves),
+// the second half features 4-7 (2-hop sums, FB step, colour-set size).
+// Outputs, each optional: feat_out[v][12] (zeros for eliminated vertices),
+// score_out[v] = round(sigmoid(logit) * 65535) (0 when eliminated), and the
+// election key [tag][65535 - score][v] offered to cur[slot of the colour]
+// (lowest key wins).
+struct ScoreArgs {
+    const int32_t *row, *col, *rowT, *colT;
+    const uint32_t *word;
+    const float4 *stat;
+    const uint32_t *cnt;
+    int slot_bits;
+    float step_f;
+    int static_deg;
+    int32_t n;
+    const int32_t *lin;
+    const uint32_t *lin_cnt;
+    const uint16_t *params;
+    uint16_t *score_out;
+    float *feat_out;
+    unsigned long long *cur;
+    unsigned long long tag;
+    const Ctl *ctl;
+};
+
+__global__ __launch_bounds__(MLP_BLOCK) void k_score(ScoreArgs a) {
+    if (a.ctl && __atomic_load_n(&a.ctl->abort, __ATOMIC_RELAXED)) return;
+    MlpRegs R;
+    if (a.params) mlp_regs_load(a.params, R);
+    const int l = threadIdx.x & (WAVE - 1), j = l & 31, h = l >> 5;
+    const int64_t total = a.lin ? (int64_t)*a.lin_cnt : (int64_t)a.n;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / WAVE;
+    // the wave's pending minimum key of one slot, carried across tiles (flushed when
+    // the slot changes and at the end)
+    uint32_t pslot = 0xFFFFFFFFu;
+    unsigned long long pmin = ~0ull;
+    for (int64_t base = wave * 32; base < total; base += nwaves * 32) {  // wave-uniform
+        const int64_t i = base + j;
+        const int32_t v = i < total ? (a.lin ? a.lin[i] : (int32_t)i) : -1;
+        const uint32_t w = v >= 0 ? a.word[v] : ST_E;
+        const bool live = !(w & ST_E);
+        float x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = 0.f;
+        if (live) {
+            const uint32_t c = colour_of(w);
+            const float4 st = a.stat[v];
+            if (h == 0) {
+                const float dout = (float)(a.row[v + 1] - a.row[v]), din = (float)(a.rowT[v + 1] - a.rowT[v]);
+                float oc = dout, ic = din;
+                if (!a.static_deg) {
+                    oc = (float)live_same_colour(a.row, a.col, a.word, v, c);
+                    ic = (float)live_same_colour(a.rowT, a.colT, a.word, v, c);
+                }
+                x[0] = log1pf(oc) * LOG_SCALE;
+                x[1] = log1pf(ic) * LOG_SCALE;
+                x[2] = st.x * LOG_SCALE;
+                x[3] = st.y * LOG_SCALE;
+                x[4] = oc / (dout + 1.f);
+                x[5] = ic / (din + 1.f);
+                const float mx = oc > ic ? oc : ic, mn = oc > ic ? ic : oc;
+                x[6] = mn / (mx + 1.f);
+                x[7] = log1pf(oc * ic) * LOG_SCALE * 0.5f;
+            } else {
+                x[0] = st.z * LOG_SCALE * 0.5f;
+                x[1] = st.w * LOG_SCALE * 0.5f;
+                x[2] = a.step_f;
+                x[3] = log1pf((float)a.cnt[slot_of(c, a.slot_bits)]) * LOG_SCALE;
             }
-            score[v] = sc;
+        }
+        if (a.feat_out && v >= 0) {
+            float *f = a.feat_out + (int64_t)v * NF;
+            if (h == 0) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    f[t] = x[t];
+                    f[8 + t] = x[4 + t];
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++) f[4 + t] = x[t];
+            }
+        }
+        if (!a.params) continue;  // kernel-uniform
+        if (h == 1) x[4] = 1.f;   // feature 12: the bias input
+        const float s = mlp_logit_mfma(R, x);
+        uint16_t sc = 0;
+        if (live) sc = (uint16_t)__float2uint_rn(65535.f / (1.f + __expf(-s)));
+        if (h == 0 && v >= 0 && a.score_out) a.score_out[v] = sc;
+        if (!a.cur) continue;  // kernel-uniform
+        // the election: one atomicMin per wave when its candidates share a slot
+        const bool cand = h == 0 && live;
+        const uint32_t sl = cand ? slot_of(colour_of(w), a.slot_bits) : 0u;
+        const unsigned long long key = cand ? (a.tag | ((unsigned long long)(0xFFFFu - sc) << 32) | (uint32_t)v)
+                                            : ~0ull;
+        const unsigned long long bm = __ballot(cand);
+        if (bm == 0) continue;  // wave-uniform
+        const uint32_t sl0 = __shfl(sl, __ffsll((long long)bm) - 1, WAVE);
+        if (__all(!cand || sl == sl0)) {
+            unsigned long long m = key;
+            for (int o = WAVE / 2; o > 0; o >>= 1) {
+                const unsigned long long y = __shfl_xor(m, o, WAVE);
+                m = y < m ? y : m;
+            }
+            if (sl0 != pslot) {
+                if (pslot != 0xFFFFFFFFu && l == 0 && pmin < a.cur[pslot]) atomicMin(&a.cur[pslot], pmin);
+                pslot = sl0;
+                pmin = m;
+            } else {
+                pmin = m < pmin ? m : pmin;
+            }
+        } else if (cand && key < a.cur[sl]) {
+            atomicMin(&a.cur[sl], key);
         }
     }
+    if (a.cur && pslot != 0xFFFFFFFFu && l == 0 && pmin < a.cur[pslot]) atomicMin(&a.cur[pslot], pmin);
 }
 
 // inference on given feature rows (tests): logits out
 __global__ __launch_bounds__(MLP_BLOCK) void k_mlp_infer(const float *X, int64_t N, const uint16_t *params, float *logit) {
-    __shared__ float sp[MLP_PARAMS];
-    load_params(params, sp);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float f[NF], h1[NH];
-        for (int k = 0; k < NF; k++) f[k] = X[i * NF + k];
-        logit[i] = mlp_logit(sp, f, h1);
+    MlpRegs R;
+    mlp_regs_load(params, R);
+    const int l = threadIdx.x & (WAVE - 1), j = l & 31, h = l >> 5;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / WAVE;
+    for (int64_t base = wave * 32; base < N; base += nwaves * 32) {  // wave-uniform
+        const int64_t i = base + j;
+        float x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = 0.f;
+        if (i < N) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                x[t] = X[i * NF + 4 * h + t];
+                if (h == 0) x[4 + t] = X[i * NF + 8 + t];
+            }
+        }
+        if (h == 1) x[4] = 1.f;
+        const float s = mlp_logit_mfma(R, x);
+        if (h == 0 && i < N) logit[i] = s;
     }
 }
 
@@ -353,14 +511,17 @@ __global__ void k_fill_f32(float *p, int64_t count, float val) {
 void launch_static_features(const Launch &l, const Graph &g, float4 *out) {
     hipLaunchKernelGGL(k_static_features, GRID(l), g.row, g.col, g.rowT, g.colT, g.n, out);
 }
-void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits) {
-    hipLaunchKernelGGL(k_colour_count, GRID(l), word, n, cnt, slot_bits);
+void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint32_t *cnt, int slot_bits,
+                         const int32_t *lin, const uint32_t *lin_cnt, const Ctl *ctl) {
+    hipLaunchKernelGGL(k_colour_count, GRID(l), word, n, cnt, slot_bits, lin, lin_cnt, ctl);
 }
-void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat,
-                  const uint32_t *cnt, int slot_bits, float step_f, const uint16_t *params,
-                  uint16_t *score, float *feat_out) {
-    hipLaunchKernelGGL(k_score, GRID_MLP(l), g.row, g.col, g.rowT, g.colT, word, stat, cnt, slot_bits,
-                       step_f, g.n, params, score, feat_out);
+void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
+                  int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
+                  const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
+                  unsigned long long tag, const Ctl *ctl) {
+    ScoreArgs a{g.row, g.col, g.rowT, g.colT, word, stat, cnt, slot_bits, step_f, static_deg, g.n,
+                lin, lin_cnt, params, score_out, feat_out, cur, tag, ctl};
+    hipLaunchKernelGGL(k_score, GRID_MLP(l), a);
 }
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit) {
     hipLaunchKernelGGL(k_mlp_infer, GRID_MLP(l), X, N, params, logit);

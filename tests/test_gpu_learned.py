@@ -88,6 +88,79 @@ def test_features_definition(ctx):
         assert abs(X[k, 6] - 0.25 * np.log1p(3)) < 1e-6
 
 
+def test_features_first_election(ctx):
+    """In the first election of a run every arc joins two vertices of one colour,
+    and the within-colour degrees (features 0, 1, 8-11) are the degrees."""
+    g = scc.gen_rmat(11, 8, seed=12)
+    ctx.upload(g)
+    n = g.n
+    ctx.set_state(scc.make_word(np.ones(n, np.uint32), np.zeros(n, np.uint32)))
+    idx = np.arange(0, n, 7)
+    X = ctx.features(idx, step=0)
+    dout = np.diff(g.row)[idx].astype(np.float64)
+    din = np.diff(g.rowT)[idx].astype(np.float64)
+    assert np.allclose(X[:, 0], np.log1p(dout) / 16, atol=1e-6)
+    assert np.allclose(X[:, 1], np.log1p(din) / 16, atol=1e-6)
+    assert np.allclose(X[:, 8], dout / (dout + 1), atol=1e-6)
+    assert np.allclose(X[:, 9], din / (din + 1), atol=1e-6)
+    assert np.allclose(X[:, 6], 0.0)
+    assert np.allclose(X[:, 7], np.log1p(n) / 16, atol=1e-6)  # one colour set of n live vertices
+
+
+def oracle_scores(p, X):
+    logit, _, _, _ = mlp.forward(p, X, bf16=True)
+    return np.rint(65535.0 / (1.0 + np.exp(-logit)))
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_learned_first_pivot_has_the_top_score(ctx, seed):
+    """The learned election (MFMA scorer, keys offered per wave) picks, in the
+    first election, a vertex whose score is the highest of the live vertices
+    (scores recomputed with the CPU restatement of the network on the dumped
+    features; +-2 of 65,535 for rounding), the lowest id among equal scores."""
+    p = learned.init_params(seed)
+    for g in (scc.gen_rmat(14, 8, seed=1), scc.gen_mesh(96, 0.6, seed=9)):
+        ctx.upload(g)
+        ctx.mlp_set(p)
+        lab, st = ctx.run("h", 0, pivot=scc.PIVOT_LEARNED)
+        assert np.array_equal(lab, oracle.tarjan(g.row, g.col)[0])
+        n = g.n
+        ctx.set_state(scc.make_word(np.ones(n, np.uint32), np.zeros(n, np.uint32)))
+        ctx.trim()  # trim-1 to its fixpoint: the live set of the first election
+        words, _ = ctx.get_state()
+        live = np.nonzero((words & 4) == 0)[0]
+        sc = oracle_scores(p, ctx.features(live, step=0))
+        fp = st["first_pivot"]
+        assert fp in set(live.tolist())
+        assert sc[np.searchsorted(live, fp)] >= sc.max() - 2
+        top = live[sc >= sc.max() - 2]
+        assert fp <= top.max()
+
+
+def test_learned_rule_uses_lists_tail_and_speculation(ctx):
+    """The learned rule runs on the plain rule's engine path: speculative steps
+    (one host wait on R-MAT), live-vertex lists and the one-workgroup tail (the
+    mesh's later steps); labels exact with the tail on and off."""
+    ctx.mlp_set(learned.init_params(3))
+    g = scc.gen_rmat(15, 8, seed=2)
+    ctx.upload(g)
+    ref, _ = oracle.tarjan(g.row, g.col)
+    for _ in range(2):  # the second run knows the chain lengths
+        lab, st = ctx.run("h", 0, pivot=scc.PIVOT_LEARNED)
+    assert np.array_equal(lab, ref) and st["host_syncs"] <= 3
+    m = scc.gen_mesh(128, 0.6, seed=3)
+    ref, _ = oracle.tarjan(m.row, m.col)
+    ctx.upload(m)
+    try:
+        for tail in (0, -1):
+            ctx.set_opts(scc.alg_opt(tail_live=tail))
+            lab, st = ctx.run("h", 0, pivot=scc.PIVOT_LEARNED)
+            assert np.array_equal(lab, ref)
+            assert (st["tail_steps"] > 0) == (tail == 0)
+    finally:
+        ctx.set_opts(scc.alg_opt())
+
+
 @pytest.mark.skipif(not os.path.exists(learned.PARAMS_PATH), reason="no trained parameters shipped")
 def test_trained_scorer_gain_on_held_out(ctx):
     """What the trained network gains over the plain rule on graphs it was not
