@@ -33,6 +33,8 @@ struct scc_ctx {
     int32_t *d_row = nullptr, *d_col = nullptr, *d_rowT = nullptr, *d_colT = nullptr;
     uint32_t *d_word = nullptr;
     int32_t *d_root = nullptr, *d_label = nullptr, *d_tmp = nullptr;
+    int32_t *d_mn = nullptr;  // learned runs: minimum id of each pivot's SCC (k_update)
+    bool track_mn = false;    // this run's updates keep d_mn
     int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
     int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
     int32_t *d_seedB = nullptr;  // backward seeds of a phase-level reach
@@ -470,7 +472,7 @@ scc_status do_features(scc_ctx *c, bool want_score, float *feat_out) {
     launch_fill_i32(l, (int32_t *)c->d_cnt, (int64_t)1 << c->slot_bits, 0);
     launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits);
     float step_f = 0.25f * std::log1p((float)c->cur_step);
-    launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0, nullptr,
+    launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, nullptr, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0, nullptr,
                  nullptr, want_score ? c->d_wb : nullptr, want_score ? c->d_score : nullptr, feat_out, nullptr, 0,
                  nullptr);
     HIPCHK(c, hipGetLastError());
@@ -497,16 +499,21 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         int32_t *lout = lists ? c->d_live[lout_sel] : nullptr;
         uint32_t *lout_cnt = lists ? &c->d_ctl->lcnt[lout_sel] : nullptr;
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), recolour, 0,
-                      nullptr, c->d_ctl, lin, lin_cnt, lout, lout_cnt);
+                      nullptr, c->d_ctl, lin, lin_cnt, lout, lout_cnt, c->track_mn ? c->d_mn : nullptr);
         if (lists) {
             c->lsel_prev = lin_sel;
             c->lsel = lout_sel;
         }
-        launch_fill_i32(l, (int32_t *)c->d_cnt, nslots, 0);
-        launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits, lout, lout_cnt, c->d_ctl);
+        // colour-set sizes: in the first election of a single graph every live vertex has
+        // the one colour, and the device's live count is the size
+        const uint32_t *one_set = !recolour && c->cur_step == 0 && c->num_graphs == 0 ? &c->d_ctl->live : nullptr;
+        if (!one_set) {
+            launch_fill_i32(l, (int32_t *)c->d_cnt, nslots, 0);
+            launch_colour_count(l, c->d_word, c->g.n, c->d_cnt, c->slot_bits, lout, lout_cnt, c->d_ctl);
+        }
         const float step_f = 0.25f * std::log1p((float)c->cur_step);
-        launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0, lout,
-                     lout_cnt, c->d_wb, nullptr, nullptr, cur, elect_tag(c->epoch), c->d_ctl);
+        launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, one_set, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0,
+                     lout, lout_cnt, c->d_wb, nullptr, nullptr, cur, elect_tag(c->epoch), c->d_ctl);
     } else if (lists) {
         // the update reads the list of the election before (all n the first time)
         // and lists the vertices still live; the confirm pass reads that list
@@ -514,12 +521,13 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         const uint32_t *lin_cnt = lin_sel >= 0 ? &c->d_ctl->lcnt[lin_sel] : nullptr;
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       recolour, 1, nullptr, c->d_ctl, lin, lin_cnt, c->d_live[lout_sel],
-                      &c->d_ctl->lcnt[lout_sel]);
+                      &c->d_ctl->lcnt[lout_sel], c->track_mn ? c->d_mn : nullptr);
         c->lsel_prev = lin_sel;
         c->lsel = lout_sel;
     } else {
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
-                      recolour, 1, nullptr, c->d_ctl);
+                      recolour, 1, nullptr, c->d_ctl, nullptr, nullptr, nullptr, nullptr,
+                      c->track_mn ? c->d_mn : nullptr);
     }
     uint32_t *vis = c->vis_mark ? c->d_vis : nullptr;
     if (lists)
@@ -599,7 +607,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 void free_graph(scc_ctx *c) {
     clear_graphs(c);  // captured launches point at the buffers freed here
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
-                        &c->d_tmp, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
+                        &c->d_tmp, &c->d_mn, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
                         &c->d_piv, &c->d_seedB, &c->d_huge[0], &c->d_huge[1], &c->d_huge[2],
                         &c->d_huge[3], (int32_t **)&c->d_stamp, (int32_t **)&c->d_dcls, &c->d_head};
     for (auto b : bufs) {
@@ -847,6 +855,7 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     HIPCHK(c, dev_alloc(c, &c->d_root, vb));
     HIPCHK(c, dev_alloc(c, &c->d_label, vb));
     HIPCHK(c, dev_alloc(c, &c->d_tmp, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_mn, vb));
     for (auto &q : c->d_q) HIPCHK(c, dev_alloc(c, &q, vb));
     for (auto &q : c->d_huge) HIPCHK(c, dev_alloc(c, &q, vb));
     HIPCHK(c, dev_alloc(c, &c->d_stamp, vb));
@@ -931,8 +940,11 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // k_init counts its list)
         c->pend_bits = c->pend_stage = 0;
         ctl_op(c, 512 | 1024 | 128 | 2 | 1);
+        // learned pivots are not the lowest ids of their SCCs: the updates keep each SCC's
+        // minimum in d_mn (k_init sets mn[v] = v), and the labels are mn[root]
+        c->track_mn = learned;
         launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
-                    c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words);
+                    c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words, learned ? c->d_mn : nullptr);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         bool wcc_done = false;
@@ -1018,6 +1030,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 ta.ctl = c->d_ctl;
                 ta.max_steps = c->h_ctl->live + 2;  // each step eliminates its pivots
                 ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
+                ta.mn = c->track_mn ? c->d_mn : nullptr;
                 st.tail_live = c->h_ctl->live;
                 if (c->pend_stage || c->pend_bits) ctl_op(c, 0);  // queued resets go out first
                 launch_fb_tail(c->stream, ta);
@@ -1209,14 +1222,15 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // they are set, and the last wait above read the count.  With the plain
         // rule every pivot is the lowest live id of its colour set, so root is
         // already the minimum of each SCC (trimmed vertices and pairs point to
-        // themselves and to the lower id); the learned rule needs the minimum pass.
-        if (learned)
-            launch_labels(l, c->d_root, c->d_tmp, c->d_label, n);
-        c->label_in_root = !learned;
+        // themselves and to the lower id); with the learned rule each root takes its
+        // SCC's minimum, kept by the updates in d_mn.
+        if (learned) launch_root_to_min(l, c->d_root, c->d_mn, n);
+        c->label_in_root = true;
         st.num_scc = (int64_t)c->h_ctl->pcount;
         return SCC_OK;
     };
     s = body();
+    c->track_mn = false;
     c->one_colour = false;
     c->best_step = false;
     c->vis_mark = c->vis_valid = false;

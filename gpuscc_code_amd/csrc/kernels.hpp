@@ -100,7 +100,7 @@ struct Graph {  // device CSR + CSR^T
 // vis (may be null): the two visited bitmaps (vis_words words each) are cleared.
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl,
-                 uint32_t *vis = nullptr, int64_t vis_words = 0);
+                 uint32_t *vis = nullptr, int64_t vis_words = 0, int32_t *mn = nullptr);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
@@ -126,7 +126,7 @@ void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,
                    Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr,
-                   int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr);
+                   int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr, int32_t *mn = nullptr);
 // vis (may be null): the winners are marked in both visited bitmaps (the first election
 // of a run: the seeds of the first reach)
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
@@ -142,6 +142,8 @@ void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int3
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
                          const Ctl *ctl);
 void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n);
+// root[v] = mn[root[v]] in place (mn kept by the run's updates, see k_update)
+void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n);
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl);
 
@@ -164,6 +166,7 @@ struct TailArgs {
     Ctl *ctl;
     uint32_t max_steps;  // live vertices + 2: bounds the steps, and each step's levels and trim rounds
     int xbits;           // the run's schedule marks expanded vertices (q, t: FExt / BExt)
+    int32_t *mn;         // may be null: FB-eliminated vertices lower mn[their pivot] (see k_update)
     int first_step;      // no pivots yet: the first step is trim-1 and the first election (lowest live id)
     int self_refill;     // before an election whose 16-bit tag would wrap, clear its slot table here
                          // (else hand back to the host, status 1)
@@ -177,9 +180,10 @@ void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint3
                          const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr, const Ctl *ctl = nullptr);
 // features (feat_out, n x 12), and with params the MFMA scorer: scores (score_out) and the
 // learned election (keys offered to cur), over the list lin (null: all n); each output optional;
-// static_deg: the first election of a run (within-colour degrees = degrees)
+// static_deg: the first election of a run (within-colour degrees = degrees); one_set (may be
+// null): every live vertex is in one colour set of *one_set vertices (cnt unused)
 void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
-                  int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
+                  const uint32_t *one_set, int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
                   const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
                   unsigned long long tag, const Ctl *ctl);
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit);

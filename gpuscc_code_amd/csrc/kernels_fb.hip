@@ -51,7 +51,7 @@ constexpr int INIT_V = 8;
 template <int VEC>
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
                        const uint8_t *cls_out, const uint8_t *cls_in, const uint8_t *hlive, int32_t *unk,
-                       Ctl *ctl, uint32_t *vis, int64_t vis_words) {
+                       Ctl *ctl, uint32_t *vis, int64_t vis_words, int32_t *mn) {
     constexpr int R = INIT_V / VEC;  // groups of VEC vertices per work-item and round
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t shw[16];
@@ -107,11 +107,13 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
             }
             if (VEC == 4) {
                 *reinterpret_cast<uint4 *>(word + v0) = make_uint4(w[0], w[1 % VEC], w[2 % VEC], w[3 % VEC]);
-                *reinterpret_cast<int4 *>(root + v0) =
-                    make_int4((int32_t)v0, (int32_t)v0 + 1, (int32_t)v0 + 2, (int32_t)v0 + 3);
+                const int4 self = make_int4((int32_t)v0, (int32_t)v0 + 1, (int32_t)v0 + 2, (int32_t)v0 + 3);
+                *reinterpret_cast<int4 *>(root + v0) = self;
+                if (mn) *reinterpret_cast<int4 *>(mn + v0) = self;
             } else {
                 word[v0] = w[0];
                 root[v0] = (int32_t)v0;
+                if (mn) mn[v0] = (int32_t)v0;
             }
         }
         if (hlive) {  // one prefix sum and one atomicAdd per round and workgroup
@@ -315,11 +317,14 @@ __device__ inline void list_append(uint32_t keep, const int32_t (&vid)[K], int32
     __syncthreads();  // sbase is rewritten by the next call
 }
 
+// mn (may be null): an FB-eliminated
+// vertex lowers mn[its pivot] to its own id (mn[v] = v before the run), so that
+// mn[root] is the minimum of each SCC whatever pivot the election chose.
 __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          const unsigned long long *__restrict__ prev, unsigned long long *cur,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
                          const uint16_t *__restrict__ score, Ctl *ctl, const int32_t *lin,
-                         const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt) {
+                         const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn) {
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t wslot[16];
     __shared__ unsigned long long wmin[16];
@@ -358,6 +363,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                     word[v] = w[k] | ST_E;
                     uint32_t pv = (uint32_t)prev[slot_of(c, slot_bits)];
                     root[v] = pv < (uint32_t)n ? (int32_t)pv : v;
+                    if (mn && pv < (uint32_t)n && v < mn[pv]) atomicMin(&mn[pv], v);
                     gone++;
                     continue;
                 }
@@ -684,6 +690,18 @@ __global__ void k_label_min(const int32_t *root, int32_t *mn, int32_t n) {
     }
 }
 
+// root[v] = mn[root[v]], in place: each work-item reads and writes only its own
+// root, and writes only where the minimum differs from the pivot
+__global__ void k_root_to_min(int32_t *root, const int32_t *__restrict__ mn, int32_t n) {
+    int64_t beg, end;
+    my_range(n, beg, end);
+#pragma unroll 8
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        const int32_t r = root[v], m = mn[r];
+        if (m != r) root[v] = m;
+    }
+}
+
 __global__ void k_label_final(const int32_t *root, const int32_t *mn, int32_t *label, int32_t n) {
     int64_t beg, end;
     my_range(n, beg, end);
@@ -723,13 +741,13 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl, uint32_t *vis,
-                 int64_t vis_words) {
+                 int64_t vis_words, int32_t *mn) {
     if ((n & 3) == 0)
         hipLaunchKernelGGL(k_init<4>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl, vis, vis_words);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn);
     else
         hipLaunchKernelGGL(k_init<1>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl, vis, vis_words);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
@@ -756,9 +774,9 @@ void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int gua
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score, Ctl *ctl,
-                   const int32_t *lin, const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt) {
+                   const int32_t *lin, const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn) {
     hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, elect,
-                       score, ctl, lin, lin_cnt, lout, lout_cnt);
+                       score, ctl, lin, lin_cnt, lout, lout_cnt, mn);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt,
@@ -793,6 +811,9 @@ void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *l
     hipLaunchKernelGGL(k_label_seed, GRID(l), root, mn, n);
     hipLaunchKernelGGL(k_label_min, GRID(l), root, mn, n);
     hipLaunchKernelGGL(k_label_final, GRID(l), root, mn, label, n);
+}
+void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n) {
+    hipLaunchKernelGGL(k_root_to_min, GRID(l), root, mn, n);
 }
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl) {

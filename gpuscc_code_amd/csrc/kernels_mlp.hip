@@ -32,12 +32,8 @@ constexpr int FEAT_ILP = 8;                // arcs per step of a feature row sca
 
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// round to nearest even on the upper 16 bits (finite inputs)
-__device__ inline uint16_t f_to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// round to nearest even (finite inputs): gfx950's v_cvt_pk_bf16_f32
+__device__ inline uint16_t f_to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ inline float bf16_round(float f) { return bf16_to_f(f_to_bf16(f)); }
 
@@ -238,21 +234,27 @@ __device__ inline float mlp_logit_mfma(const MlpRegs &R, const float (&x)[8]) {
     return part + R.b3;
 }
 
-// Features, scores and the learned election, one wave per 32 vertices of the
-// list lin[0 .. *lin_cnt) (all of [0, n) with lin null).  Lanes j and j + 32
-// share vertex j: the first half computes the degree features (0-3, 8-11; the
-// within-colour degrees by row scans, or, in the first election of a run,
-// where every arc joins two vertices of one colour, the degrees themselves),
-// the second half features 4-7 (2-hop sums, FB step, colour-set size).
-// Outputs, each optional: feat_out[v][12] (zeros for eliminated vertices),
-// score_out[v] = round(sigmoid(logit) * 65535) (0 when eliminated), and the
-// election key [tag][65535 - score][v] offered to cur[slot of the colour]
-// (lowest key wins).
+// Features, scores and the learned election, one wave per 64 vertices of the
+// list lin[0 .. *lin_cnt) (all of [0, n) with lin null): every lane loads and
+// computes the 12 features of its own vertex (the within-colour degrees by row
+// scans, or, in the first election of a run, where every arc joins two
+// vertices of one colour, the degrees themselves); the network then runs as
+// two MFMA tiles of 32 vertices, their operands gathered with shuffles, and
+// each tile's logit comes back to the lane of its vertex.  Outputs, each
+// optional: feat_out[v][12] (zeros for eliminated vertices), score_out[v] =
+// round(sigmoid(logit) * 65535) (0 when eliminated), and the election key
+// [tag][65535 - score][v] offered to cur[slot of the colour] (lowest key wins:
+// the highest score, then the lowest id).  The 16-bit probability ties the
+// most confident vertices, and the lowest id among them wins; ranking by the
+// logit itself instead (resolution 1/2048) elects the largest hub of R-MAT
+// scale 24, whose first reach level pushes its whole neighbourhood: one level
+// fewer, but 1.18 ms of reach instead of 0.87.
 struct ScoreArgs {
     const int32_t *row, *col, *rowT, *colT;
     const uint32_t *word;
     const float4 *stat;
     const uint32_t *cnt;
+    const uint32_t *one_set;  // not null: every live vertex is in one colour set of this size
     int slot_bits;
     float step_f;
     int static_deg;
@@ -275,73 +277,99 @@ __global__ __launch_bounds__(MLP_BLOCK) void k_score(ScoreArgs a) {
     const int64_t total = a.lin ? (int64_t)*a.lin_cnt : (int64_t)a.n;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / WAVE;
-    // the wave's pending minimum key of one slot, carried across tiles (flushed when
+    // the wave's pending minimum key of one slot, carried across rounds (flushed when
     // the slot changes and at the end)
     uint32_t pslot = 0xFFFFFFFFu;
     unsigned long long pmin = ~0ull;
-    for (int64_t base = wave * 32; base < total; base += nwaves * 32) {  // wave-uniform
-        const int64_t i = base + j;
+    // feature 7 of the last slot whose count the wave read (every lane asks for one
+    // slot in the first election: one read per wave instead of one per round)
+    uint32_t cslot = 0xFFFFFFFFu;
+    float cfeat = 0.f;
+    if (a.one_set) {
+        cslot = 0xFFFFFFFEu;
+        cfeat = log1pf((float)*a.one_set) * LOG_SCALE;
+    }
+    for (int64_t base = wave * WAVE; base < total; base += nwaves * WAVE) {  // wave-uniform
+        const int64_t i = base + l;
         const int32_t v = i < total ? (a.lin ? a.lin[i] : (int32_t)i) : -1;
         const uint32_t w = v >= 0 ? a.word[v] : ST_E;
         const bool live = !(w & ST_E);
-        float x[8];
+        float f[NF];
 #pragma unroll
-        for (int t = 0; t < 8; t++) x[t] = 0.f;
+        for (int k = 0; k < NF; k++) f[k] = 0.f;
+        const uint32_t sl = live ? slot_of(colour_of(w), a.slot_bits) : 0u;
+        if (!a.one_set) {  // the colour-set sizes: one read per wave while its slot repeats
+            const unsigned long long lm = __ballot(live);
+            const uint32_t s0 = lm ? __shfl(sl, __ffsll((long long)lm) - 1, WAVE) : 0u;
+            if (lm && __all(!live || sl == s0)) {
+                if (s0 != cslot) {
+                    cslot = s0;
+                    cfeat = log1pf((float)a.cnt[s0]) * LOG_SCALE;
+                }
+            } else if (lm) {
+                cslot = 0xFFFFFFFFu;
+                cfeat = live ? log1pf((float)a.cnt[sl]) * LOG_SCALE : 0.f;
+            }
+        }
         if (live) {
+            // the hardware log, exp and reciprocal (a few ulp): the kernel is bound by its
+            // VALU issue, and training reads its features from this same code
             const uint32_t c = colour_of(w);
             const float4 st = a.stat[v];
-            if (h == 0) {
-                const float dout = (float)(a.row[v + 1] - a.row[v]), din = (float)(a.rowT[v + 1] - a.rowT[v]);
-                float oc = dout, ic = din;
-                if (!a.static_deg) {
-                    oc = (float)live_same_colour(a.row, a.col, a.word, v, c);
-                    ic = (float)live_same_colour(a.rowT, a.colT, a.word, v, c);
-                }
-                x[0] = log1pf(oc) * LOG_SCALE;
-                x[1] = log1pf(ic) * LOG_SCALE;
-                x[2] = st.x * LOG_SCALE;
-                x[3] = st.y * LOG_SCALE;
-                x[4] = oc / (dout + 1.f);
-                x[5] = ic / (din + 1.f);
-                const float mx = oc > ic ? oc : ic, mn = oc > ic ? ic : oc;
-                x[6] = mn / (mx + 1.f);
-                x[7] = log1pf(oc * ic) * LOG_SCALE * 0.5f;
-            } else {
-                x[0] = st.z * LOG_SCALE * 0.5f;
-                x[1] = st.w * LOG_SCALE * 0.5f;
-                x[2] = a.step_f;
-                x[3] = log1pf((float)a.cnt[slot_of(c, a.slot_bits)]) * LOG_SCALE;
+            const float dout = (float)(a.row[v + 1] - a.row[v]), din = (float)(a.rowT[v + 1] - a.rowT[v]);
+            float oc = dout, ic = din;
+            if (!a.static_deg) {
+                oc = (float)live_same_colour(a.row, a.col, a.word, v, c);
+                ic = (float)live_same_colour(a.rowT, a.colT, a.word, v, c);
             }
+            f[0] = __logf(oc + 1.f) * LOG_SCALE;
+            f[1] = __logf(ic + 1.f) * LOG_SCALE;
+            f[2] = st.x * LOG_SCALE;
+            f[3] = st.y * LOG_SCALE;
+            f[4] = st.z * LOG_SCALE * 0.5f;
+            f[5] = st.w * LOG_SCALE * 0.5f;
+            f[6] = a.step_f;
+            f[7] = cfeat;
+            f[8] = __fdividef(oc, dout + 1.f);
+            f[9] = __fdividef(ic, din + 1.f);
+            const float mx = oc > ic ? oc : ic, mn = oc > ic ? ic : oc;
+            f[10] = __fdividef(mn, mx + 1.f);
+            f[11] = __logf(oc * ic + 1.f) * LOG_SCALE * 0.5f;
         }
         if (a.feat_out && v >= 0) {
-            float *f = a.feat_out + (int64_t)v * NF;
-            if (h == 0) {
+            float *o = a.feat_out + (int64_t)v * NF;
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    f[t] = x[t];
-                    f[8 + t] = x[4 + t];
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < 4; t++) f[4 + t] = x[t];
-            }
+            for (int k = 0; k < NF; k++) o[k] = f[k];
         }
         if (!a.params) continue;  // kernel-uniform
-        if (h == 1) x[4] = 1.f;   // feature 12: the bias input
-        const float s = mlp_logit_mfma(R, x);
+        float logit = 0.f;
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            // B operands of tile t: lane (h, j) takes features 4h.. and 8 + 4h.. of vertex 32t + j
+            const int src = 32 * t + j;
+            float g[NF];
+#pragma unroll
+            for (int k = 0; k < NF; k++) g[k] = __shfl(f[k], src, WAVE);
+            float x[8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                x[k] = h == 0 ? g[k] : g[4 + k];
+                x[4 + k] = h == 0 ? g[8 + k] : (k == 0 ? 1.f : 0.f);  // feature 12: the bias input
+            }
+            const float s = mlp_logit_mfma(R, x);  // vertex 32t + j, in lanes j and j + 32
+            if (h == t) logit = s;                  // lane 32t + j owns vertex 32t + j
+        }
         uint16_t sc = 0;
-        if (live) sc = (uint16_t)__float2uint_rn(65535.f / (1.f + __expf(-s)));
-        if (h == 0 && v >= 0 && a.score_out) a.score_out[v] = sc;
+        if (live) sc = (uint16_t)__float2uint_rn(__fdividef(65535.f, 1.f + __expf(-logit)));
+        if (v >= 0 && a.score_out) a.score_out[v] = sc;
         if (!a.cur) continue;  // kernel-uniform
-        // the election: one atomicMin per wave when its candidates share a slot
-        const bool cand = h == 0 && live;
-        const uint32_t sl = cand ? slot_of(colour_of(w), a.slot_bits) : 0u;
-        const unsigned long long key = cand ? (a.tag | ((unsigned long long)(0xFFFFu - sc) << 32) | (uint32_t)v)
+        // the election: one pending minimum per wave while its candidates share a slot
+        const unsigned long long key = live ? (a.tag | ((unsigned long long)(0xFFFFu - sc) << 32) | (uint32_t)v)
                                             : ~0ull;
-        const unsigned long long bm = __ballot(cand);
+        const unsigned long long bm = __ballot(live);
         if (bm == 0) continue;  // wave-uniform
         const uint32_t sl0 = __shfl(sl, __ffsll((long long)bm) - 1, WAVE);
-        if (__all(!cand || sl == sl0)) {
+        if (__all(!live || sl == sl0)) {
             unsigned long long m = key;
             for (int o = WAVE / 2; o > 0; o >>= 1) {
                 const unsigned long long y = __shfl_xor(m, o, WAVE);
@@ -354,7 +382,7 @@ __global__ __launch_bounds__(MLP_BLOCK) void k_score(ScoreArgs a) {
             } else {
                 pmin = m < pmin ? m : pmin;
             }
-        } else if (cand && key < a.cur[sl]) {
+        } else if (live && key < a.cur[sl]) {
             atomicMin(&a.cur[sl], key);
         }
     }
@@ -516,10 +544,10 @@ void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint3
     hipLaunchKernelGGL(k_colour_count, GRID(l), word, n, cnt, slot_bits, lin, lin_cnt, ctl);
 }
 void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
-                  int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
+                  const uint32_t *one_set, int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
                   const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
                   unsigned long long tag, const Ctl *ctl) {
-    ScoreArgs a{g.row, g.col, g.rowT, g.colT, word, stat, cnt, slot_bits, step_f, static_deg, g.n,
+    ScoreArgs a{g.row, g.col, g.rowT, g.colT, word, stat, cnt, one_set, slot_bits, step_f, static_deg, g.n,
                 lin, lin_cnt, params, score_out, feat_out, cur, tag, ctl};
     hipLaunchKernelGGL(k_score, GRID_MLP(l), a);
 }

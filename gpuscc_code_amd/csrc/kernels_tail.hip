@@ -282,6 +282,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
             sp.st(v, w | ST_E);
             const uint32_t pv = sp.prev_winner(t, r.pt, slot_of(c, t.slot_bits));
             t.root[g] = pv < (uint32_t)t.n ? (int32_t)pv : g;
+            if (t.mn && pv < (uint32_t)t.n && g < t.mn[pv]) atomicMin(&t.mn[pv], g);
             atomicAdd(&sh.red[0], 1ull);
             continue;
         }
