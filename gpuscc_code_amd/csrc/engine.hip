@@ -76,6 +76,7 @@ struct scc_ctx {
     bool uploaded = false, labels_valid = false;
     bool label_in_root = false;  // the last run's labels are the root array itself (plain rule)
     int guess[4] = {4, 4, 2, 4};  // chain batch guesses: fwd, bwd, trim after a reach, first trim
+    int guess_wcc = 4;            // the WCC pass's label chain (kept across runs)
     int64_t syncs = 0;
     int fail_alloc_at = 0;   // fault injection: the nth device allocation of the next upload fails
     int alloc_count = 0;
@@ -568,9 +569,8 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = fals
     ctl_op(c, 16);
     launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
     launch_wcc_init(l, c->d_tmp, c->g.n);
-    int wg = 4;
     CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
-                  wg, 0, iters));
+                  c->guess_wcc, 0, iters));
     launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
     return do_elect(c, 0, npiv, learned, sync);
 }
@@ -948,6 +948,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         bool wcc_done = false;
+        bool unsynced = false;  // the last election's pivot count is not read yet
         const int64_t one_pct = n / 100;
         // schedules that switch once n/100 vertices are FB-eliminated: the
         // reference's HYBRID (t, then q; scc.cu:493-563), COMBINE (l, then q;
@@ -961,8 +962,31 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // chain may need more levels; every later kernel of the step then exits
         // at once, and the host resumes that chain and finishes the step with
         // waits.  The results are those of the waiting path.
-        const bool spec_ok = variant == SCC_VARIANT_PLAIN && !c->opts.sequential_reach && !by_elim &&
-                             c->prof_flags == 0;
+        const bool spec_ok = !c->opts.sequential_reach && !by_elim && c->prof_flags == 0;
+        // scc_w / scc_2_w (scc.cu:787-821): once n/100 vertices are FB-eliminated, the
+        // pivots just elected are unset, scc_2_w peels size-2 SCCs, and one WCC pass gives
+        // fresh colours and a new election.  The eliminated count is the device's (updates
+        // add to it), read with the step's wait; the pair count arrives with the WCC
+        // chain's first wait; with sync false the new election's pivot count is read by
+        // the next step (after_wcc: no pivot there means the run is over).
+        bool after_wcc = false;
+        auto wcc_pass = [&](bool sync) -> scc_status {
+            wcc_done = true;
+            st.wcc_step = st.steps;
+            launch_unset_pivots(launch_for(c), c->d_word, n, c->d_ctl);
+            if (variant == SCC_VARIANT_2W) {
+                ctl_op(c, 8);
+                launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
+            }
+            int64_t wi = 0;
+            CHK(do_wcc(c, &wi, &npiv, learned, sync));
+            if (variant == SCC_VARIANT_2W) st.trim2_pairs += (int64_t)c->h_ctl->count;
+            if (!sync) {
+                unsynced = true;
+                after_wcc = true;
+            }
+            return SCC_OK;
+        };
         auto tp = std::chrono::steady_clock::now();
         // With speculative steps the rest of trim-1 goes out without a wait as well:
         // its chain (as long as trim-1's in the last run), a guard (stage 3), the first
@@ -971,7 +995,6 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // first two elections.
         bool spec_t1 = spec_ok && ((c->epoch + 2) & 0xFFFFu) >= 2u;
         const int Lt1 = std::max(1, std::min(c->guess[3], 1024));
-        bool unsynced = false;
         if (spec_t1) {
             if (!c->spec_ev[0])
                 for (auto &e : c->spec_ev) HIPCHK(c, hipEventCreate(&e));
@@ -1003,10 +1026,19 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             }
         }
         while (unsynced || npiv > 0) {
+            // a WCC election left unread, and the next step waits anyway (an election tag
+            // wrap): read its pivot count here
+            if (unsynced && after_wcc && !(spec_ok && ((c->epoch + 1) & 0xFFFFu) >= 2u)) {
+                CHK(read_ctl(c));
+                npiv = c->h_ctl->npiv;
+                unsynced = after_wcc = false;
+                if (npiv == 0) break;
+            }
             // The tail: once few vertices are live, the remaining steps run in one
             // workgroup and one launch (k_fb_tail); it stops early only before an
             // election tag would wrap, and the loop then goes on as before.
-            if (spec_ok && !unsynced && c->lsel >= 0 && c->opts.tail_live >= 0 &&
+            if (spec_ok && !unsynced && (variant == SCC_VARIANT_PLAIN || wcc_done) && c->lsel >= 0 &&
+                c->opts.tail_live >= 0 &&
                 c->h_ctl->live <= (c->opts.tail_live ? (uint32_t)c->opts.tail_live : kTailLive)) {
                 tp = std::chrono::steady_clock::now();
                 TailArgs ta{};
@@ -1129,13 +1161,19 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 // when it ran, moved those counts to the _prev fields)
                 const int64_t np = elected ? h.npiv_prev : h.npiv;
                 if (unsynced) {
-                    st.first_pivot = (int64_t)(elected ? h.first_piv_prev : h.first_piv);
+                    if (!after_wcc) st.first_pivot = (int64_t)(elected ? h.first_piv_prev : h.first_piv);
                     unsynced = false;
                 }
                 if (np == 0) {
                     ctl_op(c, 512);
-                    return fail(c, SCC_E_STATE, "live vertices left but no pivot elected");
+                    if (!after_wcc) return fail(c, SCC_E_STATE, "live vertices left but no pivot elected");
+                    // the WCC pass left no live vertex: the step found nothing
+                    st.steps--;
+                    after_wcc = false;
+                    npiv = 0;
+                    continue;
                 }
+                after_wcc = false;
                 st.pivots_total += np;
                 int64_t dr = h.depth[0], dt = h.depth[1];
                 if (elected) {
@@ -1168,6 +1206,13 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                 }
                 c->spec_reach[k] = (int)dr;
                 c->spec_trim[k] = (int)dt;
+                if (variant != SCC_VARIANT_PLAIN && !wcc_done) {
+                    ne = (int64_t)(elected ? h.fbelim : c->h_ctl->fbelim);
+                    if (ne >= one_pct) {
+                        if (ne == n) break;
+                        CHK(wcc_pass(false));
+                    }
+                }
                 continue;
             }
             if (c->opts.sequential_reach) {  // scc.cu:285-704 order: forward, then backward
@@ -1199,19 +1244,10 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             c->cur_step = (int)st.steps;
             CHK(do_elect(c, 1, &npiv, learned));
             if (variant != SCC_VARIANT_PLAIN && !wcc_done) {
-                CHK(do_count(c, 0, &ne));
-                if (ne >= one_pct) {  // scc.cu:787-821
+                CHK(do_count(c, 0, &ne));  // countElimmed (scc.cu:42-51, 787-790)
+                if (ne >= one_pct) {       // scc.cu:787-821
                     if (ne == n) { st.pivots_total += npiv; st.ms_update += ms_since(tp); break; }
-                    wcc_done = true;
-                    st.wcc_step = st.steps;
-                    launch_unset_pivots(l, c->d_word, n, c->d_ctl);
-                    if (variant == SCC_VARIANT_2W) {
-                        int64_t pairs = 0;
-                        CHK(do_trim2(c, &pairs));
-                        st.trim2_pairs += pairs;
-                    }
-                    int64_t wi = 0;
-                    CHK(do_wcc(c, &wi, &npiv, learned));
+                    CHK(wcc_pass(true));
                 }
             } else if (by_elim && ne < one_pct) {
                 CHK(do_count(c, 0, &ne));  // scc.cu:879-890

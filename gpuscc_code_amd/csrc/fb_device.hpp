@@ -49,6 +49,7 @@ struct Ctl {
     uint32_t first_piv;    // lowest pivot of the last election
     uint32_t npiv_prev, first_piv_prev;  // npiv, first_piv of the election before the last
     uint32_t pcount;       // vertices holding P (class representatives: #SCC at the end)
+    uint32_t fbelim;       // vertices FB-eliminated by updates (countElimmed before the WCC pass)
     uint32_t abort;        // speculative step: chain stage that did not converge (0 = none);
                            // every gated kernel exits at once while it is set
     uint32_t depth[3];     // speculative step: levels of its reach chain and trim chain; of trim-1's chain

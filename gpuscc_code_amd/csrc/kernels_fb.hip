@@ -232,6 +232,7 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
     if (what & 1024) {  // before k_init
         ctl->live = 0;
         ctl->pcount = 0;
+        ctl->fbelim = 0;
     }
     if (what & 64) {  // the pivot list seeds both directions
         ctl->seeds[0] = ctl->npiv;
@@ -439,7 +440,10 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
     if (wslot[0] == 0u && lane == 0 && pslot != 0xFFFFFFFFu && pmin < cur[pslot]) atomicMin(&cur[pslot], pmin);
     if (recolour && ctl) {
         gone = block_sum(gone, sh);
-        if (threadIdx.x == 0 && gone) atomicSub(&ctl->live, (uint32_t)gone);
+        if (threadIdx.x == 0 && gone) {
+            atomicSub(&ctl->live, (uint32_t)gone);
+            atomicAdd(&ctl->fbelim, (uint32_t)gone);
+        }
     }
 }
 

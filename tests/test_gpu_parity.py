@@ -526,29 +526,31 @@ def test_batch_rejects_crossing_arcs(ctx):
 @pytest.mark.parametrize("sched", SCHEDS)
 def test_speculative_steps_match_waiting_path(ctx, sched):
     """A step launched without waits (guards abort a chain that may need more
-    levels; the host resumes it) ends exactly where the waiting path ends: the
+    levels; the host resumes it), in every variant (the WCC pass of scc_w and
+    scc_2_w follows a speculative step), ends exactly where the waiting path ends: the
     first run after an upload guesses the chain lengths from other graphs (and
     so exercises the resume path), the second uses the lengths of the first."""
     # level counts are left out: with the workgroup-local rounds of d, l and h
     # they depend on timing (the reached set of every chain does not)
-    keys = ("steps", "num_scc", "pivots_total", "first_pivot", "trivial", "biggest")
+    keys = ("steps", "num_scc", "pivots_total", "first_pivot", "trivial", "biggest", "wcc_step", "trim2_pairs")
     for graphs in (0, 1):  # reach chains launched level by level / replayed as captured HIP graphs
         ctx.set_opts(scc.alg_opt(reach_graphs=graphs))
         for g in (demo_graph(), scc.gen_rmat(13, 8, seed=21), scc.gen_mesh(96, 0.6, seed=5), cycle(3000),
                   path_dag(500), cycle_with_tails()):
             ref, _ = oracle.tarjan(g.row, g.col)
             ctx.upload(g)
-            ctx.profile(2)  # event timing: every chain waits
-            lab0, st0 = ctx.run(sched)
-            w0, r0 = ctx.get_state()
-            ctx.profile(0)
-            for _ in range(3):  # replays with eager work and waits between them
-                lab, st = ctx.run(sched)
-                w, r = ctx.get_state()
-                assert np.array_equal(lab, ref) and np.array_equal(lab0, ref)
-                assert np.array_equal(w, w0) and np.array_equal(r, r0)
-                assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}
-            ctx.profile_read(reset=True)
+            for variant in (0, 1, 2):  # scc, scc_w, scc_2_w (the WCC pass inside speculative steps)
+                ctx.profile(2)  # event timing: every chain waits
+                lab0, st0 = ctx.run(sched, variant)
+                w0, r0 = ctx.get_state()
+                ctx.profile(0)
+                for _ in range(3):  # replays with eager work and waits between them
+                    lab, st = ctx.run(sched, variant)
+                    w, r = ctx.get_state()
+                    assert np.array_equal(lab, ref) and np.array_equal(lab0, ref)
+                    assert np.array_equal(w, w0) and np.array_equal(r, r0), f"variant {variant}"
+                    assert {k: st[k] for k in keys} == {k: st0[k] for k in keys}, f"variant {variant}"
+                ctx.profile_read(reset=True)
     ctx.set_opts(scc.alg_opt())
 
 
