@@ -567,11 +567,15 @@ scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
 scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = false, bool sync = true) {
     Launch l = launch_for(c);
     ctl_op(c, 16);
-    launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);
-    launch_wcc_init(l, c->d_tmp, c->g.n);
-    CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) { launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it); },
-                  c->guess_wcc, 0, iters));
-    launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl);
+    launch_max_colour(l, c->d_word, c->g.n, c->d_ctl);  // over every vertex, as thrust::reduce (scc.cu:802)
+    // the live vertices: the list of the last election when the run keeps one
+    const int32_t *lin = c->use_lists && c->lsel >= 0 ? c->d_live[c->lsel] : nullptr;
+    const uint32_t *lin_cnt = lin ? &c->d_ctl->lcnt[c->lsel] : nullptr;
+    launch_wcc_init(l, c->d_tmp, c->g.n, lin, lin_cnt);
+    CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) {
+            launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it, lin, lin_cnt);
+        }, c->guess_wcc, 0, iters));
+    launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl, lin, lin_cnt);
     return do_elect(c, 0, npiv, learned, sync);
 }
 
@@ -973,7 +977,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         auto wcc_pass = [&](bool sync) -> scc_status {
             wcc_done = true;
             st.wcc_step = st.steps;
-            launch_unset_pivots(launch_for(c), c->d_word, n, c->d_ctl);
+            launch_unset_pivots(launch_for(c), c->d_word, n, c->d_ctl, c->d_piv);  // the pivot list
             if (variant == SCC_VARIANT_2W) {
                 ctl_op(c, 8);
                 launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
@@ -988,6 +992,59 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             return SCC_OK;
         };
         auto tp = std::chrono::steady_clock::now();
+        // The tail: the remaining FB steps in one workgroup and one launch (k_fb_tail).
+        // gate 0: the host has read that few vertices are live; otherwise the kernel
+        // checks the device's live count against gate and does nothing above it (status
+        // 3).  Either way *npiv is the last election's pivot count afterwards.
+        const uint32_t tail_max = c->opts.tail_live ? (uint32_t)c->opts.tail_live : kTailLive;
+        auto run_tail = [&](uint32_t gate) -> scc_status {
+            auto t0 = std::chrono::steady_clock::now();
+            TailArgs ta{};
+            ta.row = c->g.row;
+            ta.col = c->g.col;
+            ta.rowT = c->g.rowT;
+            ta.colT = c->g.colT;
+            ta.word = c->d_word;
+            ta.root = c->d_root;
+            ta.n = n;
+            ta.slots[0] = c->d_slots[0];
+            ta.slots[1] = c->d_slots[1];
+            ta.slot_bits = c->slot_bits;
+            ta.prev_tab = c->prev_tab;
+            ta.epoch = c->epoch;
+            ta.piv = c->d_piv;
+            ta.list[0] = c->d_live[0];
+            ta.list[1] = c->d_live[1];
+            ta.lsel = c->lsel;
+            for (int i = 0; i < 4; i++) ta.q[i] = c->d_q[i];
+            ta.ctl = c->d_ctl;
+            ta.max_steps = gate ? 0u : c->h_ctl->live + 2;  // each step eliminates its pivots
+            ta.gate_live = gate;
+            ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
+            ta.mn = c->track_mn ? c->d_mn : nullptr;
+            if (c->pend_stage || c->pend_bits) ctl_op(c, 0);  // queued resets go out first
+            launch_fb_tail(c->stream, ta);
+            CHK(read_ctl(c));
+            const Ctl &h = *c->h_ctl;
+            npiv = h.npiv;
+            if (h.tail[1] == 3) return SCC_OK;  // not run: too many live vertices
+            if (h.tail[1] == 2) return fail(c, SCC_E_STATE, "tail steps did not converge");
+            const uint32_t ts = h.tail[0];
+            st.tail_live = h.tail_live0;
+            st.steps += ts;
+            st.tail_steps += ts;
+            st.pivots_total += (int64_t)h.tail_pivots;
+            st.fwd_depths += h.tail[2];
+            st.bwd_depths += h.tail[2];
+            st.trim_iters += h.tail[3];
+            st.ms_fwd += ms_since(t0);
+            c->epoch += ts;
+            if (ts & 1u) {
+                c->prev_tab ^= 1;
+                c->lsel ^= 1;
+            }
+            return SCC_OK;
+        };
         // With speculative steps the rest of trim-1 goes out without a wait as well:
         // its chain (as long as trim-1's in the last run), a guard (stage 3), the first
         // election and the first step, whose wait tells whether trim-1 converged.
@@ -1038,51 +1095,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             // workgroup and one launch (k_fb_tail); it stops early only before an
             // election tag would wrap, and the loop then goes on as before.
             if (spec_ok && !unsynced && (variant == SCC_VARIANT_PLAIN || wcc_done) && c->lsel >= 0 &&
-                c->opts.tail_live >= 0 &&
-                c->h_ctl->live <= (c->opts.tail_live ? (uint32_t)c->opts.tail_live : kTailLive)) {
-                tp = std::chrono::steady_clock::now();
-                TailArgs ta{};
-                ta.row = c->g.row;
-                ta.col = c->g.col;
-                ta.rowT = c->g.rowT;
-                ta.colT = c->g.colT;
-                ta.word = c->d_word;
-                ta.root = c->d_root;
-                ta.n = n;
-                ta.slots[0] = c->d_slots[0];
-                ta.slots[1] = c->d_slots[1];
-                ta.slot_bits = c->slot_bits;
-                ta.prev_tab = c->prev_tab;
-                ta.epoch = c->epoch;
-                ta.piv = c->d_piv;
-                ta.list[0] = c->d_live[0];
-                ta.list[1] = c->d_live[1];
-                ta.lsel = c->lsel;
-                for (int i = 0; i < 4; i++) ta.q[i] = c->d_q[i];
-                ta.ctl = c->d_ctl;
-                ta.max_steps = c->h_ctl->live + 2;  // each step eliminates its pivots
-                ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
-                ta.mn = c->track_mn ? c->d_mn : nullptr;
-                st.tail_live = c->h_ctl->live;
-                if (c->pend_stage || c->pend_bits) ctl_op(c, 0);  // queued resets go out first
-                launch_fb_tail(c->stream, ta);
-                CHK(read_ctl(c));
-                const Ctl &h = *c->h_ctl;
-                if (h.tail[1] == 2) return fail(c, SCC_E_STATE, "tail steps did not converge");
-                const uint32_t ts = h.tail[0];
-                st.steps += ts;
-                st.tail_steps += ts;
-                st.pivots_total += (int64_t)h.tail_pivots;
-                st.fwd_depths += h.tail[2];
-                st.bwd_depths += h.tail[2];
-                st.trim_iters += h.tail[3];
-                st.ms_fwd += ms_since(tp);
-                c->epoch += ts;
-                if (ts & 1u) {
-                    c->prev_tab ^= 1;
-                    c->lsel ^= 1;
-                }
-                npiv = h.npiv;
+                c->opts.tail_live >= 0 && c->h_ctl->live <= tail_max) {
+                CHK(run_tail(0));
                 if (npiv == 0) break;
             }
             st.steps++;
@@ -1211,6 +1225,12 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     if (ne >= one_pct) {
                         if (ne == n) break;
                         CHK(wcc_pass(false));
+                        // the tail right behind the WCC election, gated on the device's live
+                        // count; one wait reads the election's pivot count either way
+                        if (c->opts.tail_live >= 0 && ((c->epoch + 1) & 0xFFFFu) >= 2u) {
+                            CHK(run_tail(tail_max));
+                            unsynced = after_wcc = false;
+                        }
                     }
                 }
                 continue;

@@ -66,6 +66,7 @@ struct Ctl {
     uint32_t tail[4];        // k_fb_tail: steps run, status (0 done, 1 host continues, 2 error),
                              // reach levels and trim rounds
     unsigned long long tail_pivots;  // k_fb_tail: pivots elected
+    uint32_t tail_live0;             // k_fb_tail: live vertices when it started
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

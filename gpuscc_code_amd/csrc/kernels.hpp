@@ -134,13 +134,16 @@ void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned l
                     const uint32_t *lin_cnt = nullptr, uint32_t *vis = nullptr, int64_t vis_words = 0);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl);
-void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl);
+// piv (may be null): visit only the pivot list of the last election (ctl->npiv entries)
+void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl, const int32_t *piv = nullptr);
 void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl);
-void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n);
+// the WCC pass over the live list lin[0 .. *lin_cnt) (lin null: all n)
+void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n, const int32_t *lin = nullptr,
+                     const uint32_t *lin_cnt = nullptr);
 void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,
-                     int iter);
+                     int iter, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr);
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
-                         const Ctl *ctl);
+                         const Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr);
 void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n);
 // root[v] = mn[root[v]] in place (mn kept by the run's updates, see k_update)
 void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n);
@@ -165,6 +168,8 @@ struct TailArgs {
     int32_t *q[4];  // queues, n entries each
     Ctl *ctl;
     uint32_t max_steps;  // live vertices + 2: bounds the steps, and each step's levels and trim rounds
+                         // (0: the device's live count + 2)
+    uint32_t gate_live;  // not 0: run only when at most this many vertices are live (else status 3)
     int xbits;           // the run's schedule marks expanded vertices (q, t: FExt / BExt)
     int32_t *mn;         // may be null: FB-eliminated vertices lower mn[their pivot] (see k_update)
     int first_step;      // no pivots yet: the first step is trim-1 and the first election (lowest live id)

@@ -538,12 +538,15 @@ __global__ void k_max_colour(const uint32_t *word, int32_t n, Ctl *ctl) {
 }
 
 // state 19 -> 0: unsetPivots_kernel (scc_routines.cu:10-19)
-__global__ void k_unset_pivots(uint32_t *word, int32_t n, Ctl *ctl) {
+// With piv (the pivot list of the last election, ctl->npiv entries) only those
+// vertices are visited: after an election they are the only ones in state 19.
+__global__ void k_unset_pivots(uint32_t *word, int32_t n, Ctl *ctl, const int32_t *piv) {
     __shared__ unsigned long long sh[16];
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(piv ? (int64_t)ctl->npiv : (int64_t)n, beg, end);
     unsigned long long unset = 0;
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const int32_t v = piv ? piv[i] : (int32_t)i;
         uint32_t w = word[v];
         if ((w & 0xFFu) == 19u) {
             word[v] = w & ~0xFFu;
@@ -621,23 +624,28 @@ __global__ void k_trim2_commit(uint32_t *word, int32_t n, Ctl *ctl) {
 // ---------------------------------------------------------------------------
 // one WCC pass: computeWCC (wcc.cu:7-52) to its fixpoint gives each live v
 // the minimum id among live vertices it reaches over same-colour out-edges;
-// pointer jumping (wcc.cu:42-46) shortcuts chains.
+// pointer jumping (wcc.cu:42-46) shortcuts chains.  With a live-vertex list
+// (lin, *lin_cnt; null: all n) only the listed vertices are visited: labels are
+// ids of live vertices, so no other entry of lab is ever read.
 // ---------------------------------------------------------------------------
-__global__ void k_wcc_init(int32_t *lab, int32_t n) {
+__global__ void k_wcc_init(int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt) {
     int64_t beg, end;
-    my_range(n, beg, end);
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) lab[v] = (int32_t)v;
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
+    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const int32_t v = lin ? lin[i] : (int32_t)i;
+        lab[v] = v;
+    }
 }
 
 __global__ void k_wcc_iter(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                            const uint32_t *__restrict__ word, int32_t *lab, int32_t n, Ctl *ctl,
-                           int iter) {
+                           int iter, const int32_t *lin, const uint32_t *lin_cnt) {
     if (chain_exit(ctl, iter)) return;
     int64_t beg, end;
-    my_range(n, beg, end);
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
     bool found = false;
-    for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
-        int32_t v = (int32_t)vv;
+    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        int32_t v = lin ? lin[i] : (int32_t)i;
         uint32_t w = word[v];
         if (w & ST_E) continue;
         uint32_t c = colour_of(w);
@@ -661,11 +669,13 @@ __global__ void k_wcc_iter(const int32_t *__restrict__ row, const int32_t *__res
 
 // fresh colours per class (update_ranges2/3, wcc.cu:79-144): every member of
 // the class rooted at r takes (maxc + 1 + r) & mask, state 0.
-__global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, const Ctl *ctl) {
+__global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, const Ctl *ctl, const int32_t *lin,
+                               const uint32_t *lin_cnt) {
     uint32_t maxc = ctl->maxc;
     int64_t beg, end;
-    my_range(n, beg, end);
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+    my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
+    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const int32_t v = lin ? lin[i] : (int32_t)i;
         uint32_t w = word[v];
         if (w & ST_E) continue;
         word[v] = make_word((maxc + 1u + (uint32_t)lab[v]) & COLOUR_MASK, 0);
@@ -793,23 +803,23 @@ void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int wh
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl) {
     hipLaunchKernelGGL(k_max_colour, GRID(l), word, n, ctl);
 }
-void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl) {
-    hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n, ctl);
+void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl, const int32_t *piv) {
+    hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n, ctl, piv);
 }
 void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl) {
     hipLaunchKernelGGL(k_trim2_mark, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n);
     hipLaunchKernelGGL(k_trim2_commit, GRID(l), word, g.n, ctl);
 }
-void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n) {
-    hipLaunchKernelGGL(k_wcc_init, GRID(l), lab, n);
+void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt) {
+    hipLaunchKernelGGL(k_wcc_init, GRID(l), lab, n, lin, lin_cnt);
 }
 void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,
-                     int iter) {
-    hipLaunchKernelGGL(k_wcc_iter, GRID(l), g.row, g.col, word, lab, g.n, ctl, iter);
+                     int iter, const int32_t *lin, const uint32_t *lin_cnt) {
+    hipLaunchKernelGGL(k_wcc_iter, GRID(l), g.row, g.col, word, lab, g.n, ctl, iter, lin, lin_cnt);
 }
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
-                         const Ctl *ctl) {
-    hipLaunchKernelGGL(k_wcc_recolour, GRID(l), word, lab, n, ctl);
+                         const Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
+    hipLaunchKernelGGL(k_wcc_recolour, GRID(l), word, lab, n, ctl, lin, lin_cnt);
 }
 void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n) {
     hipLaunchKernelGGL(k_label_seed, GRID(l), root, mn, n);

@@ -457,6 +457,17 @@ __device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShar
 __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
     __shared__ TailShared sh;
     Ctl *ctl = t.ctl;
+    const uint32_t live0 = ctl->live;  // block-uniform: no work-item writes ctl before a barrier
+    if (t.gate_live && live0 > t.gate_live) {  // launched without a host wait: too many live vertices
+        if (threadIdx.x == 0) {
+            ctl->tail[0] = 0;
+            ctl->tail[1] = 3;
+            ctl->tail[2] = ctl->tail[3] = 0;
+            ctl->tail_pivots = 0;
+        }
+        return;
+    }
+    if (t.max_steps == 0) t.max_steps = live0 + 2;  // each step eliminates its pivots
     TailRun r{t.lsel, t.prev_tab, t.epoch, ctl->npiv, 0, 0};
     uint32_t steps = 0, status = 0, try_below = 0xFFFFFFFFu;
     unsigned long long pivots = 0;
@@ -510,6 +521,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
     }
     if (local) tail_writeback(t, ls, sh, r.lsel, r.npiv, r.pt, r.epoch);
     if (threadIdx.x == 0) {
+        ctl->tail_live0 = live0;
         ctl->tail[0] = steps;
         ctl->tail[1] = status;
         ctl->tail[2] = (uint32_t)r.depths;
