@@ -57,6 +57,9 @@ struct scc_ctx {
     uint16_t *d_score = nullptr;  // learned-rule scores (per vertex), may be null
     int32_t *d_goff = nullptr;    // batch: graph offsets [num_graphs + 1]
     int32_t num_graphs = 0;
+    int32_t *d_border = nullptr;      // batch: graphs by decreasing size (the persistent solver's order)
+    uint64_t *d_btab_off = nullptr;   // batch: each graph's election tables in d_btab (words)
+    uint32_t *d_btab = nullptr;
     // learned pivot scorer
     float4 *d_stat = nullptr;     // static features per vertex (computed once per upload)
     uint32_t *d_cnt = nullptr;    // live vertices per colour slot
@@ -608,6 +611,14 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+void free_batch(scc_ctx *c) {
+    for (void *p : {(void *)c->d_border, (void *)c->d_btab_off, (void *)c->d_btab})
+        if (p) (void)hipFree(p);
+    c->d_border = nullptr;
+    c->d_btab_off = nullptr;
+    c->d_btab = nullptr;
+}
+
 void free_graph(scc_ctx *c) {
     clear_graphs(c);  // captured launches point at the buffers freed here
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
@@ -646,6 +657,7 @@ void free_graph(scc_ctx *c) {
     c->d_feat = nullptr;
     if (c->d_goff) (void)hipFree(c->d_goff);
     c->d_goff = nullptr;
+    free_batch(c);
     c->num_graphs = 0;
     c->uploaded = false;
     c->labels_valid = false;
@@ -935,7 +947,47 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t_start = std::chrono::steady_clock::now();
     scc_status s = SCC_OK;
+    // a batch with the plain variant and rule: every graph solved on its own in one
+    // persistent launch (k_fb_batch); the graphs' schedule is the tail's (queue reach)
+    const bool batch_solve = c->num_graphs > 0 && variant == SCC_VARIANT_PLAIN && !learned &&
+                             c->opts.batch_solver == 1 && schedule != SCC_SCHED_C && schedule != SCC_SCHED_B &&
+                             !(schedule == SCC_SCHED_H && c->opts.hybrid_policy == 1);
+    auto solve_batch = [&]() -> scc_status {
+        c->pend_bits = c->pend_stage = 0;
+        ctl_op(c, 512 | 1024 | 8192);
+        TailArgs ta{};
+        ta.row = c->g.row;
+        ta.col = c->g.col;
+        ta.rowT = c->g.rowT;
+        ta.colT = c->g.colT;
+        ta.word = c->d_word;
+        ta.root = c->d_root;
+        ta.n = n;
+        ta.slot_bits = c->slot_bits;
+        ta.piv = c->d_piv;
+        ta.list[0] = c->d_live[0];
+        ta.list[1] = c->d_live[1];
+        for (int i = 0; i < 4; i++) ta.q[i] = c->d_q[i];
+        ta.ctl = c->d_ctl;
+        ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
+        const BatchArgs ba{c->num_graphs, c->d_goff, c->d_border, c->d_btab, c->d_btab_off};
+        // one workgroup per CU: each holds the 150 KB LDS copy of a small graph
+        launch_fb_batch(c->stream, (uint32_t)c->cu_count, ta, ba);
+        CHK(read_ctl(c));
+        const Ctl &h = *c->h_ctl;
+        if (h.tail[1] == 2) return fail(c, SCC_E_STATE, "batch steps did not converge");
+        st.steps = h.tail[0];
+        st.tail_steps = h.tail[0];
+        st.pivots_total = (int64_t)h.tail_pivots;
+        st.fwd_depths = st.bwd_depths = h.tail[2];
+        st.trim_iters = h.tail[3];
+        st.first_pivot = h.first_piv == 0xFFFFFFFFu ? -1 : (int64_t)h.first_piv;
+        st.num_scc = (int64_t)h.pcount;
+        c->label_in_root = true;  // lowest-id pivots: the roots are the labels
+        return SCC_OK;
+    };
     auto body = [&]() -> scc_status {
+        if (batch_solve && n > 0) return solve_batch();
         Launch l = launch_for(c);
         // k_init runs the first level of trim1 (a degree test) on the words it
         // writes and lists, as the input of trim1's first level, the live
@@ -1345,6 +1397,7 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
     c->spec_trim.clear();
     if (c->d_goff) HIPCHK(c, hipFree(c->d_goff));
     c->d_goff = nullptr;
+    free_batch(c);
     c->num_graphs = 0;
     if (G == 0) return SCC_OK;
     if (G < 0 || !off) return fail(c, SCC_E_INPUT, "bad batch");
@@ -1363,6 +1416,23 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
             return fail(c, SCC_E_INPUT, "an arc crosses graphs of the batch");
         }
     }
+    // the persistent solver: graphs by decreasing vertex count (the largest start first,
+    // so that no workgroup is left with a big graph at the end) and their election tables
+    std::vector<int32_t> order((size_t)G);
+    std::vector<uint64_t> toff((size_t)G);
+    for (int32_t i = 0; i < G; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+    uint64_t words = 0;
+    for (int32_t i = 0; i < G; i++) {
+        toff[i] = words;
+        words += batch_table_words(off[i + 1] - off[i]);
+    }
+    HIPCHK(c, hipMalloc(&c->d_border, sizeof(int32_t) * (size_t)G));
+    HIPCHK(c, hipMalloc(&c->d_btab_off, sizeof(uint64_t) * (size_t)G));
+    HIPCHK(c, hipMalloc(&c->d_btab, sizeof(uint32_t) * (size_t)std::max<uint64_t>(words, 1)));
+    HIPCHK(c, hipMemcpy(c->d_border, order.data(), sizeof(int32_t) * (size_t)G, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_btab_off, toff.data(), sizeof(uint64_t) * (size_t)G, hipMemcpyHostToDevice));
     c->num_graphs = G;
     return SCC_OK;
 }

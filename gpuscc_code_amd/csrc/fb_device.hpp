@@ -67,6 +67,7 @@ struct Ctl {
                              // reach levels and trim rounds
     unsigned long long tail_pivots;  // k_fb_tail: pivots elected
     uint32_t tail_live0;             // k_fb_tail: live vertices when it started
+    uint32_t wnext;                  // k_fb_batch: the next graph to take (work counter)
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

@@ -170,6 +170,7 @@ struct TailArgs {
     uint32_t max_steps;  // live vertices + 2: bounds the steps, and each step's levels and trim rounds
                          // (0: the device's live count + 2)
     uint32_t gate_live;  // not 0: run only when at most this many vertices are live (else status 3)
+    int32_t vbase;       // the queues are slices starting at this vertex (a batch graph's first), else 0
     int xbits;           // the run's schedule marks expanded vertices (q, t: FExt / BExt)
     int32_t *mn;         // may be null: FB-eliminated vertices lower mn[their pivot] (see k_update)
     int first_step;      // no pivots yet: the first step is trim-1 and the first election (lowest live id)
@@ -177,6 +178,18 @@ struct TailArgs {
                          // (else hand back to the host, status 1)
 };
 void launch_fb_tail(hipStream_t s, const TailArgs &t);
+// a batch in one persistent launch (k_fb_batch): graph g owns vertices [off[g], off[g + 1]);
+// workgroups take graphs in the order `order` (largest first) from ctl->wnext; graph g's
+// two election tables are tab[tab_off[g] ...] (2 * batch_table_size(n_g) words each)
+struct BatchArgs {
+    int32_t G;
+    const int32_t *off;
+    const int32_t *order;
+    uint32_t *tab;
+    const uint64_t *tab_off;
+};
+uint32_t batch_table_words(int32_t n_g);  // 4 * batch_table_size(n_g)
+void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const BatchArgs &b);
 
 // kernels_mlp.hip -- learned pivot scorer
 void launch_static_features(const Launch &l, const Graph &g, float4 *out);

@@ -276,6 +276,12 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
         ctl->st_edges = 0;
         ctl->st_visited = 0;
     }
+    if (what & 8192) {  // before a batch launch: work counter, statistics, first pivot
+        ctl->wnext = 0;
+        for (int i = 0; i < 4; i++) ctl->tail[i] = 0;
+        ctl->tail_pivots = 0;
+        ctl->first_piv = 0xFFFFFFFFu;
+    }
 }
 
 // With publish (the host's pinned mirror of the block), the block is copied

@@ -67,9 +67,52 @@ struct GlobalSlots {
     }
 };
 
+constexpr uint32_t TAB_EMPTY = 0xFFFFFFFFu;  // no slot id (slot ids are below 2^24)
+
+// The election of one graph of a batch through tables of its own in global
+// memory: open addressing keyed by slot id, H entries (a power of two above 1.5
+// times the graph's vertices, so every distinct slot has a place) of (slot id,
+// lowest id), cleared each step.  Colours of one graph collide only with each
+// other, as they would if the graph were solved alone.
+struct GraphSlots {
+    uint32_t *tab[2];  // 2H words each: slot ids, then lowest ids
+    uint32_t H;
+    __device__ void clear_cur(const TailArgs &, int pt, uint32_t tid) const {
+        uint32_t *e = tab[pt ^ 1];
+        for (uint32_t i = tid; i < 2u * H; i += TAIL_BLOCK) e[i] = TAB_EMPTY;
+    }
+    __device__ uint32_t find(int k, uint32_t s) const {
+        const uint32_t *e = tab[k & 1];
+        for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {
+            const uint32_t x = __hip_atomic_load(&e[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (x == s) return __hip_atomic_load(&e[H + h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (x == TAB_EMPTY) break;
+        }
+        return TAB_EMPTY;
+    }
+    __device__ void insert(int k, uint32_t s, uint32_t g) const {
+        uint32_t *e = tab[k & 1];
+        for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {  // H > distinct slots
+            uint32_t x = __hip_atomic_load(&e[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (x == TAB_EMPTY) x = atomicCAS(&e[h], TAB_EMPTY, s);
+            if (x == TAB_EMPTY || x == s) {
+                if (g < __hip_atomic_load(&e[H + h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                    atomicMin(&e[H + h], g);
+                return;
+            }
+        }
+    }
+    __device__ uint32_t prev_winner(const TailArgs &, int pt, uint32_t s) const { return find(pt, s); }
+    __device__ void offer(const TailArgs &, int pt, uint32_t s, unsigned long long key) const {
+        insert(pt ^ 1, s, (uint32_t)key);
+    }
+    __device__ uint32_t winner(const TailArgs &, int pt, uint32_t s) const { return find(pt ^ 1, s); }
+};
+
 // Vertices are global ids; the trim and update go over the live list of the
 // last election, the confirm over the list the update writes.
-struct GlobalSpace : GlobalSlots {
+template <class Slots>
+struct GlobalSpaceT : Slots {
     const TailArgs *t;
     const int32_t *lst;
     uint32_t lc;
@@ -102,7 +145,7 @@ struct GlobalSpace : GlobalSlots {
     }
 };
 
-constexpr uint32_t TAB_EMPTY = 0xFFFFFFFFu;  // no slot id (slot ids are below 2^24)
+using GlobalSpace = GlobalSpaceT<GlobalSlots>;
 
 // Vertices are local indexes 0..L-1 into the LDS copy; the pivot list is the
 // first two queues (a step's reach starts from them).  tail_arena holds, for L
@@ -112,11 +155,15 @@ constexpr uint32_t TAB_EMPTY = 0xFFFFFFFFu;  // no slot id (slot ids are below 2
 // (slot id, lowest global id), open addressing: the slot tables of the steps
 // in LDS, cleared every step instead of tagged.  Without the room, the global
 // slot tables serve.
-struct LocalSpace : GlobalSlots {
+template <class Slots>
+struct LocalSpaceT : Slots {
     uint32_t L, to, H, t0;
     __device__ uint32_t *tab(int k) const { return (uint32_t *)(tail_arena + t0) + (k & 1) * 2u * H; }
     __device__ void clear_cur(const TailArgs &t, int pt, uint32_t tid) const {
-        if (!H) return;
+        if (!H) {
+            Slots::clear_cur(t, pt, tid);
+            return;
+        }
         uint32_t *e = tab(pt ^ 1);
         for (uint32_t i = tid; i < 2u * H; i += TAIL_BLOCK) e[i] = TAB_EMPTY;
     }
@@ -141,16 +188,16 @@ struct LocalSpace : GlobalSlots {
         }
     }
     __device__ uint32_t prev_winner(const TailArgs &t, int pt, uint32_t s) const {
-        return H ? find(pt, s) : GlobalSlots::prev_winner(t, pt, s);
+        return H ? find(pt, s) : Slots::prev_winner(t, pt, s);
     }
     __device__ void offer(const TailArgs &t, int pt, uint32_t s, unsigned long long key) const {
         if (H)
             insert(pt ^ 1, s, (uint32_t)key);
         else
-            GlobalSlots::offer(t, pt, s, key);
+            Slots::offer(t, pt, s, key);
     }
     __device__ uint32_t winner(const TailArgs &t, int pt, uint32_t s) const {
-        return H ? find(pt ^ 1, s) : GlobalSlots::winner(t, pt, s);
+        return H ? find(pt ^ 1, s) : Slots::winner(t, pt, s);
     }
     __device__ uint32_t *w() const { return (uint32_t *)tail_arena; }
     __device__ int32_t *gidv() const { return (int32_t *)(tail_arena + 4u * L); }
@@ -184,13 +231,42 @@ struct TailRun {
     int lsel, pt;
     uint32_t epoch, npiv;
     unsigned long long depths, trims;
+    uint32_t lcnt[2];  // a batch graph's live-list lengths (the tail keeps them in ctl->lcnt)
+    uint32_t first;    // a batch graph's lowest pivot of its last election
+    int keep_colour;   // the step from no pivots: its election keeps the colours (the first
+                       // election, scc.cu:235-245, elects before any split)
+};
+
+// End-of-step bookkeeping.  The tail of a run publishes each step in the control
+// block (the host goes on from it); a graph of a batch keeps its list lengths
+// and pivot count in TailRun and adds its classes (trimmed vertices, pivots) to
+// the device's count, which other workgroups update too.
+struct TailBook {
+    __device__ void end(const TailArgs &t, TailShared &sh, TailRun &r, uint32_t ln) const {
+        if (threadIdx.x == 0) {
+            Ctl *ctl = t.ctl;
+            ctl->live -= (uint32_t)sh.red[0];
+            ctl->pcount += (uint32_t)sh.red[1] + r.npiv;
+            ctl->lcnt[r.lsel ^ 1] = ln;
+            ctl->npiv_prev = ctl->npiv;
+            ctl->first_piv_prev = ctl->first_piv;
+            ctl->npiv = r.npiv;
+            ctl->first_piv = sh.first;
+        }
+    }
+};
+struct BatchBook {
+    __device__ void end(const TailArgs &t, TailShared &sh, TailRun &r, uint32_t ln) const {
+        if (threadIdx.x == 0 && sh.red[1] + r.npiv) atomicAdd(&t.ctl->pcount, (uint32_t)sh.red[1] + r.npiv);
+        r.lcnt[r.lsel ^ 1] = ln;
+        r.first = sh.first;
+    }
 };
 
 // One FB step; false when a bound that every correct run stays within is passed.
-template <class Sp>
-__device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailRun &r) {
+template <class Sp, class Bk = TailBook>
+__device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailRun &r, const Bk &bk = Bk()) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
-    Ctl *ctl = t.ctl;
     // ---- reach: F over out-arcs, B over in-arcs, both from the pivot list
     if (tid == 0) {
         sh.cnt[0] = sh.cnt[1] = r.npiv;
@@ -286,7 +362,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
             atomicAdd(&sh.red[0], 1ull);
             continue;
         }
-        c = (3u * c + f) & COLOUR_MASK;
+        if (!r.keep_colour) c = (3u * c + f) & COLOUR_MASK;
         sp.st(v, make_word(c, 0));
         sp.keep(atomicAdd(&sh.lcnt, 1u), v);
         sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)g);
@@ -308,15 +384,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
     }
     tail_sync();
     r.npiv = sh.npiv;
-    if (tid == 0) {  // the device counts the host reads at the end
-        ctl->live -= (uint32_t)sh.red[0];
-        ctl->pcount += (uint32_t)sh.red[1] + r.npiv;
-        ctl->lcnt[r.lsel ^ 1] = ln;
-        ctl->npiv_prev = ctl->npiv;
-        ctl->first_piv_prev = ctl->first_piv;
-        ctl->npiv = r.npiv;
-        ctl->first_piv = sh.first;
-    }
+    bk.end(t, sh, r, ln);  // the counts the host reads at the end
     r.pt ^= 1;
     r.lsel ^= 1;
     tail_sync();
@@ -332,7 +400,8 @@ __device__ inline bool tail_arc(const TailArgs &t, int32_t v, int32_t u, uint32_
 // Copies the L live vertices of list lst, and the arcs between live vertices
 // of one colour, into LDS.  1: copied; 0: the arcs do not fit; -1: a live
 // vertex outside the list, which the elections never leave.
-__device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uint32_t L, const int32_t *lst,
+template <class LS>
+__device__ int tail_build(const TailArgs &t, TailShared &sh, LS &ls, uint32_t L, const int32_t *lst,
                           uint32_t npiv, int pt) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
     ls.L = L;
@@ -340,7 +409,9 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uin
     ls.H = 0;
     uint32_t *lw = ls.w(), *off0 = ls.off(0), *off1 = ls.off(1);
     int32_t *gidv = ls.gidv();
-    uint32_t *map = (uint32_t *)t.q[3];  // global id -> local index of the live vertices
+    // global id -> local index of the live vertices, indexed from t.vbase (the
+    // first vertex of a batch graph, whose queue slice this is)
+    uint32_t *map = (uint32_t *)t.q[3] - t.vbase;
     for (uint32_t i = tid; i < L; i += nt) {
         const int32_t v = lst[i];
         gidv[i] = v;
@@ -417,7 +488,7 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uin
         q0[k] = q1[k] = (uint16_t)j;
         if (j >= L || gidv[j] != t.piv[k]) sh.flag = 1;
     }
-    ls.clear_cur(t, pt ^ 1, tid);  // the table of the last election: its winners are the pivots
+    if (ls.H) ls.clear_cur(t, pt ^ 1, tid);  // the LDS table of the last election: its winners are the pivots
     tail_sync();
     if (sh.flag) return -1;
     if (ls.H)
@@ -430,6 +501,8 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LocalSpace &ls, uin
 // Words, the live list (lsel) and the pivot list back to global memory, and,
 // when the election tables were in LDS, the keys of the last election's
 // winners into its global table (the host's next election reads them there).
+using LocalSpace = LocalSpaceT<GlobalSlots>;
+
 __device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShared &sh, int lsel, uint32_t npiv, int pt,
                                uint32_t epoch) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
@@ -492,6 +565,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
             for (uint32_t i = threadIdx.x; i < (1u << t.slot_bits); i += TAIL_BLOCK) cur[i] = ~0ull;
             tail_sync();
         }
+        r.keep_colour = first ? 1 : 0;
         first = false;
         if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
             const uint32_t L = ctl->lcnt[r.lsel];
@@ -532,6 +606,127 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
 
 void launch_fb_tail(hipStream_t s, const TailArgs &t) {
     hipLaunchKernelGGL(k_fb_tail, dim3(1), dim3(TAIL_BLOCK), 0, s, t);
+}
+
+// ---------------------------------------------------------------------------
+// A batch of graphs in one persistent launch.  Each workgroup takes graphs from
+// a work counter, largest first, and solves each one alone with the steps of
+// the tail: set-up (colour g + 1, roots = self, the live list = the graph),
+// a first step from no pivots (trim-1 and the first election, lowest id),
+// then reach, trim, update and election until no pivot is left, in LDS once
+// the graph's live part fits there.  Every per-vertex buffer is used through
+// the graph's slice [o0, o1) (a graph's queues and lists never hold more than
+// its vertices), and its elections go through tables of its own, so graphs
+// never meet: each ends exactly as it would alone (same colours, states,
+// roots and steps).  Statistics: steps, levels and trim rounds are maxima over
+// the graphs, pivots a sum.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t batch_table_size(int32_t ng) {
+    uint32_t H = 64;
+    while (H < (uint32_t)ng + (uint32_t)ng / 2u + 1u) H <<= 1;
+    return H;
+}
+
+__global__ __launch_bounds__(TAIL_BLOCK) void k_fb_batch(TailArgs base, BatchArgs b) {
+    __shared__ TailShared sh;
+    __shared__ uint32_t s_next;
+    const uint32_t tid = threadIdx.x;
+    Ctl *ctl = base.ctl;
+    uint32_t max_steps = 0, max_depth = 0, max_trims = 0, status = 0;
+    unsigned long long pivots = 0;
+    for (;;) {
+        if (tid == 0) s_next = atomicAdd(&ctl->wnext, 1u);
+        tail_sync();
+        const uint32_t k = s_next;
+        tail_sync();  // s_next is written again for the next graph
+        if (k >= (uint32_t)b.G) break;  // block-uniform
+        const int32_t g = b.order[k];
+        const int32_t o0 = b.off[g], ng = b.off[g + 1] - o0;
+        if (ng <= 0) continue;
+        TailArgs t = base;
+        t.vbase = o0;
+        for (int i = 0; i < 4; i++) t.q[i] = base.q[i] + o0;
+        t.list[0] = base.list[0] + o0;
+        t.list[1] = base.list[1] + o0;
+        t.piv = base.piv + o0;
+        t.max_steps = (uint32_t)ng + 2u;
+        const uint32_t colour = (uint32_t)(g + 1) & COLOUR_MASK;
+        for (int32_t i = (int32_t)tid; i < ng; i += TAIL_BLOCK) {
+            const int32_t v = o0 + i;
+            t.word[v] = make_word(colour, 0);
+            t.root[v] = v;
+            t.list[0][i] = v;
+        }
+        GraphSlots gs;
+        gs.H = batch_table_size(ng);
+        gs.tab[0] = b.tab + b.tab_off[g];
+        gs.tab[1] = gs.tab[0] + 2u * gs.H;
+        LocalSpaceT<GraphSlots> ls{};
+        static_cast<GraphSlots &>(ls) = gs;
+        TailRun r{};
+        r.lcnt[0] = (uint32_t)ng;
+        bool local = false, first = true;
+        uint32_t steps = 0, try_below = 0xFFFFFFFFu;
+        tail_sync();
+        while (r.npiv > 0 || first) {  // block-uniform
+            if (steps >= t.max_steps) {
+                status = 2;
+                break;
+            }
+            if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
+                const uint32_t L = r.lcnt[r.lsel];
+                if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {
+                    const int bld = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv, r.pt);
+                    if (bld < 0) {
+                        status = 2;
+                        break;
+                    }
+                    local = bld > 0;
+                    try_below = L / 2;
+                }
+            }
+            r.keep_colour = first ? 1 : 0;
+            first = false;
+            steps++;
+            pivots += r.npiv;  // the pivots this step reaches from
+            bool ok;
+            if (local) {
+                ok = tail_step(t, ls, sh, r, BatchBook());
+            } else {
+                const GlobalSpaceT<GraphSlots> sp{gs, &t, t.list[r.lsel], r.lcnt[r.lsel], t.list[r.lsel ^ 1]};
+                ok = tail_step(t, sp, sh, r, BatchBook());
+            }
+            if (!ok) {
+                status = 2;
+                break;
+            }
+            if (steps == 1 && r.npiv && tid == 0) atomicMin(&ctl->first_piv, r.first);
+        }
+        if (local) {  // the words back to global memory
+            const uint32_t *lw = ls.w();
+            const int32_t *gidv = ls.gidv();
+            for (uint32_t i = tid; i < ls.L; i += TAIL_BLOCK) t.word[gidv[i]] = lw[i];
+        }
+        tail_sync();  // LDS is rebuilt for the next graph
+        // FB steps: the first step only trims and elects
+        max_steps = steps - 1 > max_steps ? steps - 1 : max_steps;
+        max_depth = (uint32_t)r.depths > max_depth ? (uint32_t)r.depths : max_depth;
+        max_trims = (uint32_t)r.trims > max_trims ? (uint32_t)r.trims : max_trims;
+        if (status) break;
+    }
+    if (tid == 0) {
+        atomicMax(&ctl->tail[0], max_steps);
+        if (status) atomicMax(&ctl->tail[1], status);
+        atomicMax(&ctl->tail[2], max_depth);
+        atomicMax(&ctl->tail[3], max_trims);
+        atomicAdd(&ctl->tail_pivots, pivots);
+    }
+}
+
+uint32_t batch_table_words(int32_t n_g) { return 4u * batch_table_size(n_g); }
+
+void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const BatchArgs &b) {
+    hipLaunchKernelGGL(k_fb_batch, dim3(grid), dim3(TAIL_BLOCK), 0, s, t, b);
 }
 
 }  // namespace gpuscc

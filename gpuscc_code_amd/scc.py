@@ -43,6 +43,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     reach_graphs: int = 0
     reach_block: int = 0
     tail_live: int = 0
+    batch_solver: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -58,6 +59,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.reach_graphs = self.reach_graphs
         o.reach_block = self.reach_block
         o.tail_live = self.tail_live
+        o.batch_solver = self.batch_solver
         return o
 
 

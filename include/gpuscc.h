@@ -90,6 +90,12 @@ typedef struct {
                                   (0 = 256) */
     int32_t tail_live;         /* once at most this many vertices are live, the remaining FB steps run in
                                   one workgroup and one launch (0 = default 65536, -1 = off) */
+    int32_t batch_solver;      /* batches (scc_set_batch), plain variant and pivot rule: 0 (default) all
+                                  graphs advance together in the level kernels, their elections sharing
+                                  one table; 1: every graph is solved on its own in one persistent launch,
+                                  a workgroup per graph at a time, largest first, each with its own
+                                  election tables (one host sync; slower than 0 once graphs exceed a
+                                  few thousand vertices) */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */

@@ -451,13 +451,74 @@ def test_huge_rows_split(sched):
                     assert st["steps"] == o["steps"]
 
 
+def batch_oracle(g, slot_bits):
+    """Each graph of a batch solved alone by the oracle's FB restatement (colour
+    g + 1 to start, the batch's election table size): the colours, states and
+    roots of all graphs, and the most FB steps any graph takes."""
+    off = g.graph_offsets
+    colour = np.empty(g.n, np.uint32)
+    state = np.empty(g.n, np.uint8)
+    root = np.empty(g.n, np.int32)
+    steps = 0
+    for k in range(len(off) - 1):
+        a, b = int(off[k]), int(off[k + 1])
+        if b == a:
+            continue
+        r0, r1, t0, t1 = int(g.row[a]), int(g.row[b]), int(g.rowT[a]), int(g.rowT[b])
+        o = oracle.fb(g.row[a:b + 1] - r0, g.col[r0:r1] - a, g.rowT[a:b + 1] - t0, g.colT[t0:t1] - a,
+                      variant=0, init_colour=np.full(b - a, k + 1, np.uint32), slot_bits=slot_bits, hash_kind=1)
+        colour[a:b], state[a:b], root[a:b] = o["colour"], o["state"], o["root"] + a
+        steps = max(steps, o["steps"])
+    return colour, state, root, steps
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+def test_batch_persistent_solver(ctx, sched):
+    """With batch_solver 1 the plain variant of a batch runs as one persistent launch, every graph
+    solved on its own by one workgroup (largest first): labels against Tarjan,
+    and the colour, E/T/P state and root of every vertex against the oracle run
+    on each graph alone; the step count is the largest of any graph.  Graphs of
+    a few vertices, empty graphs (no vertex) and graphs too big for the LDS copy
+    are in the batch."""
+    ctx.set_opts(scc.alg_opt(batch_solver=1))
+    for g in (scc.gen_batch(160, 64, 8192, 1.5, 6.0, 0.05, 0.5, seed=4),
+              scc.gen_batch(40, 2, 40000, 0.5, 4.0, 0.2, 0.6, seed=7)):
+        off = g.graph_offsets
+        ref, _ = oracle.tarjan(g.row, g.col)
+        ctx.upload(g)
+        ctx.set_batch(off)
+        lab, st = ctx.run(sched, 0)
+        assert np.array_equal(lab, ref)
+        assert st["num_scc"] == len(np.unique(ref)) and st["host_syncs"] == 1
+        colour, state, root, steps = batch_oracle(g, default_slot_bits(g.n))
+        words, roots = ctx.get_state()
+        c2, s2 = scc.split_word(words)
+        assert np.array_equal(c2, colour)
+        assert np.array_equal(s2 & 0x1C, state & 0x1C)
+        assert np.array_equal(roots, root)
+        assert st["steps"] == steps
+    # an empty graph inside a batch
+    g = scc.gen_batch(30, 50, 400, seed=11)
+    off = np.concatenate([g.graph_offsets[:5], g.graph_offsets[4:]])
+    ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.upload(g)
+    ctx.set_batch(off)
+    lab, _ = ctx.run(sched, 0)
+    assert np.array_equal(lab, ref)
+    ctx.set_batch(None)
+    ctx.set_opts(scc.alg_opt())
+
+
 @pytest.mark.parametrize("sched", SCHEDS)
 def test_batched_graphs(ctx, sched):
-    """Batch of uneven graphs: colour = graph id + 1; labels and final state vs the oracle."""
+    """Batch of uneven graphs advancing together in the level kernels
+    (the default): colour = graph id + 1, one election table for all;
+    labels and final state vs the oracle."""
     g = scc.gen_batch(160, 64, 8192, 1.5, 6.0, 0.05, 0.5, seed=4)
     off = g.graph_offsets
     gid = np.repeat(np.arange(len(off) - 1), np.diff(off)).astype(np.uint32)
     ref, _ = oracle.tarjan(g.row, g.col)
+    ctx.set_opts(scc.alg_opt())
     ctx.upload(g)
     ctx.set_batch(off)
     for variant in (0, 1, 2):
@@ -473,6 +534,7 @@ def test_batched_graphs(ctx, sched):
     # labels never leave their graph
     assert np.all(np.searchsorted(off, lab, side="right") == np.searchsorted(off, np.arange(g.n), side="right"))
     ctx.set_batch(None)
+    ctx.set_opts(scc.alg_opt())
 
 
 def test_batch_pull_bitmaps(ctx):

@@ -111,7 +111,10 @@ def test_mesh2048(ctx, mesh2048, variant):
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_batch4096(ctx, batch4096, variant):
+    """The graphs advancing together in the level kernels (the default),
+    full state against the oracle run on the whole batch."""
     g = batch4096.g
+    ctx.set_opts(scc.alg_opt())
     ctx.upload(g)
     ctx.set_batch(g.graph_offsets)
     try:
@@ -121,6 +124,29 @@ def test_batch4096(ctx, batch4096, variant):
                       np.searchsorted(off, np.arange(g.n), side="right"))
     finally:
         ctx.set_batch(None)
+        ctx.set_opts(scc.alg_opt())
+
+
+def test_batch4096_persistent(ctx, batch4096):
+    """batch_solver 1 for the plain variant: one persistent launch, every graph
+    on its own; full state against the oracle run on each graph alone."""
+    from test_gpu_parity import batch_oracle
+    g = batch4096.g
+    ctx.set_opts(scc.alg_opt(batch_solver=1))
+    ctx.upload(g)
+    ctx.set_batch(g.graph_offsets)
+    try:
+        labels, st = check_run(ctx, batch4096, "h", 0, full_state=False)
+        colour, state, root, steps = batch_oracle(g, default_slot_bits(g.n))
+        words, roots = ctx.get_state()
+        c2, s2 = scc.split_word(words)
+        assert np.array_equal(c2, colour)
+        assert np.array_equal(s2 & 0x1C, state & 0x1C)
+        assert np.array_equal(roots, root)
+        assert st["steps"] == steps and st["host_syncs"] == 1
+    finally:
+        ctx.set_batch(None)
+        ctx.set_opts(scc.alg_opt())
 
 
 def test_rmat24_plain_and_learned(ctx):
