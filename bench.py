@@ -164,6 +164,8 @@ def main():
                     help="hybrid: frontier arcs >= m / DIV also pull (0 = 14, -1 = off)")
     ap.add_argument("--pull-threshold", type=int, default=0,
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
+    ap.add_argument("--relabel", type=int, default=0, choices=[-1, 0, 1],
+                    help="renumbering at upload: 0 by degree skew (library default), 1 always, -1 never")
     ap.add_argument("--no-tuned", action="store_true",
                     help="library defaults instead of the autotuned options (gpuscc_code_amd/tuned.json)")
     ap.add_argument("--no-learned", action="store_true",
@@ -198,7 +200,7 @@ def main():
     ctx = scc.Context(0, scc.alg_opt(sequential_reach=1 if args.sequential else 0,
                                      hybrid_threshold=args.pull_threshold,
                                      hybrid_arc_div=args.pull_arc_div,
-                                     grid_blocks=args.grid_blocks))
+                                     grid_blocks=args.grid_blocks, relabel=args.relabel))
     t0 = time.time()
     ctx.upload(g)
     log(f"uploaded in {time.time() - t0:.1f}s")
@@ -360,7 +362,7 @@ def main():
                    "n": g.n, "m": g.m, "schedule": args.schedule,
                    "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
                    "reach": "sequential" if args.sequential else "fused",
-                   "tuned": tuned_key, "opts": opts},
+                   "tuned": tuned_key, "opts": opts, "renumbered": bool(last["renumbered"])},
         "bit_exact": bit_exact, "num_scc": None if st_check is None else st_check["num_scc"],
         "biggest_scc": None if tst is None else tst["biggest"],
         "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],

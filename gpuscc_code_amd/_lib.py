@@ -29,12 +29,13 @@ class SccOpts(ctypes.Structure):
                 ("huge_row", ctypes.c_int32), ("local_rounds", ctypes.c_int32),
                 ("hybrid_arc_div", ctypes.c_int32), ("reach_graphs", ctypes.c_int32),
                 ("reach_block", ctypes.c_int32),
-                ("tail_live", ctypes.c_int32), ("batch_solver", ctypes.c_int32)]
+                ("tail_live", ctypes.c_int32), ("batch_solver", ctypes.c_int32),
+                ("relabel", ctypes.c_int32)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",
               "wcc_step", "trim2_pairs", "trivial", "biggest", "host_syncs", "first_pivot",
-              "tail_steps", "tail_live"]
+              "tail_steps", "tail_live", "renumbered"]
 _STAT_DBLS = ["ms_total", "ms_fwd", "ms_bwd", "ms_trim", "ms_update"]
 
 

@@ -34,6 +34,9 @@ struct scc_ctx {
     uint32_t *d_word = nullptr;
     int32_t *d_root = nullptr, *d_label = nullptr, *d_tmp = nullptr;
     int32_t *d_mn = nullptr;  // learned runs: minimum id of each pivot's SCC (k_update)
+    // renumbering (opts.relabel): d_oid[v] = the caller's id of internal vertex v, d_nid the
+    // inverse; both null when the upload kept the caller's numbering
+    int32_t *d_oid = nullptr, *d_nid = nullptr;
     bool track_mn = false;    // this run's updates keep d_mn
     int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
     int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
@@ -309,6 +312,7 @@ struct TrimChain {
         a.n = c->g.n;
         a.ctl = c->d_ctl;
         a.root = c->d_root;
+        a.oid = c->d_oid;
         a.stamp = c->d_stamp;
         a.skip_fb = skip_fb ? 1 : 0;
         a.instr = c->instr;
@@ -503,7 +507,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         int32_t *lout = lists ? c->d_live[lout_sel] : nullptr;
         uint32_t *lout_cnt = lists ? &c->d_ctl->lcnt[lout_sel] : nullptr;
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), recolour, 0,
-                      nullptr, c->d_ctl, lin, lin_cnt, lout, lout_cnt, c->track_mn ? c->d_mn : nullptr);
+                      nullptr, c->d_ctl, lin, lin_cnt, lout, lout_cnt, c->track_mn ? c->d_mn : nullptr, c->d_oid);
         if (lists) {
             c->lsel_prev = lin_sel;
             c->lsel = lout_sel;
@@ -517,7 +521,7 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         }
         const float step_f = 0.25f * std::log1p((float)c->cur_step);
         launch_score(l, c->g, c->d_word, c->d_stat, c->d_cnt, one_set, c->slot_bits, step_f, c->cur_step == 0 ? 1 : 0,
-                     lout, lout_cnt, c->d_wb, nullptr, nullptr, cur, elect_tag(c->epoch), c->d_ctl);
+                     lout, lout_cnt, c->d_wb, nullptr, nullptr, cur, elect_tag(c->epoch), c->d_ctl, c->d_oid);
     } else if (lists) {
         // the update reads the list of the election before (all n the first time)
         // and lists the vertices still live; the confirm pass reads that list
@@ -525,21 +529,21 @@ scc_status do_elect(scc_ctx *c, int recolour, int64_t *npiv, bool learned = fals
         const uint32_t *lin_cnt = lin_sel >= 0 ? &c->d_ctl->lcnt[lin_sel] : nullptr;
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       recolour, 1, nullptr, c->d_ctl, lin, lin_cnt, c->d_live[lout_sel],
-                      &c->d_ctl->lcnt[lout_sel], c->track_mn ? c->d_mn : nullptr);
+                      &c->d_ctl->lcnt[lout_sel], c->track_mn ? c->d_mn : nullptr, c->d_oid);
         c->lsel_prev = lin_sel;
         c->lsel = lout_sel;
     } else {
         launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch),
                       recolour, 1, nullptr, c->d_ctl, nullptr, nullptr, nullptr, nullptr,
-                      c->track_mn ? c->d_mn : nullptr);
+                      c->track_mn ? c->d_mn : nullptr, c->d_oid);
     }
     uint32_t *vis = c->vis_mark ? c->d_vis : nullptr;
     if (lists)
         launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl, c->d_live[lout_sel],
-                       &c->d_ctl->lcnt[lout_sel], vis, c->vis_words);
+                       &c->d_ctl->lcnt[lout_sel], vis, c->vis_words, c->d_oid);
     else
         launch_confirm(l, c->d_word, c->g.n, cur, c->slot_bits, c->d_piv, c->d_ctl, nullptr, nullptr, vis,
-                       c->vis_words);
+                       c->vis_words, c->d_oid);
     c->vis_mark = false;
     ctl_queue(c, 64);  // the pivot list seeds both directions
     c->prev_tab ^= 1;
@@ -560,7 +564,7 @@ scc_status do_count(scc_ctx *c, int what, int64_t *out) {
 
 scc_status do_trim2(scc_ctx *c, int64_t *pairs) {
     ctl_op(c, 8);
-    launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
+    launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl, c->d_oid);
     CHK(read_ctl(c));
     *pairs = (int64_t)c->h_ctl->count;
     return SCC_OK;
@@ -574,9 +578,9 @@ scc_status do_wcc(scc_ctx *c, int64_t *iters, int64_t *npiv, bool learned = fals
     // the live vertices: the list of the last election when the run keeps one
     const int32_t *lin = c->use_lists && c->lsel >= 0 ? c->d_live[c->lsel] : nullptr;
     const uint32_t *lin_cnt = lin ? &c->d_ctl->lcnt[c->lsel] : nullptr;
-    launch_wcc_init(l, c->d_tmp, c->g.n, lin, lin_cnt);
+    launch_wcc_init(l, c->d_tmp, c->g.n, lin, lin_cnt, c->d_oid);
     CHK(run_chain(c, [&](int it, hipEvent_t, hipEvent_t) {
-            launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it, lin, lin_cnt);
+            launch_wcc_iter(l, c->g, c->d_word, c->d_tmp, c->d_ctl, it, lin, lin_cnt, c->d_nid);
         }, c->guess_wcc, 0, iters));
     launch_wcc_recolour(l, c->d_word, c->d_tmp, c->g.n, c->d_ctl, lin, lin_cnt);
     return do_elect(c, 0, npiv, learned, sync);
@@ -657,10 +661,118 @@ void free_graph(scc_ctx *c) {
     c->d_feat = nullptr;
     if (c->d_goff) (void)hipFree(c->d_goff);
     c->d_goff = nullptr;
+    for (int32_t **p : {&c->d_oid, &c->d_nid}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
     free_batch(c);
     c->num_graphs = 0;
     c->uploaded = false;
     c->labels_valid = false;
+}
+
+// ---- renumbering (opts.relabel) ----------------------------------------------
+// The caller's ids in the new internal order: by graph (offsets off[0..G], may
+// be null), then by decreasing total degree, then by id -- a stable LSD radix
+// sort on 16-bit digits of the key (graph, ~degree), the ids entering in order.
+std::vector<int32_t> degree_order(int32_t n, const std::vector<int32_t> &deg, const int32_t *off, int32_t G) {
+    std::vector<uint64_t> key((size_t)n);
+    int32_t g = 0;
+    for (int32_t v = 0; v < n; v++) {
+        while (off && g < G - 1 && off[g + 1] <= v) g++;
+        key[v] = ((uint64_t)(uint32_t)g << 32) | (uint32_t) ~(uint32_t)deg[v];
+    }
+    int passes = 2;
+    for (uint32_t x = off && G > 1 ? (uint32_t)(G - 1) : 0u; x; x >>= 16) passes++;
+    std::vector<int32_t> a((size_t)n), b((size_t)n);
+    for (int32_t v = 0; v < n; v++) a[v] = v;
+    std::vector<int64_t> cnt(65537);
+    for (int p = 0; p < passes; p++) {
+        const int sh = 16 * p;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int32_t i = 0; i < n; i++) cnt[((key[a[i]] >> sh) & 0xFFFFu) + 1]++;
+        for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+        for (int32_t i = 0; i < n; i++) b[cnt[(key[a[i]] >> sh) & 0xFFFFu]++] = a[i];
+        a.swap(b);
+    }
+    return a;
+}
+
+// The row indexes derived from the device CSR as it stands: degree classes,
+// heads, ELL entries, head-live bits and the scorer's static features.
+void row_indexes(scc_ctx *c) {
+    const int32_t n = c->g.n;
+    Launch l = launch_for(c);
+    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head, c->d_ell);
+    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n, c->d_ell + n);
+    launch_head_live(l, c->d_dcls, c->d_head, n, c->d_hlive);
+    // the scorer's static features (degrees, 2-hop degree sums): a property of the graph
+    launch_static_features(l, c->g, c->d_stat);
+    c->stat_valid = true;
+}
+
+// Renumbers the device CSR so that internal vertex i is the caller's vertex
+// order[i].  row, rowT: the current internal row offsets (host); cur_oid: the
+// caller's id of each current internal vertex (empty: the identity).  A row
+// keeps the order of its entries, so a hub-first row (decreasing degree, then
+// id) comes out ascending in a numbering that follows the same order.
+scc_status permute_graph(scc_ctx *c, const int32_t *row, const int32_t *rowT, const std::vector<int32_t> &order,
+                         const std::vector<int32_t> &cur_oid) {
+    const int32_t n = c->g.n;
+    const int64_t m = c->g.m;
+    std::vector<int32_t> nid((size_t)n), map((size_t)n), cur_of((size_t)n);
+    for (int32_t i = 0; i < n; i++) nid[order[i]] = i;
+    for (int32_t v = 0; v < n; v++) {
+        const int32_t o = cur_oid.empty() ? v : cur_oid[v];
+        map[v] = nid[o];
+        cur_of[o] = v;
+    }
+    std::vector<int32_t> nrow((size_t)n + 1), nrowT((size_t)n + 1);
+    nrow[0] = nrowT[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t u = cur_of[order[i]];
+        nrow[i + 1] = nrow[i] + (row[u + 1] - row[u]);
+        nrowT[i + 1] = nrowT[i] + (rowT[u + 1] - rowT[u]);
+    }
+    const size_t nb = sizeof(int32_t) * ((size_t)n + 1), mb = sizeof(int32_t) * std::max<size_t>((size_t)m, 1);
+    const size_t vb = sizeof(int32_t) * std::max<size_t>((size_t)n, 1);
+    int32_t *nr = nullptr, *nc = nullptr, *nrT = nullptr, *ncT = nullptr;
+    auto drop = [&]() {
+        for (int32_t *p : {nr, nc, nrT, ncT})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t e = dev_alloc(c, &nr, nb);
+    if (e == hipSuccess) e = dev_alloc(c, &nrT, nb);
+    if (e == hipSuccess) e = dev_alloc(c, &nc, mb);
+    if (e == hipSuccess) e = dev_alloc(c, &ncT, mb);
+    if (e == hipSuccess && !c->d_oid) e = dev_alloc(c, &c->d_oid, vb);
+    if (e == hipSuccess && !c->d_nid) e = dev_alloc(c, &c->d_nid, vb);
+    const size_t ib = sizeof(int32_t) * (size_t)n;  // n > 0 (the caller's check)
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_tmp, map.data(), ib, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(nr, nrow.data(), nb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(nrT, nrowT.data(), nb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_oid, order.data(), ib, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_nid, nid.data(), ib, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const Launch l = launch_for(c);
+        launch_permute_rows(l, c->d_row, c->d_col, c->d_tmp, nr, nc, n);
+        launch_permute_rows(l, c->d_rowT, c->d_colT, c->d_tmp, nrT, ncT, n);
+        e = hipGetLastError();
+    }
+    const hipError_t es = hipStreamSynchronize(c->stream);  // the host vectors go out of scope
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) {
+        drop();
+        return hip_fail(c, e, "renumbering");
+    }
+    for (int32_t *p : {c->d_row, c->d_col, c->d_rowT, c->d_colT}) (void)hipFree(p);
+    c->d_row = nr;
+    c->d_col = nc;
+    c->d_rowT = nrT;
+    c->d_colT = ncT;
+    c->g = Graph{n, m, c->d_row, c->d_col, c->d_rowT, c->d_colT};
+    clear_graphs(c);  // captured launches point at the freed rows
+    return SCC_OK;
 }
 
 }  // namespace
@@ -739,7 +851,7 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
     if ((c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
                                      (c->opts.block_size & 63) != 0)) ||
         (c->opts.reach_block != 0 && c->opts.reach_block != 256 && c->opts.reach_block != 512) ||
-        c->opts.tail_live < -1) {
+        c->opts.tail_live < -1 || c->opts.relabel < -1 || c->opts.relabel > 1) {
         delete c;
         return SCC_E_INPUT;
     }
@@ -781,6 +893,7 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
         return fail(c, SCC_E_INPUT, "reach_block must be 0, 256 or 512");
     if (opts->tail_live < -1) return fail(c, SCC_E_INPUT, "tail_live must be -1 (off), 0 (default) or a count");
     if (opts->grid_blocks < 0) return fail(c, SCC_E_INPUT, "grid_blocks < 0");
+    if (opts->relabel < -1 || opts->relabel > 1) return fail(c, SCC_E_INPUT, "relabel must be -1, 0 or 1");
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
     c->opts = *opts;
@@ -906,13 +1019,17 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
     Launch l = launch_for(c);
     launch_fill_u64(l, c->d_slots[0], (int64_t)1 << sb, ~0ull);
     launch_fill_u64(l, c->d_slots[1], (int64_t)1 << sb, ~0ull);
-    launch_degree_class(l, c->d_row, c->d_col, n, c->d_dcls, c->d_head, c->d_ell);
-    launch_degree_class(l, c->d_rowT, c->d_colT, n, c->d_dcls + n, c->d_head + n, c->d_ell + n);
-    launch_head_live(l, c->d_dcls, c->d_head, n, c->d_hlive);
-    // the scorer's static features (degrees, 2-hop degree sums): a property of the graph,
-    // built with the other row indexes
-    launch_static_features(l, c->g, c->d_stat);
-    c->stat_valid = true;
+    // renumbering by decreasing degree: always, never, or when the degrees are skewed
+    std::vector<int32_t> deg((size_t)n);
+    int64_t dmax = 0;
+    for (int32_t v = 0; v < n; v++) {
+        deg[v] = (row[v + 1] - row[v]) + (rowT[v + 1] - rowT[v]);
+        dmax = std::max<int64_t>(dmax, deg[v]);
+    }
+    const bool skewed = n >= 65536 && dmax * (int64_t)n > 64 * 2 * m;  // max above 64 x the mean (2m / n)
+    if (n > 0 && (c->opts.relabel > 0 || (c->opts.relabel == 0 && skewed)))
+        CHK(permute_graph(c, row, rowT, degree_order(n, deg, nullptr, 0), {}));
+    row_indexes(c);
     c->epoch = 0;
     c->prev_tab = 0;
     HIPCHK(c, hipGetLastError());
@@ -938,6 +1055,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     scc_stats st;
     std::memset(&st, 0, sizeof(st));
     st.first_pivot = -1;
+    st.renumbered = c->d_oid ? 1 : 0;
     const int32_t n = c->g.n;
     c->syncs = 0;
     c->cur_step = 0;
@@ -962,6 +1080,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         ta.colT = c->g.colT;
         ta.word = c->d_word;
         ta.root = c->d_root;
+        ta.oid = c->d_oid;
         ta.n = n;
         ta.slot_bits = c->slot_bits;
         ta.piv = c->d_piv;
@@ -1000,7 +1119,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // minimum in d_mn (k_init sets mn[v] = v), and the labels are mn[root]
         c->track_mn = learned;
         launch_init(l, c->d_word, c->d_root, n, c->num_graphs ? c->d_goff : nullptr, c->num_graphs, c->d_dcls,
-                    c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words, learned ? c->d_mn : nullptr);
+                    c->d_hlive, c->d_q[1], c->d_ctl, c->d_vis, c->vis_words, learned ? c->d_mn : nullptr,
+                    c->d_oid);
         if (n == 0) return SCC_OK;
         int64_t it = 0, npiv = 0, ne = 0;
         bool wcc_done = false;
@@ -1032,7 +1152,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             launch_unset_pivots(launch_for(c), c->d_word, n, c->d_ctl, c->d_piv);  // the pivot list
             if (variant == SCC_VARIANT_2W) {
                 ctl_op(c, 8);
-                launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl);
+                launch_trim2(launch_for(c), c->g, c->d_word, c->d_root, c->d_ctl, c->d_oid);
             }
             int64_t wi = 0;
             CHK(do_wcc(c, &wi, &npiv, learned, sync));
@@ -1058,6 +1178,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             ta.colT = c->g.colT;
             ta.word = c->d_word;
             ta.root = c->d_root;
+            ta.oid = c->d_oid;
             ta.n = n;
             ta.slots[0] = c->d_slots[0];
             ta.slots[1] = c->d_slots[1];
@@ -1338,6 +1459,10 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return SCC_OK;
     };
     s = body();
+    if (s == SCC_OK && c->d_oid && n > 0) {  // renumbered: the labels into the caller's order
+        launch_gather_i32(launch_for(c), c->d_root, c->d_nid, c->d_label, n);
+        c->label_in_root = false;
+    }
     c->track_mn = false;
     c->one_colour = false;
     c->best_step = false;
@@ -1404,6 +1529,17 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
     if (off[0] != 0 || off[G] != c->g.n) return fail(c, SCC_E_INPUT, "offsets must run from 0 to n");
     for (int32_t i = 0; i < G; i++)
         if (off[i + 1] < off[i]) return fail(c, SCC_E_INPUT, "offsets decrease");
+    if (c->d_oid && c->g.n > 0) {  // renumbered: again, by degree within each graph, so graphs keep their ranges
+        const int32_t n = c->g.n;
+        std::vector<int32_t> row((size_t)n + 1), rowT((size_t)n + 1), oid((size_t)n), deg((size_t)n);
+        HIPCHK(c, hipMemcpy(row.data(), c->d_row, sizeof(int32_t) * row.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(rowT.data(), c->d_rowT, sizeof(int32_t) * rowT.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(oid.data(), c->d_oid, sizeof(int32_t) * oid.size(), hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < n; i++) deg[oid[i]] = (row[i + 1] - row[i]) + (rowT[i + 1] - rowT[i]);
+        CHK(permute_graph(c, row.data(), rowT.data(), degree_order(n, deg, off, G), oid));
+        row_indexes(c);
+        c->labels_valid = false;
+    }
     HIPCHK(c, hipMalloc(&c->d_goff, sizeof(int32_t) * ((size_t)G + 1)));
     HIPCHK(c, hipMemcpy(c->d_goff, off, sizeof(int32_t) * ((size_t)G + 1), hipMemcpyHostToDevice));
     if (c->g.n > 0) {
@@ -1555,7 +1691,7 @@ scc_status scc_phase_recolour(scc_ctx *c) {
     Launch l = launch_for(c);
     unsigned long long *prev = c->d_slots[c->prev_tab], *cur = c->d_slots[c->prev_tab ^ 1];
     launch_update(l, c->d_word, c->d_root, c->g.n, prev, cur, c->slot_bits, elect_tag(c->epoch), 1, 0,
-                  nullptr, c->d_ctl);
+                  nullptr, c->d_ctl, nullptr, nullptr, nullptr, nullptr, nullptr, c->d_oid);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SCC_OK;
@@ -1574,6 +1710,10 @@ scc_status scc_phase_features(scc_ctx *c, int32_t step, const int32_t *idx, int6
     HIPCHK(c, hipMalloc(&di.p, sizeof(int32_t) * (size_t)count));
     HIPCHK(c, hipMalloc(&dx.p, sizeof(float) * MLP_IN * (size_t)count));
     HIPCHK(c, hipMemcpyAsync(di.p, idx, sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice, c->stream));
+    if (c->d_oid) {  // renumbered: the caller's ids to internal ones, in place through dx's first count words
+        launch_gather_i32(launch_for(c), c->d_nid, (const int32_t *)di.p, (int32_t *)dx.p, count);
+        HIPCHK(c, hipMemcpyAsync(di.p, dx.p, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToDevice, c->stream));
+    }
     launch_gather_rows(mlp_launch(c, count * MLP_IN), c->d_feat, (const int32_t *)di.p, count, (float *)dx.p);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(X_out, dx.p, sizeof(float) * MLP_IN * (size_t)count, hipMemcpyDeviceToHost,
@@ -1612,14 +1752,21 @@ scc_status scc_state_set(scc_ctx *c, const uint32_t *words, const int32_t *roots
     ctl_op(c, 512);  // phase-level use starts from a clean control block
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (vb == 0) return SCC_OK;
-    HIPCHK(c, hipMemcpy(c->d_word, words, vb, hipMemcpyHostToDevice));
-    if (roots) {
-        HIPCHK(c, hipMemcpy(c->d_root, roots, vb, hipMemcpyHostToDevice));
-    } else {
-        std::vector<int32_t> r((size_t)c->g.n);
-        for (int32_t v = 0; v < c->g.n; v++) r[v] = v;
-        HIPCHK(c, hipMemcpy(c->d_root, r.data(), vb, hipMemcpyHostToDevice));
+    const int32_t n = c->g.n;
+    std::vector<int32_t> oid;  // renumbered: internal vertex i is the caller's oid[i]
+    if (c->d_oid) {
+        oid.resize((size_t)n);
+        HIPCHK(c, hipMemcpy(oid.data(), c->d_oid, vb, hipMemcpyDeviceToHost));
     }
+    auto at = [&](int32_t i) { return oid.empty() ? i : oid[i]; };
+    std::vector<uint32_t> w((size_t)n);
+    std::vector<int32_t> r((size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        w[i] = words[at(i)];
+        r[i] = roots ? roots[at(i)] : at(i);
+    }
+    HIPCHK(c, hipMemcpy(c->d_word, w.data(), vb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_root, r.data(), vb, hipMemcpyHostToDevice));
     return SCC_OK;
 }
 
@@ -1631,6 +1778,19 @@ scc_status scc_state_get(scc_ctx *c, uint32_t *words, int32_t *roots) {
     if (vb == 0) return SCC_OK;
     if (words) HIPCHK(c, hipMemcpy(words, c->d_word, vb, hipMemcpyDeviceToHost));
     if (roots) HIPCHK(c, hipMemcpy(roots, c->d_root, vb, hipMemcpyDeviceToHost));
+    if (c->d_oid) {  // renumbered: into the caller's order
+        const int32_t n = c->g.n;
+        std::vector<int32_t> oid((size_t)n);
+        HIPCHK(c, hipMemcpy(oid.data(), c->d_oid, vb, hipMemcpyDeviceToHost));
+        if (words) {
+            std::vector<uint32_t> t(words, words + n);
+            for (int32_t i = 0; i < n; i++) words[oid[i]] = t[i];
+        }
+        if (roots) {
+            std::vector<int32_t> t(roots, roots + n);
+            for (int32_t i = 0; i < n; i++) roots[oid[i]] = t[i];
+        }
+    }
     return SCC_OK;
 }
 

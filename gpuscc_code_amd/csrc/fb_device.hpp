@@ -33,6 +33,14 @@ __host__ __device__ inline uint64_t elect_tag(uint32_t epoch) {
     return (uint64_t)(0xFFFFu - (epoch & 0xFFFFu)) << 48;
 }
 
+// Vertex ids.  An upload may renumber the vertices by decreasing degree (hubs
+// first, so the words, visited bits and rows the scans touch most share cache
+// lines); oid[v] is then the caller's id of internal vertex v, and nid the
+// inverse.  Everything that orders vertices by id -- election keys, roots,
+// trim-2's lower id, WCC labels -- uses the caller's ids, so every result is
+// the one of the graph as given.  Null: no renumbering.
+__device__ inline int32_t orig_id(const int32_t *oid, int32_t v) { return oid ? oid[v] : v; }
+
 // Device control block: level-chain flags, queue sizes and reduction scratch.
 struct Ctl {
     uint32_t changed[4];   // chain ring: kernel i sets [i&3] on progress, clears [(i+1)&3]

@@ -75,6 +75,7 @@ struct ReachArgs {
     int init_list;      // the first level of trim1: check only the vertices k_init listed in inq[0]
     const int32_t *live;        // trim sweeps: only the vertices live[0 .. *live_cnt) (null: all n)
     const uint32_t *live_cnt;
+    const int32_t *oid;         // the caller's ids (roots of trimmed vertices), null: the same
 };
 
 // Local rounds start at this level in the first FB step: its first levels are
@@ -98,9 +99,11 @@ struct Graph {  // device CSR + CSR^T
 // stay live are appended to unk, their count added to ctl->qsize[0][0] (zeroed before):
 // the first level of trim1 checks only them.
 // vis (may be null): the two visited bitmaps (vis_words words each) are cleared.
+// oid (may be null): the caller's id of each vertex (see orig_id); roots are set to it.
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl,
-                 uint32_t *vis = nullptr, int64_t vis_words = 0, int32_t *mn = nullptr);
+                 uint32_t *vis = nullptr, int64_t vis_words = 0, int32_t *mn = nullptr,
+                 const int32_t *oid = nullptr);
 // ctl->count += arcs that cross graphs of the batch
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl);
 void launch_fill_u64(const Launch &l, unsigned long long *p, int64_t count, unsigned long long v);
@@ -126,27 +129,36 @@ void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score,
                    Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr,
-                   int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr, int32_t *mn = nullptr);
+                   int32_t *lout = nullptr, uint32_t *lout_cnt = nullptr, int32_t *mn = nullptr,
+                   const int32_t *oid = nullptr);
 // vis (may be null): the winners are marked in both visited bitmaps (the first election
 // of a run: the seeds of the first reach)
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin = nullptr,
-                    const uint32_t *lin_cnt = nullptr, uint32_t *vis = nullptr, int64_t vis_words = 0);
+                    const uint32_t *lin_cnt = nullptr, uint32_t *vis = nullptr, int64_t vis_words = 0,
+                    const int32_t *oid = nullptr);
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl);
 void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ctl);
 // piv (may be null): visit only the pivot list of the last election (ctl->npiv entries)
 void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl, const int32_t *piv = nullptr);
-void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl);
+void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl,
+                  const int32_t *oid = nullptr);
 // the WCC pass over the live list lin[0 .. *lin_cnt) (lin null: all n)
+// (labels in the caller's ids: oid and its inverse nid, may be null)
 void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n, const int32_t *lin = nullptr,
-                     const uint32_t *lin_cnt = nullptr);
+                     const uint32_t *lin_cnt = nullptr, const int32_t *oid = nullptr);
 void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,
-                     int iter, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr);
+                     int iter, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr,
+                     const int32_t *nid = nullptr);
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
                          const Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr);
-void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n);
 // root[v] = mn[root[v]] in place (mn kept by the run's updates, see k_update)
 void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n);
+// out[i] = in[idx[i]], i < count
+void launch_gather_i32(const Launch &l, const int32_t *in, const int32_t *idx, int32_t *out, int64_t count);
+// renumbered CSR: row v becomes row map[v] at nrow[map[v]] in ncol, entries renamed by map
+void launch_permute_rows(const Launch &l, const int32_t *row, const int32_t *col, const int32_t *map,
+                         const int32_t *nrow, int32_t *ncol, int32_t n);
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl);
 
@@ -176,6 +188,7 @@ struct TailArgs {
     int first_step;      // no pivots yet: the first step is trim-1 and the first election (lowest live id)
     int self_refill;     // before an election whose 16-bit tag would wrap, clear its slot table here
                          // (else hand back to the host, status 1)
+    const int32_t *oid;  // the caller's ids (keys, roots), null: the same
 };
 void launch_fb_tail(hipStream_t s, const TailArgs &t);
 // a batch in one persistent launch (k_fb_batch): graph g owns vertices [off[g], off[g + 1]);
@@ -203,7 +216,7 @@ void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint3
 void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
                   const uint32_t *one_set, int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
                   const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
-                  unsigned long long tag, const Ctl *ctl);
+                  unsigned long long tag, const Ctl *ctl, const int32_t *oid = nullptr);
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit);
 void launch_mlp_fwdbwd(const Launch &l, const float *X, const float *y, const int32_t *idx, int64_t N,
                        const uint16_t *params, MlpWork &w);

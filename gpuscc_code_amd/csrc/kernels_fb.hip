@@ -51,7 +51,7 @@ constexpr int INIT_V = 8;
 template <int VEC>
 __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *off, int32_t G,
                        const uint8_t *cls_out, const uint8_t *cls_in, const uint8_t *hlive, int32_t *unk,
-                       Ctl *ctl, uint32_t *vis, int64_t vis_words, int32_t *mn) {
+                       Ctl *ctl, uint32_t *vis, int64_t vis_words, int32_t *mn, const int32_t *oid) {
     constexpr int R = INIT_V / VEC;  // groups of VEC vertices per work-item and round
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t shw[16];
@@ -108,11 +108,12 @@ __global__ void k_init(uint32_t *word, int32_t *root, int32_t n, const int32_t *
             if (VEC == 4) {
                 *reinterpret_cast<uint4 *>(word + v0) = make_uint4(w[0], w[1 % VEC], w[2 % VEC], w[3 % VEC]);
                 const int4 self = make_int4((int32_t)v0, (int32_t)v0 + 1, (int32_t)v0 + 2, (int32_t)v0 + 3);
-                *reinterpret_cast<int4 *>(root + v0) = self;
+                // roots in the caller's ids; mn is indexed by them, its identity is the same either way
+                *reinterpret_cast<int4 *>(root + v0) = oid ? *reinterpret_cast<const int4 *>(oid + v0) : self;
                 if (mn) *reinterpret_cast<int4 *>(mn + v0) = self;
             } else {
                 word[v0] = w[0];
-                root[v0] = (int32_t)v0;
+                root[v0] = orig_id(oid, (int32_t)v0);
                 if (mn) mn[v0] = (int32_t)v0;
             }
         }
@@ -331,7 +332,8 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                          const unsigned long long *__restrict__ prev, unsigned long long *cur,
                          int slot_bits, unsigned long long tag, int recolour, int elect,
                          const uint16_t *__restrict__ score, Ctl *ctl, const int32_t *lin,
-                         const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn) {
+                         const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn,
+                         const int32_t *oid) {
     __shared__ unsigned long long sh[16];
     __shared__ uint32_t wslot[16];
     __shared__ unsigned long long wmin[16];
@@ -363,14 +365,15 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
             key[k] = ~0ull;
             if (w[k] & ST_E) continue;
             int32_t v = vid[k];
+            const int32_t ov = orig_id(oid, v);
             uint32_t c = colour_of(w[k]);
             if (recolour) {
                 uint32_t f = ((w[k] & ST_F) ? 1u : 0u) + ((w[k] & ST_B) ? 2u : 0u);
                 if (f == 3) {
                     word[v] = w[k] | ST_E;
                     uint32_t pv = (uint32_t)prev[slot_of(c, slot_bits)];
-                    root[v] = pv < (uint32_t)n ? (int32_t)pv : v;
-                    if (mn && pv < (uint32_t)n && v < mn[pv]) atomicMin(&mn[pv], v);
+                    root[v] = pv < (uint32_t)n ? (int32_t)pv : ov;
+                    if (mn && pv < (uint32_t)n && ov < mn[pv]) atomicMin(&mn[pv], ov);
                     gone++;
                     continue;
                 }
@@ -382,7 +385,7 @@ __global__ void k_update(uint32_t *word, int32_t *root, int32_t n,
                 cmask |= 1u << k;
                 sl[k] = slot_of(c, slot_bits);
                 uint32_t sc = score ? (uint32_t)(0xFFFFu - score[v]) : 0u;
-                key[k] = tag | ((unsigned long long)sc << 32) | (uint32_t)v;
+                key[k] = tag | ((unsigned long long)sc << 32) | (uint32_t)ov;
             }
         }
         if (lout) list_append(keep, vid, lout, lout_cnt, shw, sbase);
@@ -459,7 +462,7 @@ constexpr int CONF_V = 16;  // vertices per work-item per round
 
 __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *__restrict__ cur,
                           int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt,
-                          uint32_t *vis, int64_t vis_words) {
+                          uint32_t *vis, int64_t vis_words, const int32_t *oid) {
     __shared__ uint32_t shw[16];
     __shared__ uint32_t sbase;
     if (__atomic_load_n(&ctl->abort, __ATOMIC_RELAXED)) return;
@@ -481,10 +484,11 @@ __global__ void k_confirm(uint32_t *word, int32_t n, const unsigned long long *_
             if (w[k] & ST_E) continue;
             uint32_t v = (uint32_t)vid[k];
             uint32_t c = colour_of(w[k]);
-            if ((uint32_t)cur[slot_of(c, slot_bits)] == v) {
+            const uint32_t ov = (uint32_t)orig_id(oid, (int32_t)v);
+            if ((uint32_t)cur[slot_of(c, slot_bits)] == ov) {
                 word[v] = make_word(c, 19);
                 won |= 1u << k;
-                atomicMin(&ctl->first_piv, v);
+                atomicMin(&ctl->first_piv, ov);
                 if (vis) {  // a pivot is reached both ways
                     atomicOr(&vis[v >> 5], 1u << (v & 31));
                     atomicOr(&vis[vis_words + (v >> 5)], 1u << (v & 31));
@@ -585,7 +589,7 @@ __device__ inline int32_t only_nb(const int32_t *__restrict__ row, const int32_t
 
 __global__ void k_trim2_mark(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                              const int32_t *__restrict__ rowT, const int32_t *__restrict__ colT,
-                             uint32_t *word, int32_t *root, int32_t n) {
+                             uint32_t *word, int32_t *root, int32_t n, const int32_t *oid) {
     int64_t beg, end;
     my_range(n, beg, end);
     for (int64_t vv = beg + threadIdx.x; vv < end; vv += blockDim.x) {
@@ -601,9 +605,9 @@ __global__ void k_trim2_mark(const int32_t *__restrict__ row, const int32_t *__r
             if (a >= 0 && only_nb(rowT, colT, word, a, c) == v) u = a;
         }
         if (u < 0) continue;
-        int32_t lo = v < u ? v : u;
-        word[v] = make_word(c, ST_PAIR | (v == lo ? ST_P : 0u));
-        root[v] = lo;
+        const int32_t ov = orig_id(oid, v), ou = orig_id(oid, u);  // the pair's lower id by the caller's ids
+        word[v] = make_word(c, ST_PAIR | (ov < ou ? ST_P : 0u));
+        root[v] = ov < ou ? ov : ou;
     }
 }
 
@@ -634,18 +638,22 @@ __global__ void k_trim2_commit(uint32_t *word, int32_t n, Ctl *ctl) {
 // (lin, *lin_cnt; null: all n) only the listed vertices are visited: labels are
 // ids of live vertices, so no other entry of lab is ever read.
 // ---------------------------------------------------------------------------
-__global__ void k_wcc_init(int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt) {
+// Labels are the caller's ids (oid, nid: the renumbering, see orig_id), so the
+// minima and the colours they give are those of the graph as given; a pointer
+// jump reads the label of the vertex a label names through nid.
+__global__ void k_wcc_init(int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt,
+                           const int32_t *oid) {
     int64_t beg, end;
     my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
     for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
         const int32_t v = lin ? lin[i] : (int32_t)i;
-        lab[v] = v;
+        lab[v] = orig_id(oid, v);
     }
 }
 
 __global__ void k_wcc_iter(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                            const uint32_t *__restrict__ word, int32_t *lab, int32_t n, Ctl *ctl,
-                           int iter, const int32_t *lin, const uint32_t *lin_cnt) {
+                           int iter, const int32_t *lin, const uint32_t *lin_cnt, const int32_t *nid) {
     if (chain_exit(ctl, iter)) return;
     int64_t beg, end;
     my_range(lin ? (int64_t)*lin_cnt : (int64_t)n, beg, end);
@@ -663,7 +671,7 @@ __global__ void k_wcc_iter(const int32_t *__restrict__ row, const int32_t *__res
             int32_t lu = lab[u];
             m = lu < m ? lu : m;
         }
-        int32_t j = lab[m];  // pointer jump: the label of a reached vertex is reached too
+        int32_t j = lab[nid ? nid[m] : m];  // pointer jump: the label of a reached vertex is reached too
         m = j < m ? j : m;
         if (m < cur) {
             lab[v] = m;
@@ -688,28 +696,6 @@ __global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, co
     }
 }
 
-// ---------------------------------------------------------------------------
-// canonical labels: label = minimum id of the class of root[] (root[r] == r
-// for every class representative r)
-// ---------------------------------------------------------------------------
-__global__ void k_label_seed(const int32_t *root, int32_t *mn, int32_t n) {
-    int64_t beg, end;
-    my_range(n, beg, end);
-#pragma unroll 8
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x)
-        if (root[v] == (int32_t)v) mn[v] = (int32_t)v;
-}
-
-__global__ void k_label_min(const int32_t *root, int32_t *mn, int32_t n) {
-    int64_t beg, end;
-    my_range(n, beg, end);
-#pragma unroll 8
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
-        int32_t r = root[v];
-        if ((int32_t)v < mn[r]) atomicMin(&mn[r], (int32_t)v);
-    }
-}
-
 // root[v] = mn[root[v]], in place: each work-item reads and writes only its own
 // root, and writes only where the minimum differs from the pivot
 __global__ void k_root_to_min(int32_t *root, const int32_t *__restrict__ mn, int32_t n) {
@@ -722,11 +708,28 @@ __global__ void k_root_to_min(int32_t *root, const int32_t *__restrict__ mn, int
     }
 }
 
-__global__ void k_label_final(const int32_t *root, const int32_t *mn, int32_t *label, int32_t n) {
-    int64_t beg, end;
-    my_range(n, beg, end);
-#pragma unroll 8
-    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) label[v] = mn[root[v]];
+// out[i] = in[idx[i]]: labels (or any per-vertex array) from internal to the
+// caller's order, idx being nid
+__global__ void k_gather_i32(const int32_t *__restrict__ in, const int32_t *__restrict__ idx, int32_t *out,
+                             int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[idx[i]];
+}
+
+// Renumbering of a CSR: row v of (row, col) becomes row map[v] of (nrow, ncol),
+// its entries renamed through map, in the same order (nrow from the host: the
+// prefix sums of the lengths in the new order).  One wave per row, its lanes
+// copying 64 entries at a time.
+__global__ void k_permute_rows(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                               const int32_t *__restrict__ map, const int32_t *__restrict__ nrow,
+                               int32_t *__restrict__ ncol, int32_t n) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t v = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; v < n; v += waves) {
+        const int32_t e0 = row[v], e1 = row[v + 1], o = nrow[map[v]];
+        for (int32_t e = e0 + lane; e < e1; e += WAVE) ncol[o + (e - e0)] = map[col[e]];
+    }
 }
 
 // seeds of a reach given as vertex words: live vertices holding `bit`
@@ -761,13 +764,13 @@ __global__ void k_collect(const uint32_t *word, int32_t n, uint32_t bit, int32_t
 
 void launch_init(const Launch &l, uint32_t *word, int32_t *root, int32_t n, const int32_t *off,
                  int32_t G, const uint8_t *cls, const uint8_t *hlive, int32_t *unk, Ctl *ctl, uint32_t *vis,
-                 int64_t vis_words, int32_t *mn) {
+                 int64_t vis_words, int32_t *mn, const int32_t *oid) {
     if ((n & 3) == 0)
         hipLaunchKernelGGL(k_init<4>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn, oid);
     else
         hipLaunchKernelGGL(k_init<1>, GRID(l), word, root, n, off, G, cls, cls ? cls + n : nullptr,
-                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn);
+                           cls ? hlive : nullptr, unk, ctl, vis, vis_words, mn, oid);
 }
 void launch_batch_check(const Launch &l, const Graph &g, const int32_t *off, int32_t G, Ctl *ctl) {
     hipLaunchKernelGGL(k_batch_check, GRID(l), g.row, g.col, g.n, off, G, ctl);
@@ -794,14 +797,15 @@ void launch_ctl_reset(hipStream_t s, Ctl *ctl, int what, int guard_last, int gua
 void launch_update(const Launch &l, uint32_t *word, int32_t *root, int32_t n,
                    const unsigned long long *prev, unsigned long long *cur, int slot_bits,
                    unsigned long long tag, int recolour, int elect, const uint16_t *score, Ctl *ctl,
-                   const int32_t *lin, const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn) {
+                   const int32_t *lin, const uint32_t *lin_cnt, int32_t *lout, uint32_t *lout_cnt, int32_t *mn,
+                   const int32_t *oid) {
     hipLaunchKernelGGL(k_update, GRID(l), word, root, n, prev, cur, slot_bits, tag, recolour, elect,
-                       score, ctl, lin, lin_cnt, lout, lout_cnt, mn);
+                       score, ctl, lin, lin_cnt, lout, lout_cnt, mn, oid);
 }
 void launch_confirm(const Launch &l, uint32_t *word, int32_t n, const unsigned long long *cur,
                     int slot_bits, int32_t *piv, Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt,
-                    uint32_t *vis, int64_t vis_words) {
-    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl, lin, lin_cnt, vis, vis_words);
+                    uint32_t *vis, int64_t vis_words, const int32_t *oid) {
+    hipLaunchKernelGGL(k_confirm, GRID(l), word, n, cur, slot_bits, piv, ctl, lin, lin_cnt, vis, vis_words, oid);
 }
 void launch_count_state(const Launch &l, const uint32_t *word, int32_t n, int what, Ctl *ctl) {
     hipLaunchKernelGGL(k_count_state, GRID(l), word, n, what, ctl);
@@ -812,28 +816,31 @@ void launch_max_colour(const Launch &l, const uint32_t *word, int32_t n, Ctl *ct
 void launch_unset_pivots(const Launch &l, uint32_t *word, int32_t n, Ctl *ctl, const int32_t *piv) {
     hipLaunchKernelGGL(k_unset_pivots, GRID(l), word, n, ctl, piv);
 }
-void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl) {
-    hipLaunchKernelGGL(k_trim2_mark, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n);
+void launch_trim2(const Launch &l, const Graph &g, uint32_t *word, int32_t *root, Ctl *ctl, const int32_t *oid) {
+    hipLaunchKernelGGL(k_trim2_mark, GRID(l), g.row, g.col, g.rowT, g.colT, word, root, g.n, oid);
     hipLaunchKernelGGL(k_trim2_commit, GRID(l), word, g.n, ctl);
 }
-void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt) {
-    hipLaunchKernelGGL(k_wcc_init, GRID(l), lab, n, lin, lin_cnt);
+void launch_wcc_init(const Launch &l, int32_t *lab, int32_t n, const int32_t *lin, const uint32_t *lin_cnt,
+                     const int32_t *oid) {
+    hipLaunchKernelGGL(k_wcc_init, GRID(l), lab, n, lin, lin_cnt, oid);
 }
 void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int32_t *lab, Ctl *ctl,
-                     int iter, const int32_t *lin, const uint32_t *lin_cnt) {
-    hipLaunchKernelGGL(k_wcc_iter, GRID(l), g.row, g.col, word, lab, g.n, ctl, iter, lin, lin_cnt);
+                     int iter, const int32_t *lin, const uint32_t *lin_cnt, const int32_t *nid) {
+    hipLaunchKernelGGL(k_wcc_iter, GRID(l), g.row, g.col, word, lab, g.n, ctl, iter, lin, lin_cnt, nid);
 }
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
                          const Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
     hipLaunchKernelGGL(k_wcc_recolour, GRID(l), word, lab, n, ctl, lin, lin_cnt);
 }
-void launch_labels(const Launch &l, const int32_t *root, int32_t *mn, int32_t *label, int32_t n) {
-    hipLaunchKernelGGL(k_label_seed, GRID(l), root, mn, n);
-    hipLaunchKernelGGL(k_label_min, GRID(l), root, mn, n);
-    hipLaunchKernelGGL(k_label_final, GRID(l), root, mn, label, n);
-}
 void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n) {
     hipLaunchKernelGGL(k_root_to_min, GRID(l), root, mn, n);
+}
+void launch_gather_i32(const Launch &l, const int32_t *in, const int32_t *idx, int32_t *out, int64_t count) {
+    hipLaunchKernelGGL(k_gather_i32, GRID(l), in, idx, out, count);
+}
+void launch_permute_rows(const Launch &l, const int32_t *row, const int32_t *col, const int32_t *map,
+                         const int32_t *nrow, int32_t *ncol, int32_t n) {
+    hipLaunchKernelGGL(k_permute_rows, GRID(l), row, col, map, nrow, ncol, n);
 }
 void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t bit, int32_t *q,
                     Ctl *ctl) {

@@ -267,6 +267,7 @@ struct ScoreArgs {
     unsigned long long *cur;
     unsigned long long tag;
     const Ctl *ctl;
+    const int32_t *oid;  // the caller's ids (the keys' last field), null: the same
 };
 
 __global__ __launch_bounds__(MLP_BLOCK) void k_score(ScoreArgs a) {
@@ -364,8 +365,8 @@ __global__ __launch_bounds__(MLP_BLOCK) void k_score(ScoreArgs a) {
         if (v >= 0 && a.score_out) a.score_out[v] = sc;
         if (!a.cur) continue;  // kernel-uniform
         // the election: one pending minimum per wave while its candidates share a slot
-        const unsigned long long key = live ? (a.tag | ((unsigned long long)(0xFFFFu - sc) << 32) | (uint32_t)v)
-                                            : ~0ull;
+        const unsigned long long key =
+            live ? (a.tag | ((unsigned long long)(0xFFFFu - sc) << 32) | (uint32_t)orig_id(a.oid, v)) : ~0ull;
         const unsigned long long bm = __ballot(live);
         if (bm == 0) continue;  // wave-uniform
         const uint32_t sl0 = __shfl(sl, __ffsll((long long)bm) - 1, WAVE);
@@ -546,9 +547,9 @@ void launch_colour_count(const Launch &l, const uint32_t *word, int32_t n, uint3
 void launch_score(const Launch &l, const Graph &g, const uint32_t *word, const float4 *stat, const uint32_t *cnt,
                   const uint32_t *one_set, int slot_bits, float step_f, int static_deg, const int32_t *lin, const uint32_t *lin_cnt,
                   const uint16_t *params, uint16_t *score_out, float *feat_out, unsigned long long *cur,
-                  unsigned long long tag, const Ctl *ctl) {
+                  unsigned long long tag, const Ctl *ctl, const int32_t *oid) {
     ScoreArgs a{g.row, g.col, g.rowT, g.colT, word, stat, cnt, one_set, slot_bits, step_f, static_deg, g.n,
-                lin, lin_cnt, params, score_out, feat_out, cur, tag, ctl};
+                lin, lin_cnt, params, score_out, feat_out, cur, tag, ctl, oid};
     hipLaunchKernelGGL(k_score, GRID_MLP(l), a);
 }
 void launch_mlp_infer(const Launch &l, const float *X, int64_t N, const uint16_t *params, float *logit) {

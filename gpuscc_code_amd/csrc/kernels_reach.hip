@@ -1254,7 +1254,7 @@ __device__ inline bool trim_check(const ReachArgs &a, int32_t v, uint32_t c, uin
     smax = steps > smax ? steps : smax;
     if (keep) return false;
     a.word[v] = make_word(c, 28);  // P|T|E (scc_routines.cu:381)
-    a.root[v] = v;
+    a.root[v] = orig_id(a.oid, v);
     return true;
 }
 
@@ -1318,7 +1318,7 @@ __device__ inline uint32_t trim_round_scan(const ReachArgs &a, int64_t base, int
         for (int j = 0; j < SCAN_ILP; j++) {
             if (v[j] >= 0 && (((hit >> j) & (hit >> (SCAN_ILP + j))) & 1u) == 0) {
                 a.word[v[j]] = make_word(c[j], 28);  // P|T|E (scc_routines.cu:381)
-                a.root[v[j]] = v[j];
+                a.root[v[j]] = orig_id(a.oid, v[j]);
                 trimmed |= 1u << kk[j];
                 newp += (w[j] & ST_P) ? 0u : 1u;  // a trimmed pivot holds P already
             }

@@ -326,7 +326,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
             if (sp.each_nb(0, v, live) && sp.each_nb(1, v, live)) continue;
             sp.st(v, make_word(c, 28));  // P|T|E (scc_routines.cu:381)
             const int32_t g = sp.gid(v);
-            t.root[g] = g;
+            t.root[g] = orig_id(t.oid, g);
             atomicAdd(&sh.red[0], 1ull);
             if (!(w & ST_P)) atomicAdd(&sh.red[1], 1ull);  // a trimmed pivot holds P already
         }
@@ -353,19 +353,19 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         if (w & ST_E) continue;
         uint32_t c = colour_of(w);
         const uint32_t f = ((w & ST_F) ? 1u : 0u) + ((w & ST_B) ? 2u : 0u);
-        const int32_t g = sp.gid(v);
+        const int32_t g = sp.gid(v), og = orig_id(t.oid, g);  // keys and roots: the caller's ids
         if (f == 3) {
             sp.st(v, w | ST_E);
             const uint32_t pv = sp.prev_winner(t, r.pt, slot_of(c, t.slot_bits));
-            t.root[g] = pv < (uint32_t)t.n ? (int32_t)pv : g;
-            if (t.mn && pv < (uint32_t)t.n && g < t.mn[pv]) atomicMin(&t.mn[pv], g);
+            t.root[g] = pv < (uint32_t)t.n ? (int32_t)pv : og;
+            if (t.mn && pv < (uint32_t)t.n && og < t.mn[pv]) atomicMin(&t.mn[pv], og);
             atomicAdd(&sh.red[0], 1ull);
             continue;
         }
         if (!r.keep_colour) c = (3u * c + f) & COLOUR_MASK;
         sp.st(v, make_word(c, 0));
         sp.keep(atomicAdd(&sh.lcnt, 1u), v);
-        sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)g);
+        sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)og);
     }
     tail_sync();
     // ---- confirm: the lowest key of each slot becomes its colour's pivot (state 19)
@@ -375,12 +375,12 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         const uint32_t w = sp.ld(v);
         if (w & ST_E) continue;
         const uint32_t c = colour_of(w);
-        const int32_t g = sp.gid(v);
+        const uint32_t og = (uint32_t)orig_id(t.oid, sp.gid(v));
         const uint32_t win = sp.winner(t, r.pt, slot_of(c, t.slot_bits));
-        if (win != (uint32_t)g) continue;
+        if (win != og) continue;
         sp.st(v, make_word(c, 19));
         sp.pivot(atomicAdd(&sh.npiv, 1u), v);
-        atomicMin(&sh.first, (uint32_t)g);
+        atomicMin(&sh.first, og);
     }
     tail_sync();
     r.npiv = sh.npiv;
@@ -493,7 +493,7 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LS &ls, uint32_t L,
     if (sh.flag) return -1;
     if (ls.H)
         for (uint32_t k = tid; k < npiv; k += nt)
-            ls.insert(pt, slot_of(colour_of(lw[q0[k]]), t.slot_bits), (uint32_t)gidv[q0[k]]);
+            ls.insert(pt, slot_of(colour_of(lw[q0[k]]), t.slot_bits), (uint32_t)orig_id(t.oid, gidv[q0[k]]));
     tail_sync();
     return 1;
 }
@@ -521,7 +521,8 @@ __device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShar
     for (uint32_t k = tid; k < npiv; k += nt) {
         const int32_t g = gidv[q0[k]];
         t.piv[k] = g;
-        if (ls.H) t.slots[pt][slot_of(colour_of(lw[q0[k]]), t.slot_bits)] = elect_tag(epoch) | (uint32_t)g;
+        if (ls.H)
+            t.slots[pt][slot_of(colour_of(lw[q0[k]]), t.slot_bits)] = elect_tag(epoch) | (uint32_t)orig_id(t.oid, g);
     }
     tail_sync();
     if (tid == 0) t.ctl->lcnt[lsel] = sh.lcnt;
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_batch(TailArgs base, BatchArg
         for (int32_t i = (int32_t)tid; i < ng; i += TAIL_BLOCK) {
             const int32_t v = o0 + i;
             t.word[v] = make_word(colour, 0);
-            t.root[v] = v;
+            t.root[v] = orig_id(t.oid, v);
             t.list[0][i] = v;
         }
         GraphSlots gs;

@@ -26,6 +26,9 @@ MLP_PARAMS, MLP_FEATURES, MLP_HIDDEN = 1505, 12, 32
 LEVEL_FIELDS = ("kind", "level", "ms", "queue_f", "queue_b", "arcs_f", "arcs_b", "pulled",
                 "expanded", "edges", "visited", "max_scan")  # one row of scc_profile_levels
 COLOUR_MASK = 0x00FFFFFF
+# default of alg_opt.relabel (0: the library decides by the degree skew); a test
+# session may set it to 1 so that every upload renumbers its vertices
+DEFAULT_RELABEL = 0
 
 
 @dataclass
@@ -44,6 +47,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     reach_block: int = 0
     tail_live: int = 0
     batch_solver: int = 0
+    relabel: int = None  # None: DEFAULT_RELABEL
 
     def to_c(self):
         o = SccOpts()
@@ -60,6 +64,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.reach_block = self.reach_block
         o.tail_live = self.tail_live
         o.batch_solver = self.batch_solver
+        o.relabel = DEFAULT_RELABEL if self.relabel is None else self.relabel
         return o
 
 

@@ -96,6 +96,12 @@ typedef struct {
                                   a workgroup per graph at a time, largest first, each with its own
                                   election tables (one host sync; slower than 0 once graphs exceed a
                                   few thousand vertices) */
+    int32_t relabel;           /* read by scc_upload (and scc_set_batch): renumber the vertices inside the
+                                  library by decreasing degree (within each graph of a batch), hubs
+                                  first, so the words and rows the scans touch most share cache lines.
+                                  Results are unchanged: elections, roots and labels use the caller's
+                                  ids.  0 (default): when the degrees are skewed (largest above 64
+                                  times the mean) and n >= 65536; 1: always; -1: never */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -114,6 +120,7 @@ typedef struct {
     int64_t first_pivot;    /* first pivot (lowest live id after trim-1, scc.cu:235-245), -1 if none */
     int64_t tail_steps;     /* of steps, those run in the one-workgroup tail (scc_opts.tail_live) */
     int64_t tail_live;      /* live vertices when the tail started, 0 if it did not run */
+    int64_t renumbered;     /* 1: the upload renumbered the vertices by degree (scc_opts.relabel) */
     double ms_total;        /* upload-excluded wall time, labels ready (scc.cu:217->1011) */
     double ms_fwd, ms_bwd, ms_trim, ms_update; /* per phase, host clock (STOPWATCH, timing.h:14) */
 } scc_stats;
