@@ -1062,6 +1062,239 @@ __device__ inline void commit_round(const ReachArgs &a, int64_t base, const uint
     __syncthreads();  // sbase is rewritten by the next round
 }
 
+// ---------------------------------------------------------------------------
+// Pull sweep of one level (the directions in pull0 / pull1): every live vertex
+// not yet reached in a pulling direction looks for a reached neighbour over its
+// reverse row.  Rounds of PULL_V vertices per work-item (vertex base + k * nt +
+// lane for bit k); each round loads its PULL_V words, giving the candidates per
+// direction, then settles them in batches of PULL_T (vertex, direction) tasks:
+//   heads -- the reverse row's first entry (from the head array) and its
+//            visited bit (first FB step) or word; hits are claimed, and a miss
+//            whose row holds more than its head is left for
+//   rows  -- the rest of those rows, a batch in lockstep.
+// A round with many candidates (at least PULL_DENSE per lane over its wave)
+// takes the bits k in order, every lane the same k, so that a batch's loads and
+// claims hit consecutive vertices (a few cache lines per wave); a sparse round
+// packs each lane's own candidates into its batches instead.  Every load of a
+// batch is at an address valid whatever the branch (an idle task reads a
+// vertex of the round or beg, an empty head vertex 0, an idle scan the column
+// array's first entry) and is selected afterwards, so the loads of a batch
+// leave together.  No barrier until the round's claims are queued.
+// ---------------------------------------------------------------------------
+constexpr int PULL_T = 4;      // tasks per batch (more spill registers)
+constexpr int PULL_DENSE = 4;  // candidates per lane (wave average) for the in-order batches
+
+struct PullTasks {
+    int kk[PULL_T];      // bit of the task's vertex in the round (-1: idle)
+    int32_t v[PULL_T];   // its vertex (a valid id when idle)
+    uint32_t dm;         // bit t: task t pulls B (else F)
+};
+
+// Claims of the tasks in hit: the words' atomics and the class bytes of the
+// claimed rows (the next level's arc estimate) leave together.
+__device__ inline void pull_claims(const ReachArgs &a, uint32_t hit, const PullTasks &q, uint32_t (&claimed)[2],
+                                   unsigned long long (&arcs)[2]) {
+    uint32_t old[PULL_T], cs[PULL_T];
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        const bool b = (q.dm >> t) & 1u;
+        old[t] = ST_F | ST_B;
+        cs[t] = 0u;
+        if ((hit >> t) & 1u) {
+            old[t] = atomicOr(&a.word[q.v[t]], b ? ST_B : ST_F);
+            cs[t] = (b ? a.dcls[1] : a.dcls[0])[q.v[t]];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        const bool b = (q.dm >> t) & 1u;
+        if (old[t] & (b ? ST_B : ST_F)) continue;
+        if (b) {
+            claimed[1] |= 1u << q.kk[t];
+            arcs[1] += class_arcs(cs[t]);
+        } else {
+            claimed[0] |= 1u << q.kk[t];
+            arcs[0] += class_arcs(cs[t]);
+        }
+    }
+}
+
+// The heads of a batch; misses with more entries join pend0 / pend1.
+__device__ inline void pull_heads(const ReachArgs &a, bool vis, const PullTasks &q, uint32_t &pend0, uint32_t &pend1,
+                                  uint32_t (&claimed)[2], unsigned long long (&arcs)[2], unsigned long long &cx,
+                                  unsigned long long &ce, uint32_t &smax) {
+    int32_t h[PULL_T];
+    uint32_t cr[PULL_T], wv[PULL_T], x[PULL_T];
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        const bool b = (q.dm >> t) & 1u;
+        h[t] = (b ? a.head[0] : a.head[1])[q.v[t]];   // head of the reverse row
+        cr[t] = (b ? a.dcls[0] : a.dcls[1])[q.v[t]];  // the reverse row's length class
+        wv[t] = vis ? 0u : a.word[q.v[t]];           // the candidate's colour
+    }
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        const int32_t u = h[t] >= 0 ? h[t] : 0;
+        x[t] = vis ? (((q.dm >> t) & 1u) ? a.vis[1] : a.vis[0])[u >> 5] : a.word[u];
+    }
+    uint32_t hit = 0u;
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        if (q.kk[t] < 0 || h[t] < 0) continue;  // an empty reverse row: never reached by a pull
+        cx++;
+        ce++;
+        smax = smax > 1u ? smax : 1u;  // a scan of one entry
+        const uint32_t bit = ((q.dm >> t) & 1u) ? ST_B : ST_F;
+        const bool ok = vis ? ((x[t] >> (h[t] & 31)) & 1u) != 0u
+                            : (x[t] & (bit | ST_E)) == bit && colour_of(x[t]) == colour_of(wv[t]);
+        if (ok) {
+            hit |= 1u << t;
+        } else if ((cr[t] & 0x3Fu) >= 2u) {  // at least two entries
+            if ((q.dm >> t) & 1u) pend1 |= 1u << q.kk[t];
+            else pend0 |= 1u << q.kk[t];
+        }
+    }
+    pull_claims(a, hit, q, claimed, arcs);
+}
+
+// Entries 1.. of a batch's rows (each of at least two entries), in lockstep.
+__device__ inline void pull_rows(const ReachArgs &a, bool vis, const PullTasks &q, uint32_t (&claimed)[2],
+                                 unsigned long long (&arcs)[2], unsigned long long &ce, uint32_t &smax) {
+    int32_t e[PULL_T], e1[PULL_T];
+    uint32_t wv[PULL_T], act = 0u;
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        const int32_t *rw = ((q.dm >> t) & 1u) ? a.row : a.rowT;
+        e[t] = rw[q.v[t]] + 1;  // the head is checked
+        e1[t] = rw[q.v[t] + 1];
+        wv[t] = vis ? 0u : a.word[q.v[t]];
+        if (q.kk[t] >= 0) act |= 1u << t;
+    }
+    uint32_t hit = 0u;
+    int s = 0;
+    for (; act; s++) {
+        int32_t u[PULL_T];
+        uint32_t x[PULL_T];
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++) u[t] = (((q.dm >> t) & 1u) ? a.col : a.colT)[((act >> t) & 1u) ? e[t] : 0];
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++)
+            x[t] = vis ? (((q.dm >> t) & 1u) ? a.vis[1] : a.vis[0])[u[t] >> 5] : a.word[u[t]];
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++) {
+            if (!((act >> t) & 1u)) continue;
+            ce++;
+            const uint32_t bit = ((q.dm >> t) & 1u) ? ST_B : ST_F;
+            const bool ok = vis ? ((x[t] >> (u[t] & 31)) & 1u) != 0u
+                                : (x[t] & (bit | ST_E)) == bit && colour_of(x[t]) == colour_of(wv[t]);
+            if (ok) {
+                hit |= 1u << t;
+                act &= ~(1u << t);
+            } else if (++e[t] >= e1[t]) {
+                act &= ~(1u << t);
+            }
+        }
+    }
+    smax = (uint32_t)s + 1u > smax ? (uint32_t)s + 1u : smax;
+    pull_claims(a, hit, q, claimed, arcs);
+}
+
+// Packs the next PULL_T tasks of one lane's masks (F first).
+__device__ inline void pull_pack(PullTasks &q, uint32_t &m0, uint32_t &m1, int64_t base, uint32_t nt, int64_t beg) {
+    q.dm = 0u;
+#pragma unroll
+    for (int t = 0; t < PULL_T; t++) {
+        q.kk[t] = -1;
+        if (m0) {
+            q.kk[t] = __ffs(m0) - 1;
+            m0 &= m0 - 1u;
+        } else if (m1) {
+            q.kk[t] = __ffs(m1) - 1;
+            m1 &= m1 - 1u;
+            q.dm |= 1u << t;
+        }
+        q.v[t] = q.kk[t] >= 0 ? (int32_t)(base + (int64_t)q.kk[t] * nt + threadIdx.x) : (int32_t)beg;
+    }
+}
+
+__device__ inline void pull_sweep(const ReachArgs &a, bool pull0, bool pull1, Items &it, Stage &st,
+                                  unsigned long long *sh, bool &found, unsigned long long &cx,
+                                  unsigned long long &ce, unsigned long long &cv, uint32_t &smax) {
+    const bool vis = a.vis[0] != nullptr;  // kernel-uniform
+    int64_t beg, end;
+    my_range(a.n, beg, end);
+    for (int64_t base = beg; base < end; base += (int64_t)a.nt * PULL_V) {  // block-uniform
+        uint32_t cm0 = 0u, cm1 = 0u;  // candidates per pulling direction
+#pragma unroll
+        for (int k = 0; k < PULL_V; k++) {
+            const int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
+            const uint32_t w = a.word[vv < end ? vv : beg];
+            if (vv < end && !(w & ST_E)) {
+                if (pull0 && !(w & ST_F)) cm0 |= 1u << k;
+                if (pull1 && !(w & ST_B)) cm1 |= 1u << k;
+            }
+        }
+        uint32_t claimed[2] = {0u, 0u};
+        unsigned long long arcs[2] = {0ull, 0ull};
+        uint32_t pend0 = 0u, pend1 = 0u;  // misses whose rows go on past the head
+        uint32_t wc = (uint32_t)(__popc(cm0) + __popc(cm1));
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) wc += (uint32_t)__shfl_xor((int)wc, o, WAVE);
+        PullTasks q;
+        if (wc >= (uint32_t)(PULL_DENSE * WAVE)) {  // wave-uniform: bits in order, lanes in step
+            constexpr int KB = PULL_T / 2;  // bits per batch, both directions each
+            for (int k0 = 0; k0 < PULL_V; k0 += KB) {
+                const uint32_t sel = ((1u << KB) - 1u) << k0;
+                if (!__any(((cm0 | cm1) & sel) != 0u)) continue;  // wave-uniform
+                q.dm = 0u;
+#pragma unroll
+                for (int t = 0; t < PULL_T; t++) {
+                    const int k = k0 + t / 2;
+                    const bool b = (t & 1) != 0;
+                    const int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
+                    q.kk[t] = (((b ? cm1 : cm0) >> k) & 1u) ? k : -1;
+                    q.v[t] = (int32_t)(vv < end ? vv : beg);
+                    if (b) q.dm |= 1u << t;
+                }
+                pull_heads(a, vis, q, pend0, pend1, claimed, arcs, cx, ce, smax);
+            }
+        } else {
+            while (cm0 | cm1) {  // per work-item
+                pull_pack(q, cm0, cm1, base, a.nt, beg);
+                pull_heads(a, vis, q, pend0, pend1, claimed, arcs, cx, ce, smax);
+            }
+        }
+        while (pend0 | pend1) {  // per work-item
+            pull_pack(q, pend0, pend1, base, a.nt, beg);
+            pull_rows(a, vis, q, claimed, arcs, ce, smax);
+        }
+        if (claimed[0] | claimed[1]) {
+            found = true;
+            cv += (unsigned long long)(__popc(claimed[0]) + __popc(claimed[1]));
+        }
+        if (vis) {
+            // the round's claims into the visited bitmaps: vertex base + k * nt + lane
+            // of a wave is one of 64 consecutive vertices starting at a multiple of 64
+            // (beg and nt are), two bitmap words, one atomic each
+            const uint32_t lane = threadIdx.x & (WAVE - 1);
+            const int64_t w0 = base + (int64_t)(threadIdx.x - lane);
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                uint32_t *bm = dvis(a, d);
+#pragma unroll
+                for (int k = 0; k < PULL_V; k++) {
+                    const unsigned long long m = __ballot((claimed[d] >> k) & 1u);
+                    if (m == 0) continue;  // wave-uniform
+                    const int64_t v0 = w0 + (int64_t)k * a.nt;
+                    if (lane == 0 && (uint32_t)m) atomicOr(&bm[v0 >> 5], (uint32_t)m);
+                    if (lane == 32 && (uint32_t)(m >> 32)) atomicOr(&bm[(v0 >> 5) + 1], (uint32_t)(m >> 32));
+                }
+            }
+        }
+        commit_round(a, base, claimed, arcs, it.shw, st.base, sh);
+    }
+}
+
 __global__ void k_reach_hybrid(ReachArgs a) {
     __shared__ ItemsLocal ul;
     __shared__ Stage st;
@@ -1098,115 +1331,7 @@ __global__ void k_reach_hybrid(ReachArgs a) {
     if (pull0 || pull1) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             a.ctl->pull_levels += (pull0 ? 1u : 0u) + (pull1 ? 1u : 0u);
-        // Rounds of PULL_V vertices per work-item: all their words are loaded at
-        // once, then only the candidates (live, not reached in a pulling
-        // direction) are scanned, SCAN_ILP at a time in lockstep.  No barrier
-        // until the round's claims are queued.
-        int64_t beg, end;
-        my_range(a.n, beg, end);
-        constexpr int T = 2 * SCAN_ILP;  // task d * SCAN_ILP + j: candidate j, direction d
-        for (int64_t base = beg; base < end; base += (int64_t)a.nt * PULL_V) {  // block-uniform
-            uint32_t cm[2] = {0u, 0u};  // candidates per pulling direction
-#pragma unroll
-            for (int k = 0; k < PULL_V; k++) {
-                int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
-                uint32_t w = vv < end ? a.word[vv] : ST_E;
-                if (!(w & ST_E)) {
-                    if (pull0 && !(w & ST_F)) cm[0] |= 1u << k;
-                    if (pull1 && !(w & ST_B)) cm[1] |= 1u << k;
-                }
-            }
-            uint32_t cand = cm[0] | cm[1];
-            uint32_t claimed[2] = {0u, 0u};
-            unsigned long long arcs[2] = {0ull, 0ull};
-            while (cand) {  // per work-item
-                int32_t v[SCAN_ILP];
-                int kk[SCAN_ILP];
-#pragma unroll
-                for (int j = 0; j < SCAN_ILP; j++) {
-                    kk[j] = cand ? __ffs(cand) - 1 : -1;
-                    cand &= cand ? cand - 1 : 0u;
-                    v[j] = kk[j] >= 0 ? (int32_t)(base + (int64_t)kk[j] * a.nt + threadIdx.x) : -1;
-                }
-                // the words (colours) load together with the heads and row bounds:
-                // which rows to probe is known from the round's first load
-                uint32_t w[SCAN_ILP];
-#pragma unroll
-                for (int j = 0; j < SCAN_ILP; j++) w[j] = v[j] >= 0 ? a.word[v[j]] : ST_E;
-                const int32_t *col[T];
-                int32_t e[T], e1[T], u0[T], self[T];
-                uint32_t mask[T], val[T], c[T], act = 0;
-#pragma unroll
-                for (int t = 0; t < T; t++) {
-                    const int d = t / SCAN_ILP, j = t % SCAN_ILP;
-                    // a visit in direction d needs a reached neighbour over the reverse arcs
-                    const int32_t *row = drow(a, 1 - d);
-                    col[t] = dcol(a, 1 - d);
-                    mask[t] = dbit(d) | ST_E;
-                    val[t] = dbit(d);
-                    self[t] = -1;
-                    u0[t] = -1;
-                    e[t] = e1[t] = 0;
-                    if (kk[j] >= 0 && ((cm[d] >> kk[j]) & 1u)) {
-                        u0[t] = a.head[1 - d][v[j]];
-                        e[t] = row[v[j]] + 1;
-                        e1[t] = row[v[j] + 1];
-                        cx++;
-                        if (u0[t] >= 0) act |= 1u << t;
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < T; t++) c[t] = colour_of(w[t % SCAN_ILP]);
-                uint32_t hit;
-                if (a.vis[0]) {
-                    const uint32_t *bm[T];
-#pragma unroll
-                    for (int t = 0; t < T; t++) bm[t] = dvis(a, t / SCAN_ILP);
-                    hit = scan_rows_vis<T>(bm, col, e, e1, u0, self, act, ce, smax);
-                } else {
-                    hit = scan_rows<T>(a.word, col, e, e1, u0, mask, val, c, self, act, ce, smax);
-                }
-                // claims: the class bytes load with the atomics
-                uint32_t cls[T];
-#pragma unroll
-                for (int t = 0; t < T; t++)
-                    cls[t] = ((hit >> t) & 1u) ? a.dcls[t / SCAN_ILP][v[t % SCAN_ILP]] : 0u;
-#pragma unroll
-                for (int t = 0; t < T; t++) {
-                    if (!((hit >> t) & 1u)) continue;
-                    const int d = t / SCAN_ILP, j = t % SCAN_ILP;
-                    uint32_t old = atomicOr(&a.word[v[j]], dbit(d));
-                    if (!(old & dbit(d))) {
-                        claimed[d] |= 1u << kk[j];
-                        arcs[d] += class_arcs(cls[t]);
-                    }
-                }
-            }
-            if (claimed[0] | claimed[1]) {
-                found = true;
-                cv += (unsigned long long)(__popc(claimed[0]) + __popc(claimed[1]));
-            }
-            if (a.vis[0]) {
-                // the round's claims into the visited bitmaps: vertex base + k * nt + lane
-                // of a wave is one of 64 consecutive vertices starting at a multiple of 64
-                // (beg and nt are), two bitmap words, one atomic each
-                const uint32_t lane = threadIdx.x & (WAVE - 1);
-                const int64_t w0 = base + (int64_t)(threadIdx.x - lane);
-#pragma unroll
-                for (int d = 0; d < 2; d++) {
-                    uint32_t *bm = dvis(a, d);
-#pragma unroll
-                    for (int k = 0; k < PULL_V; k++) {
-                        const unsigned long long m = __ballot((claimed[d] >> k) & 1u);
-                        if (m == 0) continue;  // wave-uniform
-                        const int64_t v0 = w0 + (int64_t)k * a.nt;
-                        if (lane == 0 && (uint32_t)m) atomicOr(&bm[v0 >> 5], (uint32_t)m);
-                        if (lane == 32 && (uint32_t)(m >> 32)) atomicOr(&bm[(v0 >> 5) + 1], (uint32_t)(m >> 32));
-                    }
-                }
-            }
-            commit_round(a, base, claimed, arcs, it.shw, st.base, sh);
-        }
+        pull_sweep(a, pull0, pull1, it, st, sh, found, cx, ce, cv, smax);
     }
     local_rounds(a, (pull0 || pull1) ? 0xFFFFFFFFu : qs0 + qs1, st, lq, found, cx, ce, cv);
     st_flush(st, a);
