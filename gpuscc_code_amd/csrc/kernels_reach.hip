@@ -1199,6 +1199,69 @@ __device__ inline void pull_rows(const ReachArgs &a, bool vis, const PullTasks &
     pull_claims(a, hit, q, claimed, arcs);
 }
 
+// The heads of bits k0 .. k0 + PULL_KB - 1 of a dense round, every lane the
+// same bits (consecutive vertices per wave), both pulling directions at once:
+// one class byte per row direction serves as the reverse row's length in one
+// direction and the claimed row's arcs in the other, and a vertex reached both
+// ways is claimed by one atomic.
+constexpr int PULL_KB = 4;
+
+__device__ inline void pull_dense(const ReachArgs &a, bool vis, bool pull0, bool pull1, int64_t base, int64_t beg,
+                                  int64_t end, int k0, uint32_t cm0, uint32_t cm1, uint32_t &pend0, uint32_t &pend1,
+                                  uint32_t (&claimed)[2], unsigned long long (&arcs)[2], unsigned long long &cx,
+                                  unsigned long long &ce, uint32_t &smax) {
+    int32_t v[PULL_KB], hf[PULL_KB], hb[PULL_KB];
+    uint32_t c0[PULL_KB], c1[PULL_KB], wv[PULL_KB], xf[PULL_KB], xb[PULL_KB];
+#pragma unroll
+    for (int j = 0; j < PULL_KB; j++) {
+        const int64_t vv = base + (int64_t)(k0 + j) * a.nt + threadIdx.x;
+        v[j] = (int32_t)(vv < end ? vv : beg);
+        hf[j] = pull0 ? a.head[1][v[j]] : -1;  // F pulls over the in-row
+        hb[j] = pull1 ? a.head[0][v[j]] : -1;  // B over the out-row
+        c0[j] = a.dcls[0][v[j]];
+        c1[j] = a.dcls[1][v[j]];
+        wv[j] = vis ? 0u : a.word[v[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < PULL_KB; j++) {
+        const int32_t uf = hf[j] >= 0 ? hf[j] : 0, ub = hb[j] >= 0 ? hb[j] : 0;
+        xf[j] = pull0 ? (vis ? a.vis[0][uf >> 5] : a.word[uf]) : 0u;
+        xb[j] = pull1 ? (vis ? a.vis[1][ub >> 5] : a.word[ub]) : 0u;
+    }
+    uint32_t bits[PULL_KB], old[PULL_KB];
+#pragma unroll
+    for (int j = 0; j < PULL_KB; j++) {
+        const int k = k0 + j;
+        bits[j] = 0u;
+        const bool cf = ((cm0 >> k) & 1u) && hf[j] >= 0, cb = ((cm1 >> k) & 1u) && hb[j] >= 0;
+        const bool okf = cf && (vis ? ((xf[j] >> (hf[j] & 31)) & 1u) != 0u
+                                    : (xf[j] & (ST_F | ST_E)) == ST_F && colour_of(xf[j]) == colour_of(wv[j]));
+        const bool okb = cb && (vis ? ((xb[j] >> (hb[j] & 31)) & 1u) != 0u
+                                    : (xb[j] & (ST_B | ST_E)) == ST_B && colour_of(xb[j]) == colour_of(wv[j]));
+        cx += (cf ? 1u : 0u) + (cb ? 1u : 0u);
+        ce += (cf ? 1u : 0u) + (cb ? 1u : 0u);
+        if (cf || cb) smax = smax > 1u ? smax : 1u;  // a scan of one entry
+        if (okf) bits[j] |= ST_F;
+        else if (cf && (c1[j] & 0x3Fu) >= 2u) pend0 |= 1u << k;  // at least two entries
+        if (okb) bits[j] |= ST_B;
+        else if (cb && (c0[j] & 0x3Fu) >= 2u) pend1 |= 1u << k;
+        old[j] = bits[j] ? atomicOr(&a.word[v[j]], bits[j]) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < PULL_KB; j++) {
+        const int k = k0 + j;
+        const uint32_t won = bits[j] & ~old[j];
+        if (won & ST_F) {
+            claimed[0] |= 1u << k;
+            arcs[0] += class_arcs(c0[j]);
+        }
+        if (won & ST_B) {
+            claimed[1] |= 1u << k;
+            arcs[1] += class_arcs(c1[j]);
+        }
+    }
+}
+
 // Packs the next PULL_T tasks of one lane's masks (F first).
 __device__ inline void pull_pack(PullTasks &q, uint32_t &m0, uint32_t &m1, int64_t base, uint32_t nt, int64_t beg) {
     q.dm = 0u;
@@ -1242,21 +1305,11 @@ __device__ inline void pull_sweep(const ReachArgs &a, bool pull0, bool pull1, It
         for (int o = WAVE / 2; o > 0; o >>= 1) wc += (uint32_t)__shfl_xor((int)wc, o, WAVE);
         PullTasks q;
         if (wc >= (uint32_t)(PULL_DENSE * WAVE)) {  // wave-uniform: bits in order, lanes in step
-            constexpr int KB = PULL_T / 2;  // bits per batch, both directions each
-            for (int k0 = 0; k0 < PULL_V; k0 += KB) {
-                const uint32_t sel = ((1u << KB) - 1u) << k0;
+            for (int k0 = 0; k0 < PULL_V; k0 += PULL_KB) {
+                const uint32_t sel = ((1u << PULL_KB) - 1u) << k0;
                 if (!__any(((cm0 | cm1) & sel) != 0u)) continue;  // wave-uniform
-                q.dm = 0u;
-#pragma unroll
-                for (int t = 0; t < PULL_T; t++) {
-                    const int k = k0 + t / 2;
-                    const bool b = (t & 1) != 0;
-                    const int64_t vv = base + (int64_t)k * a.nt + threadIdx.x;
-                    q.kk[t] = (((b ? cm1 : cm0) >> k) & 1u) ? k : -1;
-                    q.v[t] = (int32_t)(vv < end ? vv : beg);
-                    if (b) q.dm |= 1u << t;
-                }
-                pull_heads(a, vis, q, pend0, pend1, claimed, arcs, cx, ce, smax);
+                pull_dense(a, vis, pull0, pull1, base, beg, end, k0, cm0, cm1, pend0, pend1, claimed, arcs, cx, ce,
+                           smax);
             }
         } else {
             while (cm0 | cm1) {  // per work-item
