@@ -1391,8 +1391,12 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
                     c->spec_reach.resize(k + 1, 1);
                     c->spec_trim.resize(k + 1, 1);
                 }
-                c->spec_reach[k] = (int)dr;
-                c->spec_trim[k] = (int)dt;
+                // the longest chains seen: a pull level may reach further in one run than in
+                // another (claims made earlier in the same level are seen by later rows), so
+                // the depth of a step varies by a level or so; a chain one level too long
+                // costs an empty launch, one too short a wait and a resumed chain
+                c->spec_reach[k] = std::max(c->spec_reach[k], (int)dr);
+                c->spec_trim[k] = std::max(c->spec_trim[k], (int)dt);
                 if (variant != SCC_VARIANT_PLAIN && !wcc_done) {
                     ne = (int64_t)(elected ? h.fbelim : c->h_ctl->fbelim);
                     if (ne >= one_pct) {
