@@ -51,6 +51,31 @@ struct TailShared {
     uint32_t scan[TAIL_BLOCK / WAVE];
 };
 
+// The state of one run of steps.  The buffers a run works in (queues, live
+// lists, pivot list: a batch graph's slices of them) and its step bound are
+// kept here rather than in a modified copy of the kernel's TailArgs: a copy
+// whose pointer arrays are indexed at run time would live in scratch memory,
+// and every access to it would be a global load.  Two-element choices are
+// selects for the same reason.
+struct TailRun {
+    int lsel, pt;
+    uint32_t epoch, npiv;
+    unsigned long long depths, trims;
+    uint32_t lc0, lc1;  // a batch graph's live-list lengths (the tail keeps them in ctl->lcnt)
+    uint32_t first;     // a batch graph's lowest pivot of its last election
+    int keep_colour;    // the step from no pivots: its election keeps the colours (the first
+                        // election, scc.cu:235-245, elects before any split)
+    int32_t *q0, *q1, *q2, *q3, *l0, *l1, *piv;  // queues, live lists, pivot list
+    int32_t vbase;       // the vertex the slices start at (a batch graph's first), else 0
+    uint32_t max_steps;  // bounds the steps, and each step's levels and trim rounds
+    __device__ int32_t *list(int k) const { return k ? l1 : l0; }
+    __device__ uint32_t lcnt(int k) const { return k ? lc1 : lc0; }
+    __device__ void set_lcnt(int k, uint32_t v) {
+        if (k) lc1 = v;
+        else lc0 = v;
+    }
+};
+
 // The election through the global slot tables (the multi-kernel path's):
 // prev = slots[pt] holds the last election's keys, cur = slots[pt ^ 1] this one's.
 struct GlobalSlots {
@@ -77,12 +102,13 @@ constexpr uint32_t TAB_EMPTY = 0xFFFFFFFFu;  // no slot id (slot ids are below 2
 struct GraphSlots {
     uint32_t *tab[2];  // 2H words each: slot ids, then lowest ids
     uint32_t H;
+    __device__ uint32_t *tb(int k) const { return (k & 1) ? tab[1] : tab[0]; }  // a select, not an indexed load
     __device__ void clear_cur(const TailArgs &, int pt, uint32_t tid) const {
-        uint32_t *e = tab[pt ^ 1];
+        uint32_t *e = tb(pt ^ 1);
         for (uint32_t i = tid; i < 2u * H; i += TAIL_BLOCK) e[i] = TAB_EMPTY;
     }
     __device__ uint32_t find(int k, uint32_t s) const {
-        const uint32_t *e = tab[k & 1];
+        const uint32_t *e = tb(k);
         for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {
             const uint32_t x = __hip_atomic_load(&e[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (x == s) return __hip_atomic_load(&e[H + h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -91,7 +117,7 @@ struct GraphSlots {
         return TAB_EMPTY;
     }
     __device__ void insert(int k, uint32_t s, uint32_t g) const {
-        uint32_t *e = tab[k & 1];
+        uint32_t *e = tb(k);
         for (uint32_t j = 0, h = s & (H - 1); j < H; j++, h = (h + 1) & (H - 1)) {  // H > distinct slots
             uint32_t x = __hip_atomic_load(&e[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (x == TAB_EMPTY) x = atomicCAS(&e[h], TAB_EMPTY, s);
@@ -114,6 +140,7 @@ struct GraphSlots {
 template <class Slots>
 struct GlobalSpaceT : Slots {
     const TailArgs *t;
+    const TailRun *r;
     const int32_t *lst;
     uint32_t lc;
     int32_t *out;
@@ -127,15 +154,15 @@ struct GlobalSpaceT : Slots {
     __device__ uint32_t orw(int32_t v, uint32_t b) const { return atomicOr(&t->word[v], b); }
     __device__ int32_t gid(int32_t v) const { return v; }
     __device__ int32_t *qptr(int b) const {  // select, not an indexed load of the pointer
-        int32_t *q0 = t->q[0], *q1 = t->q[1], *q2 = t->q[2], *q3 = t->q[3];
+        int32_t *q0 = r->q0, *q1 = r->q1, *q2 = r->q2, *q3 = r->q3;
         return b == 0 ? q0 : b == 1 ? q1 : b == 2 ? q2 : q3;
     }
     __device__ int32_t qget(int b, uint32_t k) const { return qptr(b)[k]; }
     __device__ void qput(int b, uint32_t k, int32_t v) const { qptr(b)[k] = v; }
     __device__ void start(uint32_t np, uint32_t tid) const {
-        for (uint32_t i = tid; i < np; i += TAIL_BLOCK) t->q[0][i] = t->q[1][i] = t->piv[i];
+        for (uint32_t i = tid; i < np; i += TAIL_BLOCK) r->q0[i] = r->q1[i] = r->piv[i];
     }
-    __device__ void pivot(uint32_t k, int32_t v) const { t->piv[k] = v; }
+    __device__ void pivot(uint32_t k, int32_t v) const { r->piv[k] = v; }
     template <class F>
     __device__ bool each_nb(int d, int32_t v, F f) const {  // true once f is
         const int32_t *row = d == 0 ? t->row : t->rowT, *col = d == 0 ? t->col : t->colT;
@@ -227,15 +254,6 @@ struct LocalSpaceT : Slots {
     }
 };
 
-struct TailRun {
-    int lsel, pt;
-    uint32_t epoch, npiv;
-    unsigned long long depths, trims;
-    uint32_t lcnt[2];  // a batch graph's live-list lengths (the tail keeps them in ctl->lcnt)
-    uint32_t first;    // a batch graph's lowest pivot of its last election
-    int keep_colour;   // the step from no pivots: its election keeps the colours (the first
-                       // election, scc.cu:235-245, elects before any split)
-};
 
 // End-of-step bookkeeping.  The tail of a run publishes each step in the control
 // block (the host goes on from it); a graph of a batch keeps its list lengths
@@ -258,7 +276,7 @@ struct TailBook {
 struct BatchBook {
     __device__ void end(const TailArgs &t, TailShared &sh, TailRun &r, uint32_t ln) const {
         if (threadIdx.x == 0 && sh.red[1] + r.npiv) atomicAdd(&t.ctl->pcount, (uint32_t)sh.red[1] + r.npiv);
-        r.lcnt[r.lsel ^ 1] = ln;
+        r.set_lcnt(r.lsel ^ 1, ln);
         r.first = sh.first;
     }
 };
@@ -279,7 +297,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
     for (uint32_t level = 0;; level++) {
         const uint32_t n0 = sh.cnt[0], n1 = sh.cnt[1];
         if (n0 + n1 == 0) break;                     // block-uniform (read after the sync)
-        if (level > 2u * t.max_steps) return false;  // every level queues a new vertex
+        if (level > 2u * r.max_steps) return false;  // every level queues a new vertex
         r.depths++;
         for (uint32_t i = tid; i < n0 + n1; i += nt) {
             const int d = i < n0 ? 0 : 1;
@@ -336,7 +354,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         r.trims++;
         if (now == seen) break;  // block-uniform
         seen = now;
-        if (it > t.max_steps) return false;  // every round but the last trims a vertex
+        if (it > r.max_steps) return false;  // every round but the last trims a vertex
     }
     // ---- update: eliminate F and B into the pivot's SCC, split the rest, elect
     r.epoch++;
@@ -401,17 +419,17 @@ __device__ inline bool tail_arc(const TailArgs &t, int32_t v, int32_t u, uint32_
 // of one colour, into LDS.  1: copied; 0: the arcs do not fit; -1: a live
 // vertex outside the list, which the elections never leave.
 template <class LS>
-__device__ int tail_build(const TailArgs &t, TailShared &sh, LS &ls, uint32_t L, const int32_t *lst,
-                          uint32_t npiv, int pt) {
+__device__ int tail_build(const TailArgs &t, const TailRun &r, TailShared &sh, LS &ls, uint32_t L,
+                          const int32_t *lst, uint32_t npiv, int pt) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
     ls.L = L;
     ls.to = 0;
     ls.H = 0;
     uint32_t *lw = ls.w(), *off0 = ls.off(0), *off1 = ls.off(1);
     int32_t *gidv = ls.gidv();
-    // global id -> local index of the live vertices, indexed from t.vbase (the
+    // global id -> local index of the live vertices, indexed from r.vbase (the
     // first vertex of a batch graph, whose queue slice this is)
-    uint32_t *map = (uint32_t *)t.q[3] - t.vbase;
+    uint32_t *map = (uint32_t *)r.q3 - r.vbase;
     for (uint32_t i = tid; i < L; i += nt) {
         const int32_t v = lst[i];
         gidv[i] = v;
@@ -484,9 +502,9 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LS &ls, uint32_t L,
         off1[L] = ti;
     }
     for (uint32_t k = tid; k < npiv; k += nt) {
-        const uint32_t j = map[t.piv[k]];
+        const uint32_t j = map[r.piv[k]];
         q0[k] = q1[k] = (uint16_t)j;
-        if (j >= L || gidv[j] != t.piv[k]) sh.flag = 1;
+        if (j >= L || gidv[j] != r.piv[k]) sh.flag = 1;
     }
     if (ls.H) ls.clear_cur(t, pt ^ 1, tid);  // the LDS table of the last election: its winners are the pivots
     tail_sync();
@@ -503,12 +521,12 @@ __device__ int tail_build(const TailArgs &t, TailShared &sh, LS &ls, uint32_t L,
 // winners into its global table (the host's next election reads them there).
 using LocalSpace = LocalSpaceT<GlobalSlots>;
 
-__device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShared &sh, int lsel, uint32_t npiv, int pt,
-                               uint32_t epoch) {
+__device__ void tail_writeback(const TailArgs &t, const TailRun &r, const LocalSpace &ls, TailShared &sh, int lsel,
+                               uint32_t npiv, int pt, uint32_t epoch) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
     if (tid == 0) sh.lcnt = 0;
     tail_sync();
-    int32_t *out = t.list[lsel];
+    int32_t *out = r.list(lsel);
     const uint32_t *lw = ls.w();
     const int32_t *gidv = ls.gidv();
     const uint16_t *q0 = ls.q(0);
@@ -520,7 +538,7 @@ __device__ void tail_writeback(const TailArgs &t, const LocalSpace &ls, TailShar
     }
     for (uint32_t k = tid; k < npiv; k += nt) {
         const int32_t g = gidv[q0[k]];
-        t.piv[k] = g;
+        r.piv[k] = g;
         if (ls.H)
             t.slots[pt][slot_of(colour_of(lw[q0[k]]), t.slot_bits)] = elect_tag(epoch) | (uint32_t)orig_id(t.oid, g);
     }
@@ -541,8 +559,20 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         }
         return;
     }
-    if (t.max_steps == 0) t.max_steps = live0 + 2;  // each step eliminates its pivots
-    TailRun r{t.lsel, t.prev_tab, t.epoch, ctl->npiv, 0, 0};
+    TailRun r{};
+    r.lsel = t.lsel;
+    r.pt = t.prev_tab;
+    r.epoch = t.epoch;
+    r.npiv = ctl->npiv;
+    r.q0 = t.q[0];
+    r.q1 = t.q[1];
+    r.q2 = t.q[2];
+    r.q3 = t.q[3];
+    r.l0 = t.list[0];
+    r.l1 = t.list[1];
+    r.piv = t.piv;
+    r.vbase = t.vbase;
+    r.max_steps = t.max_steps ? t.max_steps : live0 + 2;  // each step eliminates its pivots
     uint32_t steps = 0, status = 0, try_below = 0xFFFFFFFFu;
     unsigned long long pivots = 0;
     bool local = false;
@@ -551,7 +581,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
     // or B) and an election whose colour split 3c + 0 keeps one colour per graph
     bool first = t.first_step != 0;
     while (r.npiv > 0 || first) {
-        if (steps >= t.max_steps) {
+        if (steps >= r.max_steps) {
             status = 1;
             break;
         }
@@ -562,7 +592,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
             }
             // clear the table this step's election writes: no key of an election 2^16
             // earlier may outrank the new tags
-            unsigned long long *cur = t.slots[r.pt ^ 1];
+            unsigned long long *cur = r.pt ? t.slots[0] : t.slots[1];
             for (uint32_t i = threadIdx.x; i < (1u << t.slot_bits); i += TAIL_BLOCK) cur[i] = ~0ull;
             tail_sync();
         }
@@ -571,7 +601,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
             const uint32_t L = ctl->lcnt[r.lsel];
             if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {
-                const int b = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv, r.pt);
+                const int b = tail_build(t, r, sh, ls, L, r.list(r.lsel), r.npiv, r.pt);
                 if (b < 0) {
                     status = 2;
                     break;
@@ -586,7 +616,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         if (local) {
             ok = tail_step(t, ls, sh, r);
         } else {
-            const GlobalSpace gs{{}, &t, t.list[r.lsel], ctl->lcnt[r.lsel], t.list[r.lsel ^ 1]};
+            const GlobalSpace gs{{}, &t, &r, r.list(r.lsel), ctl->lcnt[r.lsel], r.list(r.lsel ^ 1)};
             ok = tail_step(t, gs, sh, r);
         }
         if (!ok) {
@@ -594,7 +624,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
             break;
         }
     }
-    if (local) tail_writeback(t, ls, sh, r.lsel, r.npiv, r.pt, r.epoch);
+    if (local) tail_writeback(t, r, ls, sh, r.lsel, r.npiv, r.pt, r.epoch);
     if (threadIdx.x == 0) {
         ctl->tail_live0 = live0;
         ctl->tail[0] = steps;
@@ -644,19 +674,24 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_batch(TailArgs base, BatchArg
         const int32_t g = b.order[k];
         const int32_t o0 = b.off[g], ng = b.off[g + 1] - o0;
         if (ng <= 0) continue;
-        TailArgs t = base;
-        t.vbase = o0;
-        for (int i = 0; i < 4; i++) t.q[i] = base.q[i] + o0;
-        t.list[0] = base.list[0] + o0;
-        t.list[1] = base.list[1] + o0;
-        t.piv = base.piv + o0;
-        t.max_steps = (uint32_t)ng + 2u;
+        const TailArgs &t = base;
+        TailRun r{};
+        r.q0 = base.q[0] + o0;
+        r.q1 = base.q[1] + o0;
+        r.q2 = base.q[2] + o0;
+        r.q3 = base.q[3] + o0;
+        r.l0 = base.list[0] + o0;
+        r.l1 = base.list[1] + o0;
+        r.piv = base.piv + o0;
+        r.vbase = o0;
+        r.max_steps = (uint32_t)ng + 2u;
+        r.lc0 = (uint32_t)ng;
         const uint32_t colour = (uint32_t)(g + 1) & COLOUR_MASK;
         for (int32_t i = (int32_t)tid; i < ng; i += TAIL_BLOCK) {
             const int32_t v = o0 + i;
             t.word[v] = make_word(colour, 0);
             t.root[v] = orig_id(t.oid, v);
-            t.list[0][i] = v;
+            r.l0[i] = v;
         }
         GraphSlots gs;
         gs.H = batch_table_size(ng);
@@ -664,20 +699,18 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_batch(TailArgs base, BatchArg
         gs.tab[1] = gs.tab[0] + 2u * gs.H;
         LocalSpaceT<GraphSlots> ls{};
         static_cast<GraphSlots &>(ls) = gs;
-        TailRun r{};
-        r.lcnt[0] = (uint32_t)ng;
         bool local = false, first = true;
         uint32_t steps = 0, try_below = 0xFFFFFFFFu;
         tail_sync();
         while (r.npiv > 0 || first) {  // block-uniform
-            if (steps >= t.max_steps) {
+            if (steps >= r.max_steps) {
                 status = 2;
                 break;
             }
             if (!local) {  // the LDS copy, once the live list is short enough and its arcs fit
-                const uint32_t L = r.lcnt[r.lsel];
+                const uint32_t L = r.lcnt(r.lsel);
                 if (L < try_below && L < 65536u && tail_fixed_bytes(L) <= TAIL_ARENA) {
-                    const int bld = tail_build(t, sh, ls, L, t.list[r.lsel], r.npiv, r.pt);
+                    const int bld = tail_build(t, r, sh, ls, L, r.list(r.lsel), r.npiv, r.pt);
                     if (bld < 0) {
                         status = 2;
                         break;
@@ -694,7 +727,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_batch(TailArgs base, BatchArg
             if (local) {
                 ok = tail_step(t, ls, sh, r, BatchBook());
             } else {
-                const GlobalSpaceT<GraphSlots> sp{gs, &t, t.list[r.lsel], r.lcnt[r.lsel], t.list[r.lsel ^ 1]};
+                const GlobalSpaceT<GraphSlots> sp{gs, &t, &r, r.list(r.lsel), r.lcnt(r.lsel), r.list(r.lsel ^ 1)};
                 ok = tail_step(t, sp, sh, r, BatchBook());
             }
             if (!ok) {
