@@ -30,7 +30,7 @@ class SccOpts(ctypes.Structure):
                 ("hybrid_arc_div", ctypes.c_int32), ("reach_graphs", ctypes.c_int32),
                 ("reach_block", ctypes.c_int32),
                 ("tail_live", ctypes.c_int32), ("batch_solver", ctypes.c_int32),
-                ("relabel", ctypes.c_int32)]
+                ("relabel", ctypes.c_int32), ("tile_reach", ctypes.c_int32)]
 
 
 _STAT_INTS = ["steps", "fwd_depths", "bwd_depths", "trim_iters", "num_scc", "pivots_total",

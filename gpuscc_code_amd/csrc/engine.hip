@@ -37,6 +37,17 @@ struct scc_ctx {
     // renumbering (opts.relabel): d_oid[v] = the caller's id of internal vertex v, d_nid the
     // inverse; both null when the upload kept the caller's numbering
     int32_t *d_oid = nullptr, *d_nid = nullptr;
+    // tiles (opts.tile_reach; regular graphs): the upload numbered the vertices tile by tile;
+    // the first FB step's reach runs over them (kernels_tile.hip)
+    int32_t ntiles = 0;
+    int32_t *d_toff = nullptr, *d_tile_of = nullptr, *d_lpos = nullptr, *d_inbox = nullptr;
+    int32_t *d_tlist = nullptr;
+    uint16_t *d_larc = nullptr;
+    uint8_t *d_xflag = nullptr;
+    uint32_t *d_incnt = nullptr;
+    unsigned long long *d_tstamp = nullptr;
+    int64_t tile_arcs0 = 0;  // intra-tile arcs of direction 0 (the in-arcs follow in d_larc)
+    unsigned long long tile_run = 0;
     bool track_mn = false;    // this run's updates keep d_mn
     int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
     int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
@@ -353,6 +364,8 @@ struct ReachChain {
     const int32_t *bseed;
     Launch l;
     ReachArgs a{};
+    bool tiles = false;  // the first FB step of a tiled single graph: the tile reach (kernels_tile.hip)
+    TileArgs ta{};
     ReachChain(scc_ctx *cc, uint32_t dirs, int sch, const int32_t *bs)
         : c(cc), sched(sch), bseed(bs), l(sch == SCC_SCHED_Q ? launch_for(cc) : fixed_launch(cc)) {
         a.row = c->g.row;
@@ -396,6 +409,33 @@ struct ReachChain {
             a.vis[0] = c->d_vis;
             a.vis[1] = c->d_vis + c->vis_words;
         }
+        tiles = c->ntiles > 0 && sched == SCC_SCHED_H && dirs == 3 && c->one_colour && c->num_graphs == 0 &&
+                !c->best_step && c->opts.hybrid_policy == 0;
+        if (tiles) {
+            const int32_t n = c->g.n;
+            ta.row = c->g.row;
+            ta.col = c->g.col;
+            ta.rowT = c->g.rowT;
+            ta.colT = c->g.colT;
+            ta.word = c->d_word;
+            ta.n = n;
+            ta.ntiles = c->ntiles;
+            ta.toff = c->d_toff;
+            ta.tile_of = c->d_tile_of;
+            ta.lpos[0] = c->d_lpos;
+            ta.lpos[1] = c->d_lpos + n + 1;
+            ta.larc[0] = c->d_larc;
+            ta.larc[1] = c->d_larc + c->tile_arcs0;
+            ta.xflag = c->d_xflag;
+            ta.inbox = c->d_inbox;
+            ta.incnt = c->d_incnt;
+            ta.stamp = c->d_tstamp;
+            ta.list[0] = c->d_tlist;
+            ta.list[1] = c->d_tlist + c->ntiles;
+            ta.piv = c->d_piv;
+            ta.ctl = c->d_ctl;
+            ta.run = ++c->tile_run;
+        }
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
     void operator()(int it, hipEvent_t e0, hipEvent_t e1) {
@@ -409,6 +449,15 @@ struct ReachChain {
         Launch lt = l;
         lt.ev_start = e0;
         lt.ev_stop = e1;
+        if (tiles) {
+            if (it == 0)  // inbox counts start at zero (each run of a tile clears what it took)
+                launch_fill_i32(launch_for(c), (int32_t *)c->d_incnt, 4 * (int64_t)c->ntiles, 0);
+            ta.iter = it;
+            if (e0) (void)hipEventRecord(e0, c->stream);
+            launch_tile_reach(c->stream, (uint32_t)c->cu_count, ta);
+            if (e1) (void)hipEventRecord(e1, c->stream);
+            return;
+        }
         launch_reach(lt, sched, a);
     }
     void account(int64_t depth) {  // profile counters of a finished chain
@@ -623,7 +672,21 @@ void free_batch(scc_ctx *c) {
     c->d_btab = nullptr;
 }
 
+void free_tiles(scc_ctx *c) {
+    for (void *p : {(void *)c->d_toff, (void *)c->d_tile_of, (void *)c->d_lpos, (void *)c->d_inbox,
+                    (void *)c->d_tlist, (void *)c->d_larc, (void *)c->d_xflag, (void *)c->d_incnt,
+                    (void *)c->d_tstamp})
+        if (p) (void)hipFree(p);
+    c->d_toff = c->d_tile_of = c->d_lpos = c->d_inbox = c->d_tlist = nullptr;
+    c->d_larc = nullptr;
+    c->d_xflag = nullptr;
+    c->d_incnt = nullptr;
+    c->d_tstamp = nullptr;
+    c->ntiles = 0;
+}
+
 void free_graph(scc_ctx *c) {
+    free_tiles(c);
     clear_graphs(c);  // captured launches point at the buffers freed here
     int32_t **bufs[] = {&c->d_row, &c->d_col, &c->d_rowT, &c->d_colT, &c->d_root, &c->d_label,
                         &c->d_tmp, &c->d_mn, &c->d_q[0], &c->d_q[1], &c->d_q[2], &c->d_q[3],
@@ -709,6 +772,108 @@ void row_indexes(scc_ctx *c) {
     // the scorer's static features (degrees, 2-hop degree sums): a property of the graph
     launch_static_features(l, c->g, c->d_stat);
     c->stat_valid = true;
+}
+
+// Tiles for the tile reach: regions of up to T vertices grown by BFS over the
+// arcs taken both ways, each from the lowest vertex not yet taken and, when the
+// region runs out of unvisited neighbours before T, from the next lowest, so
+// every tile but the last holds T vertices.  Returns the caller's ids in tile
+// order (a region's vertices in BFS order) and the tile offsets.
+std::vector<int32_t> tile_partition(int32_t n, const int32_t *row, const int32_t *col, const int32_t *rowT,
+                                    const int32_t *colT, int32_t T, std::vector<int32_t> &toff) {
+    std::vector<int32_t> order;
+    order.reserve((size_t)n);
+    std::vector<uint8_t> seen((size_t)n, 0);
+    toff.assign(1, 0);
+    int32_t seed = 0;
+    while ((int32_t)order.size() < n) {
+        const size_t start = order.size();
+        size_t head = start;
+        while ((int64_t)(order.size() - start) < T && (int32_t)order.size() < n) {
+            if (head == order.size()) {  // the region is closed: go on from the next lowest vertex
+                while (seen[seed]) seed++;
+                seen[seed] = 1;
+                order.push_back(seed);
+            }
+            const int32_t v = order[head++];
+            for (int d = 0; d < 2; d++) {
+                const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
+                for (int32_t e = r[v]; e < r[v + 1] && (int64_t)(order.size() - start) < T; e++) {
+                    const int32_t u = cl[e];
+                    if (seen[u]) continue;
+                    seen[u] = 1;
+                    order.push_back(u);
+                }
+            }
+        }
+        toff.push_back((int32_t)order.size());
+    }
+    return order;
+}
+
+// The tile structures of a graph renumbered by `order` (tiles toff): the arcs of
+// each vertex inside its tile as 16-bit indices from the tile's first vertex,
+// per direction, and a flag per vertex and direction for arcs that leave the
+// tile.  row .. colT: the CSR pair in the caller's ids.  False when a tile does
+// not fit a workgroup's LDS (the graph then runs without tiles).
+scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const int32_t *rowT,
+                       const int32_t *colT, const std::vector<int32_t> &order, const std::vector<int32_t> &toff,
+                       bool *ok) {
+    *ok = false;
+    const int32_t n = c->g.n, nt = (int32_t)toff.size() - 1;
+    std::vector<int32_t> nid((size_t)n), tof((size_t)n), lpos(2 * ((size_t)n + 1));
+    for (int32_t i = 0; i < n; i++) nid[order[i]] = i;
+    for (int32_t t = 0; t < nt; t++)
+        for (int32_t i = toff[t]; i < toff[t + 1]; i++) tof[i] = t;
+    std::vector<uint16_t> arc[2];
+    std::vector<uint8_t> xf((size_t)n, 0);
+    for (int d = 0; d < 2; d++) {
+        const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
+        int32_t *lp = lpos.data() + (size_t)d * ((size_t)n + 1);
+        for (int32_t i = 0; i < n; i++) {
+            const int32_t o = order[i], t = tof[i];
+            lp[i] = (int32_t)arc[d].size();
+            for (int32_t e = r[o]; e < r[o + 1]; e++) {
+                const int32_t u = nid[cl[e]];
+                if (tof[u] == t) arc[d].push_back((uint16_t)(u - toff[t]));
+                else xf[i] |= (uint8_t)(1u << d);
+            }
+        }
+        lp[n] = (int32_t)arc[d].size();
+    }
+    for (int32_t t = 0; t < nt; t++) {
+        const int32_t lo = toff[t], hi = toff[t + 1];
+        const int64_t A = (int64_t)(lpos[hi] - lpos[lo]) + (lpos[(size_t)n + 1 + hi] - lpos[(size_t)n + 1 + lo]);
+        if (hi - lo > (int32_t)TILE_MAX || A > 65535 || tile_lds_bytes((uint32_t)(hi - lo), (uint32_t)A) > TILE_ARENA)
+            return SCC_OK;  // a tile too dense for LDS
+    }
+    free_tiles(c);
+    const size_t vb = sizeof(int32_t) * (size_t)n;
+    const size_t A0 = arc[0].size(), A1 = arc[1].size();
+    HIPCHK(c, dev_alloc(c, &c->d_toff, sizeof(int32_t) * toff.size()));
+    HIPCHK(c, dev_alloc(c, &c->d_tile_of, vb));
+    HIPCHK(c, dev_alloc(c, &c->d_lpos, sizeof(int32_t) * lpos.size()));
+    HIPCHK(c, dev_alloc(c, &c->d_larc, sizeof(uint16_t) * std::max<size_t>(A0 + A1, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_xflag, (size_t)n));
+    HIPCHK(c, dev_alloc(c, &c->d_inbox, 4 * vb));
+    HIPCHK(c, dev_alloc(c, &c->d_incnt, sizeof(uint32_t) * 4 * (size_t)nt));
+    HIPCHK(c, dev_alloc(c, &c->d_tstamp, sizeof(unsigned long long) * (size_t)nt));
+    HIPCHK(c, dev_alloc(c, &c->d_tlist, sizeof(int32_t) * 2 * (size_t)nt));
+    HIPCHK(c, hipMemcpy(c->d_toff, toff.data(), sizeof(int32_t) * toff.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tile_of, tof.data(), vb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_lpos, lpos.data(), sizeof(int32_t) * lpos.size(), hipMemcpyHostToDevice));
+    if (A0) HIPCHK(c, hipMemcpy(c->d_larc, arc[0].data(), sizeof(uint16_t) * A0, hipMemcpyHostToDevice));
+    if (A1) HIPCHK(c, hipMemcpy(c->d_larc + A0, arc[1].data(), sizeof(uint16_t) * A1, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_xflag, xf.data(), (size_t)n, hipMemcpyHostToDevice));
+    const Launch l = launch_for(c);
+    launch_fill_i32(l, (int32_t *)c->d_incnt, 4 * (int64_t)nt, 0);
+    launch_fill_u64(l, c->d_tstamp, nt, 0ull);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ntiles = nt;
+    c->tile_arcs0 = (int64_t)A0;
+    *ok = true;
+    return SCC_OK;
 }
 
 // Renumbers the device CSR so that internal vertex i is the caller's vertex
@@ -851,7 +1016,7 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
     if ((c->opts.block_size != 0 && (c->opts.block_size < 64 || c->opts.block_size > 1024 ||
                                      (c->opts.block_size & 63) != 0)) ||
         (c->opts.reach_block != 0 && c->opts.reach_block != 256 && c->opts.reach_block != 512) ||
-        c->opts.tail_live < -1 || c->opts.relabel < -1 || c->opts.relabel > 1) {
+        c->opts.tail_live < -1 || c->opts.relabel < -1 || c->opts.relabel > 1 || c->opts.tile_reach < -1) {
         delete c;
         return SCC_E_INPUT;
     }
@@ -894,6 +1059,7 @@ scc_status scc_set_opts(scc_ctx *c, const scc_opts *opts) {
     if (opts->tail_live < -1) return fail(c, SCC_E_INPUT, "tail_live must be -1 (off), 0 (default) or a count");
     if (opts->grid_blocks < 0) return fail(c, SCC_E_INPUT, "grid_blocks < 0");
     if (opts->relabel < -1 || opts->relabel > 1) return fail(c, SCC_E_INPUT, "relabel must be -1, 0 or 1");
+    if (opts->tile_reach < -1) return fail(c, SCC_E_INPUT, "tile_reach must be -1, 0, 1 or a tile size");
     if (c->uploaded && opts->slot_bits != 0 && opts->slot_bits != c->slot_bits)
         return fail(c, SCC_E_STATE, "slot_bits is fixed once a graph is uploaded");
     c->opts = *opts;
@@ -1027,8 +1193,19 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
         dmax = std::max<int64_t>(dmax, deg[v]);
     }
     const bool skewed = n >= 65536 && dmax * (int64_t)n > 64 * 2 * m;  // max above 64 x the mean (2m / n)
-    if (n > 0 && (c->opts.relabel > 0 || (c->opts.relabel == 0 && skewed)))
+    // tiles for the tile reach: when asked for (the level kernels are faster by default)
+    const bool tiles = n > 0 && c->opts.tile_reach > 0;
+    if (tiles) {
+        const int32_t T = c->opts.tile_reach > 1 ? std::min<int32_t>(c->opts.tile_reach, (int32_t)TILE_MAX)
+                                                 : (int32_t)TILE_MAX;
+        std::vector<int32_t> toff;
+        const std::vector<int32_t> order = tile_partition(n, row, col, rowT, colT, T, toff);
+        CHK(permute_graph(c, row, rowT, order, {}));
+        bool ok = false;
+        CHK(build_tiles(c, row, col, rowT, colT, order, toff, &ok));
+    } else if (n > 0 && (c->opts.relabel > 0 || (c->opts.relabel == 0 && skewed))) {
         CHK(permute_graph(c, row, rowT, degree_order(n, deg, nullptr, 0), {}));
+    }
     row_indexes(c);
     c->epoch = 0;
     c->prev_tab = 0;
@@ -1055,7 +1232,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
     scc_stats st;
     std::memset(&st, 0, sizeof(st));
     st.first_pivot = -1;
-    st.renumbered = c->d_oid ? 1 : 0;
+    st.renumbered = c->ntiles > 0 ? 2 : c->d_oid ? 1 : 0;
     const int32_t n = c->g.n;
     c->syncs = 0;
     c->cur_step = 0;
@@ -1463,6 +1640,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return SCC_OK;
     };
     s = body();
+    if (s == SCC_OK && c->ntiles > 0 && c->h_ctl->terr != 0)  // never: the upload checked every tile
+        s = fail(c, SCC_E_STATE, "a tile did not fit its workgroup's LDS");
     if (s == SCC_OK && c->d_oid && n > 0) {  // renumbered: the labels into the caller's order
         launch_gather_i32(launch_for(c), c->d_root, c->d_nid, c->d_label, n);
         c->label_in_root = false;
@@ -1533,6 +1712,7 @@ scc_status scc_set_batch(scc_ctx *c, const int32_t *off, int32_t G) {
     if (off[0] != 0 || off[G] != c->g.n) return fail(c, SCC_E_INPUT, "offsets must run from 0 to n");
     for (int32_t i = 0; i < G; i++)
         if (off[i + 1] < off[i]) return fail(c, SCC_E_INPUT, "offsets decrease");
+    free_tiles(c);  // the tile reach serves single graphs
     if (c->d_oid && c->g.n > 0) {  // renumbered: again, by degree within each graph, so graphs keep their ranges
         const int32_t n = c->g.n;
         std::vector<int32_t> row((size_t)n + 1), rowT((size_t)n + 1), oid((size_t)n), deg((size_t)n);

@@ -76,6 +76,9 @@ struct Ctl {
     unsigned long long tail_pivots;  // k_fb_tail: pivots elected
     uint32_t tail_live0;             // k_fb_tail: live vertices when it started
     uint32_t wnext;                  // k_fb_batch: the next graph to take (work counter)
+    uint32_t tcnt[4];                // tile reach: lengths of the tile lists (iteration i reads [i & 3])
+    uint32_t terr;                   // tile reach: a tile that did not fit its workgroup's LDS (never, by
+                                     // the upload's check)
 };
 
 // Persistent-grid helpers.  Workgroups are dealt to the 8 XCDs round-robin;

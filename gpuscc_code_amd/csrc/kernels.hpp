@@ -204,6 +204,33 @@ struct BatchArgs {
 uint32_t batch_table_words(int32_t n_g);  // 4 * batch_table_size(n_g)
 void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const BatchArgs &b);
 
+// kernels_tile.hip -- the reach of a first FB step over tiles (regular graphs renumbered tile by
+// tile at upload).  Iteration 0 puts the pivots (seed list, ctl->qsize[0][0]) into their tiles'
+// inboxes; iteration i >= 1 runs every tile of list i in LDS and claims across tile boundaries.
+constexpr uint32_t TILE_MAX = 4096;           // vertices per tile
+constexpr uint32_t TILE_ARENA = 150u * 1024u;  // LDS per workgroup
+// LDS bytes of a tile of T vertices and A arcs inside it (both directions)
+__host__ __device__ inline uint32_t tile_lds_bytes(uint32_t T, uint32_t A) { return 13u * T + 4u + 2u * A + 2u; }
+struct TileArgs {
+    const int32_t *row, *col, *rowT, *colT;
+    uint32_t *word;
+    int32_t n, ntiles;
+    const int32_t *toff;      // tile t = internal vertices [toff[t], toff[t + 1])
+    const int32_t *tile_of;   // tile of each vertex
+    const int32_t *lpos[2];   // arcs of v inside its tile: larc[d][lpos[d][v] .. lpos[d][v + 1]) (d 0 out, 1 in)
+    const uint16_t *larc[2];  // as indices from the tile's first vertex
+    const uint8_t *xflag;     // bit d: v has an arc of direction d that leaves its tile
+    int32_t *inbox;           // [parity][direction][n]: claimed into tile t at toff[t] + k
+    uint32_t *incnt;          // [parity][direction][ntiles]
+    unsigned long long *stamp;  // [ntiles]: (run << 32) | (iteration + 2) of the last list the tile joined
+    int32_t *list[2];         // tile lists, by iteration parity
+    const int32_t *piv;       // iteration 0: the pivot list
+    Ctl *ctl;
+    int iter;
+    unsigned long long run;   // distinct per reach (stamps are never cleared)
+};
+void launch_tile_reach(hipStream_t s, uint32_t grid, const TileArgs &a);
+
 // kernels_mlp.hip -- learned pivot scorer
 void launch_static_features(const Launch &l, const Graph &g, float4 *out);
 // live vertices per colour slot over the list lin[0 .. *lin_cnt) (lin null: all n)

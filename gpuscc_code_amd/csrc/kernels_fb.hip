@@ -234,6 +234,7 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
         ctl->live = 0;
         ctl->pcount = 0;
         ctl->fbelim = 0;
+        ctl->terr = 0;
     }
     if (what & 64) {  // the pivot list seeds both directions
         ctl->seeds[0] = ctl->npiv;
@@ -245,6 +246,7 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
     }
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
+        for (int i = 0; i < 4; i++) ctl->tcnt[i] = 0;
         for (int d = 0; d < 2; d++)
             for (int i = 0; i < 4; i++) ctl->hsize[d][i] = 0;
         ctl->ran = 0;

@@ -48,6 +48,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
     tail_live: int = 0
     batch_solver: int = 0
     relabel: int = None  # None: DEFAULT_RELABEL
+    tile_reach: int = 0
 
     def to_c(self):
         o = SccOpts()
@@ -65,6 +66,7 @@ class alg_opt:  # noqa: N801 -- the reference's name (scc.h:19-27)
         o.tail_live = self.tail_live
         o.batch_solver = self.batch_solver
         o.relabel = DEFAULT_RELABEL if self.relabel is None else self.relabel
+        o.tile_reach = self.tile_reach
         return o
 
 

@@ -102,6 +102,12 @@ typedef struct {
                                   Results are unchanged: elections, roots and labels use the caller's
                                   ids.  0 (default): when the degrees are skewed (largest above 64
                                   times the mean) and n >= 65536; 1: always; -1: never */
+    int32_t tile_reach;        /* read by scc_upload: number the vertices tile by tile (regions of up to
+                                  4,096 vertices grown by BFS) so that the reach of the first FB step
+                                  (schedule h, hybrid_policy 0, single graph) runs inside tiles, in LDS,
+                                  with claims across tile boundaries between iterations.  Results are
+                                  unchanged.  0 (default) and -1: never; 1: tiles of up to 4,096
+                                  vertices; N > 1: tiles of at most N vertices */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -120,7 +126,8 @@ typedef struct {
     int64_t first_pivot;    /* first pivot (lowest live id after trim-1, scc.cu:235-245), -1 if none */
     int64_t tail_steps;     /* of steps, those run in the one-workgroup tail (scc_opts.tail_live) */
     int64_t tail_live;      /* live vertices when the tail started, 0 if it did not run */
-    int64_t renumbered;     /* 1: the upload renumbered the vertices by degree (scc_opts.relabel) */
+    int64_t renumbered;     /* 1: the upload renumbered the vertices by degree (scc_opts.relabel),
+                               2: tile by tile (scc_opts.tile_reach); 0: neither */
     double ms_total;        /* upload-excluded wall time, labels ready (scc.cu:217->1011) */
     double ms_fwd, ms_bwd, ms_trim, ms_update; /* per phase, host clock (STOPWATCH, timing.h:14) */
 } scc_stats;
