@@ -166,6 +166,9 @@ def main():
                     help="hybrid: frontier size at which a direction pulls (0 = n/20)")
     ap.add_argument("--relabel", type=int, default=0, choices=[-1, 0, 1],
                     help="renumbering at upload: 0 by degree skew (library default), 1 always, -1 never")
+    ap.add_argument("--tile-reach", type=int, default=0,
+                    help="tile reach of the first FB step (scc_opts.tile_reach): 0 off (library default), "
+                         "1 tiles of 4096 vertices, T > 1 tiles of T vertices")
     ap.add_argument("--no-tuned", action="store_true",
                     help="library defaults instead of the autotuned options (gpuscc_code_amd/tuned.json)")
     ap.add_argument("--no-learned", action="store_true",
@@ -200,7 +203,8 @@ def main():
     ctx = scc.Context(0, scc.alg_opt(sequential_reach=1 if args.sequential else 0,
                                      hybrid_threshold=args.pull_threshold,
                                      hybrid_arc_div=args.pull_arc_div,
-                                     grid_blocks=args.grid_blocks, relabel=args.relabel))
+                                     grid_blocks=args.grid_blocks, relabel=args.relabel,
+                                     tile_reach=args.tile_reach))
     t0 = time.time()
     ctx.upload(g)
     log(f"uploaded in {time.time() - t0:.1f}s")
@@ -214,7 +218,8 @@ def main():
     if not args.no_tuned and not manual:
         arch, _ = ctx.device_info()
         opt, tuned_key = autotune.opts_for(g, arch, args.schedule, args.variant,
-                                           batch=args.workload == "batch")
+                                           batch=args.workload == "batch",
+                                           base=scc.alg_opt(relabel=args.relabel, tile_reach=args.tile_reach))
         if tuned_key:
             ctx.set_opts(opt)
             opts = autotune.load_table()[arch][tuned_key]["opts"]
@@ -349,7 +354,8 @@ def main():
         "events_per_kernel": {"launches_per_step": launches_ev, "kernel_ms_per_step": round(kernel_ms_ev, 4),
                               "avg_launch_us": round(kernel_ms_ev * 1e3 / max(launches_ev, 1), 2)},
         "work_per_step": {k: work[k] for k in ("sweep_levels", "queue_levels", "expanded", "edges",
-                                               "visited")},
+                                               "visited", "tile_runs", "tile_rounds", "tile_phase_ms",
+                                               "tile_busy_ms", "tile_idle_polls")},
     }
     out = {
         "metric": METRIC, "value": round(mteps, 2), "unit": "MTEPS", "n_gpus": 1,
@@ -362,7 +368,8 @@ def main():
                    "n": g.n, "m": g.m, "schedule": args.schedule,
                    "variant": ["scc", "scc_w", "scc_2_w"][args.variant], "pivot_rule": "plain",
                    "reach": "sequential" if args.sequential else "fused",
-                   "tuned": tuned_key, "opts": opts, "renumbered": bool(last["renumbered"])},
+                   "tuned": tuned_key, "opts": opts, "renumbered": bool(last["renumbered"]),
+                   "tiles": last["renumbered"] == 2},
         "bit_exact": bit_exact, "num_scc": None if st_check is None else st_check["num_scc"],
         "biggest_scc": None if tst is None else tst["biggest"],
         "fb_steps": last["steps"], "fwd_depths": last["fwd_depths"], "bwd_depths": last["bwd_depths"],

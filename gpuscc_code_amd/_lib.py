@@ -51,13 +51,19 @@ _i32p = ctypes.POINTER(ctypes.c_int32)
 _PROF_INTS = ["reach_launches", "reach_levels", "sweep_levels", "expanded", "edges", "visited",
               "queue_levels", "trim_launches"]
 _PROF_DBLS = ["reach_kernel_ms", "trim_kernel_ms"]
+_PROF_TAIL = ["tile_runs", "tile_rounds"]
 
 
 class SccProfile(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_int64) for k in _PROF_INTS] + [(k, ctypes.c_double) for k in _PROF_DBLS]
+    _fields_ = ([(k, ctypes.c_int64) for k in _PROF_INTS] + [(k, ctypes.c_double) for k in _PROF_DBLS]
+                + [(k, ctypes.c_int64) for k in _PROF_TAIL] + [("tile_phase_ms", ctypes.c_double * 4),
+                                                               ("tile_busy_ms", ctypes.c_double),
+                                                               ("tile_idle_polls", ctypes.c_int64)])
 
     def as_dict(self):
-        return {k: getattr(self, k) for k in _PROF_INTS + _PROF_DBLS}
+        d = {k: getattr(self, k) for k in _PROF_INTS + _PROF_DBLS + _PROF_TAIL + ["tile_busy_ms", "tile_idle_polls"]}
+        d["tile_phase_ms"] = list(self.tile_phase_ms)
+        return d
 
 
 class SccHostGraph(ctypes.Structure):

@@ -40,13 +40,14 @@ struct scc_ctx {
     // tiles (opts.tile_reach; regular graphs): the upload numbered the vertices tile by tile;
     // the first FB step's reach runs over them (kernels_tile.hip)
     int32_t ntiles = 0;
-    int32_t *d_toff = nullptr, *d_tile_of = nullptr, *d_lpos = nullptr, *d_inbox = nullptr;
+    int32_t *d_toff = nullptr, *d_tile_of = nullptr, *d_inbox = nullptr;
     int32_t *d_tlist = nullptr;
-    uint16_t *d_larc = nullptr;
+    unsigned long long *d_tadj = nullptr;   // [2n]: neighbours in the vertex's cell (out, then in)
+    unsigned long long *d_txarc = nullptr;  // [2n]: up to four neighbours in other cells of its tile
+    unsigned long long *d_tct = nullptr;    // [2n]: cell closures, written by each tile reach
     uint8_t *d_xflag = nullptr;
     uint32_t *d_incnt = nullptr;
     unsigned long long *d_tstamp = nullptr;
-    int64_t tile_arcs0 = 0;  // intra-tile arcs of direction 0 (the in-arcs follow in d_larc)
     unsigned long long tile_run = 0;
     bool track_mn = false;    // this run's updates keep d_mn
     int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
@@ -281,12 +282,17 @@ scc_status run_chain(scc_ctx *c, F &&launch_level, int &guess, int reset_mask, i
             const int i = (int)((launched - 1) & 3);
             float t = 0;
             HIPCHK(c, hipEventElapsedTime(&t, c->evs[0], c->evs[1]));
+            // a tile-reach iteration logs its tiles run and the most rounds in place of the
+            // queues, and its longest tile run (microseconds) in place of the longest scan
+            const bool tl = kind == 0 && h.tlog[i][0] > 0;
             double row[SCC_LEVEL_FIELDS] = {(double)kind, (double)(launched - 1), (double)t,
-                                            (double)h.qsize[0][i], (double)h.qsize[1][i],
+                                            (double)(tl ? h.tlog[i][0] : h.qsize[0][i]),
+                                            (double)(tl ? h.tlog[i][1] : h.qsize[1][i]),
                                             (double)h.qarcs[0][i], (double)h.qarcs[1][i],
                                             (double)(kind == 0 ? h.pull_mask : 0),
                                             (double)(h.st_expanded - last[0]), (double)(h.st_edges - last[1]),
-                                            (double)(h.st_visited - last[2]), (double)h.st_maxscan[i]};
+                                            (double)(h.st_visited - last[2]),
+                                            tl ? h.tlog[i][2] * 0.01 : (double)h.st_maxscan[i]};
             last[0] = h.st_expanded;
             last[1] = h.st_edges;
             last[2] = h.st_visited;
@@ -422,10 +428,12 @@ struct ReachChain {
             ta.ntiles = c->ntiles;
             ta.toff = c->d_toff;
             ta.tile_of = c->d_tile_of;
-            ta.lpos[0] = c->d_lpos;
-            ta.lpos[1] = c->d_lpos + n + 1;
-            ta.larc[0] = c->d_larc;
-            ta.larc[1] = c->d_larc + c->tile_arcs0;
+            ta.adj[0] = c->d_tadj;
+            ta.adj[1] = c->d_tadj + n;
+            ta.xarc[0] = c->d_txarc;
+            ta.xarc[1] = c->d_txarc + n;
+            ta.ct[0] = c->d_tct;
+            ta.ct[1] = c->d_tct + n;
             ta.xflag = c->d_xflag;
             ta.inbox = c->d_inbox;
             ta.incnt = c->d_incnt;
@@ -435,6 +443,7 @@ struct ReachChain {
             ta.piv = c->d_piv;
             ta.ctl = c->d_ctl;
             ta.run = ++c->tile_run;
+            ta.instr = c->instr;
         }
     }
     int reset_mask() const { return sched >= SCC_SCHED_D ? 2 : 0; }
@@ -471,6 +480,11 @@ struct ReachChain {
             c->prof.expanded += (int64_t)c->h_ctl->st_expanded;
             c->prof.edges += (int64_t)c->h_ctl->st_edges;
             c->prof.visited += (int64_t)c->h_ctl->st_visited;
+            c->prof.tile_runs += (int64_t)c->h_ctl->st_truns;
+            c->prof.tile_rounds += (int64_t)c->h_ctl->st_trounds;
+            for (int q = 0; q < 4; q++) c->prof.tile_phase_ms[q] += (double)c->h_ctl->st_tclk[q] * 1e-5;
+            c->prof.tile_busy_ms += (double)c->h_ctl->st_tbusy * 1e-5;
+            c->prof.tile_idle_polls += (int64_t)c->h_ctl->st_tidle;
         }
     }
 };
@@ -673,12 +687,12 @@ void free_batch(scc_ctx *c) {
 }
 
 void free_tiles(scc_ctx *c) {
-    for (void *p : {(void *)c->d_toff, (void *)c->d_tile_of, (void *)c->d_lpos, (void *)c->d_inbox,
-                    (void *)c->d_tlist, (void *)c->d_larc, (void *)c->d_xflag, (void *)c->d_incnt,
-                    (void *)c->d_tstamp})
+    for (void *p : {(void *)c->d_toff, (void *)c->d_tile_of, (void *)c->d_inbox, (void *)c->d_tlist,
+                    (void *)c->d_tadj, (void *)c->d_txarc, (void *)c->d_tct, (void *)c->d_xflag,
+                    (void *)c->d_incnt, (void *)c->d_tstamp})
         if (p) (void)hipFree(p);
-    c->d_toff = c->d_tile_of = c->d_lpos = c->d_inbox = c->d_tlist = nullptr;
-    c->d_larc = nullptr;
+    c->d_toff = c->d_tile_of = c->d_inbox = c->d_tlist = nullptr;
+    c->d_tadj = c->d_txarc = c->d_tct = nullptr;
     c->d_xflag = nullptr;
     c->d_incnt = nullptr;
     c->d_tstamp = nullptr;
@@ -811,49 +825,96 @@ std::vector<int32_t> tile_partition(int32_t n, const int32_t *row, const int32_t
     return order;
 }
 
-// The tile structures of a graph renumbered by `order` (tiles toff): the arcs of
-// each vertex inside its tile as 16-bit indices from the tile's first vertex,
-// per direction, and a flag per vertex and direction for arcs that leave the
-// tile.  row .. colT: the CSR pair in the caller's ids.  False when a tile does
-// not fit a workgroup's LDS (the graph then runs without tiles).
+// Cells: every tile's vertices regrouped into cells of CELL vertices grown by
+// BFS over the tile's own vertices (arcs taken both ways), each from the
+// tile's first vertex not yet taken (in the tile's order) and, when the cell's
+// BFS runs out of neighbours in the tile, from the next one; so every cell of
+// a tile but its last is full.  order/toff: tile_partition's result, changed
+// in place (the tiles keep their vertex sets and offsets).
+void cell_order(int32_t n, const int32_t *row, const int32_t *col, const int32_t *rowT, const int32_t *colT,
+                std::vector<int32_t> &order, const std::vector<int32_t> &toff) {
+    const int32_t nt = (int32_t)toff.size() - 1;
+    std::vector<int32_t> tile((size_t)n), out;
+    out.reserve((size_t)n);
+    for (int32_t t = 0; t < nt; t++)
+        for (int32_t i = toff[t]; i < toff[t + 1]; i++) tile[order[i]] = t;
+    std::vector<uint8_t> taken((size_t)n, 0);
+    for (int32_t t = 0; t < nt; t++) {
+        int32_t next = toff[t];  // the tile's next seed candidate, in the tile's order
+        while ((int32_t)out.size() < toff[t + 1]) {
+            const size_t start = out.size();
+            size_t head = start;
+            auto full = [&] { return out.size() - start >= CELL || (int32_t)out.size() >= toff[t + 1]; };
+            while (!full()) {
+                if (head == out.size()) {
+                    while (taken[order[next]]) next++;
+                    taken[order[next]] = 1;
+                    out.push_back(order[next]);
+                    continue;
+                }
+                const int32_t v = out[head++];
+                for (int d = 0; d < 2 && !full(); d++) {
+                    const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
+                    for (int32_t e = r[v]; e < r[v + 1] && !full(); e++) {
+                        const int32_t u = cl[e];
+                        if (taken[u] || tile[u] != t) continue;
+                        taken[u] = 1;
+                        out.push_back(u);
+                    }
+                }
+            }
+        }
+    }
+    order.swap(out);
+}
+
+// The tile structures of a graph renumbered by `order` (tiles toff, cells of CELL
+// vertices from each tile's first vertex): per vertex and direction its neighbours
+// in its cell as a mask (bit = lane), up to CELL_XARCS neighbours in other cells of
+// its tile as 16-bit indices from the tile's first vertex, and flags (bit d: an arc
+// of direction d leaves the tile; bit 2 + d: more than CELL_XARCS go to other
+// cells).  row .. colT: the CSR pair in the caller's ids.
 scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const int32_t *rowT,
-                       const int32_t *colT, const std::vector<int32_t> &order, const std::vector<int32_t> &toff,
-                       bool *ok) {
-    *ok = false;
+                       const int32_t *colT, const std::vector<int32_t> &order, const std::vector<int32_t> &toff) {
     const int32_t n = c->g.n, nt = (int32_t)toff.size() - 1;
-    std::vector<int32_t> nid((size_t)n), tof((size_t)n), lpos(2 * ((size_t)n + 1));
+    std::vector<int32_t> nid((size_t)n), tof((size_t)n);
     for (int32_t i = 0; i < n; i++) nid[order[i]] = i;
     for (int32_t t = 0; t < nt; t++)
         for (int32_t i = toff[t]; i < toff[t + 1]; i++) tof[i] = t;
-    std::vector<uint16_t> arc[2];
+    std::vector<unsigned long long> adj(2 * (size_t)n, 0ull), xarc(2 * (size_t)n, ~0ull);
     std::vector<uint8_t> xf((size_t)n, 0);
     for (int d = 0; d < 2; d++) {
         const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
-        int32_t *lp = lpos.data() + (size_t)d * ((size_t)n + 1);
         for (int32_t i = 0; i < n; i++) {
-            const int32_t o = order[i], t = tof[i];
-            lp[i] = (int32_t)arc[d].size();
+            const int32_t o = order[i], t = tof[i], lo = toff[t];
+            const int32_t cell = (i - lo) / (int32_t)CELL;
+            unsigned long long m = 0ull, x = ~0ull;
+            int nx = 0;
             for (int32_t e = r[o]; e < r[o + 1]; e++) {
                 const int32_t u = nid[cl[e]];
-                if (tof[u] == t) arc[d].push_back((uint16_t)(u - toff[t]));
-                else xf[i] |= (uint8_t)(1u << d);
+                if (u == i) continue;  // a self-loop reaches nothing new
+                if (tof[u] != t) {
+                    xf[i] |= (uint8_t)(1u << d);
+                } else if ((u - lo) / (int32_t)CELL == cell) {
+                    m |= 1ull << ((u - lo) % (int32_t)CELL);
+                } else if (nx < CELL_XARCS) {
+                    x = (x & ~(0xFFFFull << (16 * nx))) | ((unsigned long long)(u - lo) << (16 * nx));
+                    nx++;
+                } else {
+                    xf[i] |= (uint8_t)(4u << d);
+                }
             }
+            adj[(size_t)d * n + i] = m;
+            xarc[(size_t)d * n + i] = x;
         }
-        lp[n] = (int32_t)arc[d].size();
-    }
-    for (int32_t t = 0; t < nt; t++) {
-        const int32_t lo = toff[t], hi = toff[t + 1];
-        const int64_t A = (int64_t)(lpos[hi] - lpos[lo]) + (lpos[(size_t)n + 1 + hi] - lpos[(size_t)n + 1 + lo]);
-        if (hi - lo > (int32_t)TILE_MAX || A > 65535 || tile_lds_bytes((uint32_t)(hi - lo), (uint32_t)A) > TILE_ARENA)
-            return SCC_OK;  // a tile too dense for LDS
     }
     free_tiles(c);
     const size_t vb = sizeof(int32_t) * (size_t)n;
-    const size_t A0 = arc[0].size(), A1 = arc[1].size();
     HIPCHK(c, dev_alloc(c, &c->d_toff, sizeof(int32_t) * toff.size()));
     HIPCHK(c, dev_alloc(c, &c->d_tile_of, vb));
-    HIPCHK(c, dev_alloc(c, &c->d_lpos, sizeof(int32_t) * lpos.size()));
-    HIPCHK(c, dev_alloc(c, &c->d_larc, sizeof(uint16_t) * std::max<size_t>(A0 + A1, 1)));
+    HIPCHK(c, dev_alloc(c, &c->d_tadj, sizeof(unsigned long long) * adj.size()));
+    HIPCHK(c, dev_alloc(c, &c->d_txarc, sizeof(unsigned long long) * xarc.size()));
+    HIPCHK(c, dev_alloc(c, &c->d_tct, sizeof(unsigned long long) * adj.size()));
     HIPCHK(c, dev_alloc(c, &c->d_xflag, (size_t)n));
     HIPCHK(c, dev_alloc(c, &c->d_inbox, 4 * vb));
     HIPCHK(c, dev_alloc(c, &c->d_incnt, sizeof(uint32_t) * 4 * (size_t)nt));
@@ -861,9 +922,8 @@ scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const
     HIPCHK(c, dev_alloc(c, &c->d_tlist, sizeof(int32_t) * 2 * (size_t)nt));
     HIPCHK(c, hipMemcpy(c->d_toff, toff.data(), sizeof(int32_t) * toff.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tile_of, tof.data(), vb, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_lpos, lpos.data(), sizeof(int32_t) * lpos.size(), hipMemcpyHostToDevice));
-    if (A0) HIPCHK(c, hipMemcpy(c->d_larc, arc[0].data(), sizeof(uint16_t) * A0, hipMemcpyHostToDevice));
-    if (A1) HIPCHK(c, hipMemcpy(c->d_larc + A0, arc[1].data(), sizeof(uint16_t) * A1, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tadj, adj.data(), sizeof(unsigned long long) * adj.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_txarc, xarc.data(), sizeof(unsigned long long) * xarc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_xflag, xf.data(), (size_t)n, hipMemcpyHostToDevice));
     const Launch l = launch_for(c);
     launch_fill_i32(l, (int32_t *)c->d_incnt, 4 * (int64_t)nt, 0);
@@ -871,8 +931,6 @@ scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ntiles = nt;
-    c->tile_arcs0 = (int64_t)A0;
-    *ok = true;
     return SCC_OK;
 }
 
@@ -1199,10 +1257,10 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
         const int32_t T = c->opts.tile_reach > 1 ? std::min<int32_t>(c->opts.tile_reach, (int32_t)TILE_MAX)
                                                  : (int32_t)TILE_MAX;
         std::vector<int32_t> toff;
-        const std::vector<int32_t> order = tile_partition(n, row, col, rowT, colT, T, toff);
+        std::vector<int32_t> order = tile_partition(n, row, col, rowT, colT, T, toff);
+        cell_order(n, row, col, rowT, colT, order, toff);
         CHK(permute_graph(c, row, rowT, order, {}));
-        bool ok = false;
-        CHK(build_tiles(c, row, col, rowT, colT, order, toff, &ok));
+        CHK(build_tiles(c, row, col, rowT, colT, order, toff));
     } else if (n > 0 && (c->opts.relabel > 0 || (c->opts.relabel == 0 && skewed))) {
         CHK(permute_graph(c, row, rowT, degree_order(n, deg, nullptr, 0), {}));
     }
@@ -1640,8 +1698,8 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         return SCC_OK;
     };
     s = body();
-    if (s == SCC_OK && c->ntiles > 0 && c->h_ctl->terr != 0)  // never: the upload checked every tile
-        s = fail(c, SCC_E_STATE, "a tile did not fit its workgroup's LDS");
+    if (s == SCC_OK && c->ntiles > 0 && c->h_ctl->terr != 0)  // never: the upload cuts tiles of <= TILE_MAX
+        s = fail(c, SCC_E_STATE, "a tile larger than TILE_MAX vertices");
     if (s == SCC_OK && c->d_oid && n > 0) {  // renumbered: the labels into the caller's order
         launch_gather_i32(launch_for(c), c->d_root, c->d_nid, c->d_label, n);
         c->label_in_root = false;

@@ -205,21 +205,26 @@ uint32_t batch_table_words(int32_t n_g);  // 4 * batch_table_size(n_g)
 void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const BatchArgs &b);
 
 // kernels_tile.hip -- the reach of a first FB step over tiles (regular graphs renumbered tile by
-// tile at upload).  Iteration 0 puts the pivots (seed list, ctl->qsize[0][0]) into their tiles'
-// inboxes; iteration i >= 1 runs every tile of list i in LDS and claims across tile boundaries.
-constexpr uint32_t TILE_MAX = 4096;           // vertices per tile
-constexpr uint32_t TILE_ARENA = 150u * 1024u;  // LDS per workgroup
-// LDS bytes of a tile of T vertices and A arcs inside it (both directions)
-__host__ __device__ inline uint32_t tile_lds_bytes(uint32_t T, uint32_t A) { return 13u * T + 4u + 2u * A + 2u; }
+// tile at upload).  A tile (one workgroup) is made of cells of 64 vertices (one wave lane each).
+// Iteration 0 puts the pivots (seed list, ctl->qsize[0][0]) into their tiles' inboxes; iteration
+// i >= 1 runs every tile of list i to its fixpoint and claims across tile boundaries.
+constexpr uint32_t TILE_MAX = 4096;  // vertices per tile: 64 cells of 64
+constexpr uint32_t CELL = 64;        // vertices per cell (the wavefront width)
+constexpr int CELL_XARCS = 4;        // arcs to other cells of the tile kept per vertex and direction
 struct TileArgs {
     const int32_t *row, *col, *rowT, *colT;
     uint32_t *word;
     int32_t n, ntiles;
-    const int32_t *toff;      // tile t = internal vertices [toff[t], toff[t + 1])
+    const int32_t *toff;      // tile t = internal vertices [toff[t], toff[t + 1]); cell k of it starts at toff[t] + 64k
     const int32_t *tile_of;   // tile of each vertex
-    const int32_t *lpos[2];   // arcs of v inside its tile: larc[d][lpos[d][v] .. lpos[d][v + 1]) (d 0 out, 1 in)
-    const uint16_t *larc[2];  // as indices from the tile's first vertex
-    const uint8_t *xflag;     // bit d: v has an arc of direction d that leaves its tile
+    const unsigned long long *adj[2];  // per vertex and direction (0 out, 1 in): its neighbours in its cell, bit = lane
+    unsigned long long *ct[2];         // written by iteration 0, per vertex: the vertices of its cell it is reached
+                                       // from inside the cell ([0] forward: that reach it, [1] backward: that it
+                                       // reaches), itself included
+    const unsigned long long *xarc[2]; // up to 4 neighbours in other cells of its tile, 16 bits each (index from
+                                       // the tile's first vertex; 0xFFFF none)
+    const uint8_t *xflag;     // bit d: an arc of direction d leaves the tile; bit 2 + d: more than 4 arcs of
+                              // direction d go to other cells of the tile (the row is scanned)
     int32_t *inbox;           // [parity][direction][n]: claimed into tile t at toff[t] + k
     uint32_t *incnt;          // [parity][direction][ntiles]
     unsigned long long *stamp;  // [ntiles]: (run << 32) | (iteration + 2) of the last list the tile joined
@@ -227,6 +232,7 @@ struct TileArgs {
     const int32_t *piv;       // iteration 0: the pivot list
     Ctl *ctl;
     int iter;
+    int instr;                // count work: reached vertices and their arcs, tile runs and rounds
     unsigned long long run;   // distinct per reach (stamps are never cleared)
 };
 void launch_tile_reach(hipStream_t s, uint32_t grid, const TileArgs &a);

@@ -278,6 +278,13 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
         ctl->st_expanded = 0;
         ctl->st_edges = 0;
         ctl->st_visited = 0;
+        ctl->st_truns = 0;
+        ctl->st_trounds = 0;
+        ctl->st_tbusy = ctl->st_tidle = 0;
+        for (int i = 0; i < 4; i++) {
+            ctl->st_tclk[i] = 0;
+            ctl->tlog[i][0] = ctl->tlog[i][1] = ctl->tlog[i][2] = 0;
+        }
     }
     if (what & 8192) {  // before a batch launch: work counter, statistics, first pivot
         ctl->wnext = 0;

@@ -103,11 +103,12 @@ typedef struct {
                                   ids.  0 (default): when the degrees are skewed (largest above 64
                                   times the mean) and n >= 65536; 1: always; -1: never */
     int32_t tile_reach;        /* read by scc_upload: number the vertices tile by tile (regions of up to
-                                  4,096 vertices grown by BFS) so that the reach of the first FB step
-                                  (schedule h, hybrid_policy 0, single graph) runs inside tiles, in LDS,
-                                  with claims across tile boundaries between iterations.  Results are
-                                  unchanged.  0 (default) and -1: never; 1: tiles of up to 4,096
-                                  vertices; N > 1: tiles of at most N vertices */
+                                  4,096 vertices grown by BFS, made of cells of 64 grown by BFS inside
+                                  them) so that the reach of the first FB step (schedule h,
+                                  hybrid_policy 0, single graph) closes cells in registers, tiles in
+                                  one workgroup each, and crosses tile boundaries between iterations.
+                                  Results are unchanged.  0 (default) and -1: never; 1: tiles of up
+                                  to 4,096 vertices; N > 1: tiles of at most N vertices */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -264,6 +265,12 @@ typedef struct {
     int64_t trim_launches;
     double reach_kernel_ms;   /* sum of level-kernel durations (flag 2) */
     double trim_kernel_ms;    /* sum of trim-sweep durations (flag 2) */
+    int64_t tile_runs;        /* tile reach: runs of a tile by a workgroup (flag 1) */
+    int64_t tile_rounds;      /* tile reach: claim words of a cell taken and closed in those runs (flag 1) */
+    double tile_phase_ms[4];  /* tile reach, summed over runs (flag 1): set-up and inbox, fixpoint,
+                                 write-back, claims across tile boundaries */
+    double tile_busy_ms;      /* tile reach, summed over waves (flag 1): time spent on takes */
+    int64_t tile_idle_polls;  /* tile reach (flag 1): polls of a wave that found nothing to take */
 } scc_profile_data;
 scc_status scc_profile_set(scc_ctx *ctx, int flags);
 scc_status scc_profile_get(scc_ctx *ctx, scc_profile_data *out, int reset);

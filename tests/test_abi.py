@@ -71,7 +71,7 @@ def header_struct_fields(name):
     for decl in body.split(";"):
         decl = decl.strip()
         if decl:
-            fields += [re.sub(r"[\s*]", "", f) for f in re.sub(r"^[A-Za-z0-9_ ]+?\s+(?=[*A-Za-z_])", "", decl,
+            fields += [re.sub(r"[\s*]|\[.*\]", "", f) for f in re.sub(r"^[A-Za-z0-9_ ]+?\s+(?=[*A-Za-z_])", "", decl,
                                                                    count=1).split(",")]
     return fields
 

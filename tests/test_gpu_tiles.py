@@ -46,9 +46,7 @@ def test_tile_reach_full_state(tctx, tile, variant):
         for sched in ("h", "t"):
             print(f"[tiles] {name} n={g.n} tile {tile} schedule {sched} variant {variant}", flush=True)
             labels, st = check_against_oracle(tctx, g, sched, variant)
-            # one tile of 4,096 cannot hold all arcs of R-MAT s12 (the upload then keeps the
-            # tile numbering without tiles)
-            assert st["renumbered"] == 2 or (name == "rmat12" and tile == 1)
+            assert st["renumbered"] == 2
             if name == "demo":
                 assert np.array_equal(labels, DEMO_EXPECT)
 
