@@ -40,15 +40,16 @@ struct scc_ctx {
     // tiles (opts.tile_reach; regular graphs): the upload numbered the vertices tile by tile;
     // the first FB step's reach runs over them (kernels_tile.hip)
     int32_t ntiles = 0;
-    int32_t *d_toff = nullptr, *d_tile_of = nullptr, *d_inbox = nullptr;
-    int32_t *d_tlist = nullptr;
+    int32_t *d_toff = nullptr;
+    uint8_t *d_tflags = nullptr;            // [3][ntiles]: tiles flagged for the coming iterations
+    unsigned long long *d_tdone = nullptr;  // [ntiles][128]: expanded vertices per cell and direction
     unsigned long long *d_tadj = nullptr;   // [2n]: neighbours in the vertex's cell (out, then in)
     unsigned long long *d_txarc = nullptr;  // [2n]: up to four neighbours in other cells of its tile
     unsigned long long *d_tct = nullptr;    // [2n]: cell closures, written by each tile reach
+    int2 *d_txout = nullptr;                // [2n]: up to two neighbours outside the vertex's tile
+    int32_t tile_size = 0;                  // vertices of every tile but the last
     uint8_t *d_xflag = nullptr;
-    uint32_t *d_incnt = nullptr;
-    unsigned long long *d_tstamp = nullptr;
-    unsigned long long tile_run = 0;
+
     bool track_mn = false;    // this run's updates keep d_mn
     int32_t *d_q[4] = {nullptr, nullptr, nullptr, nullptr};  // [direction * 2 + parity]
     int32_t *d_piv = nullptr;    // pivot list (also the forward seeds)
@@ -427,22 +428,20 @@ struct ReachChain {
             ta.n = n;
             ta.ntiles = c->ntiles;
             ta.toff = c->d_toff;
-            ta.tile_of = c->d_tile_of;
             ta.adj[0] = c->d_tadj;
             ta.adj[1] = c->d_tadj + n;
             ta.xarc[0] = c->d_txarc;
             ta.xarc[1] = c->d_txarc + n;
             ta.ct[0] = c->d_tct;
             ta.ct[1] = c->d_tct + n;
+            ta.xout[0] = c->d_txout;
+            ta.xout[1] = c->d_txout + n;
+            ta.tsize = c->tile_size;
             ta.xflag = c->d_xflag;
-            ta.inbox = c->d_inbox;
-            ta.incnt = c->d_incnt;
-            ta.stamp = c->d_tstamp;
-            ta.list[0] = c->d_tlist;
-            ta.list[1] = c->d_tlist + c->ntiles;
+            ta.flags = c->d_tflags;
+            ta.done = c->d_tdone;
             ta.piv = c->d_piv;
             ta.ctl = c->d_ctl;
-            ta.run = ++c->tile_run;
             ta.instr = c->instr;
         }
     }
@@ -459,11 +458,11 @@ struct ReachChain {
         lt.ev_start = e0;
         lt.ev_stop = e1;
         if (tiles) {
-            if (it == 0)  // inbox counts start at zero (each run of a tile clears what it took)
-                launch_fill_i32(launch_for(c), (int32_t *)c->d_incnt, 4 * (int64_t)c->ntiles, 0);
+            if (it == 0)  // no tile flagged (each iteration clears the flags it ran)
+                launch_fill_i32(launch_for(c), (int32_t *)c->d_tflags, (3 * (int64_t)c->ntiles + 3) / 4, 0);
             ta.iter = it;
             if (e0) (void)hipEventRecord(e0, c->stream);
-            launch_tile_reach(c->stream, (uint32_t)c->cu_count, ta);
+            launch_tile_reach(c->stream, (uint32_t)std::max(c->cu_count, 64), ta);
             if (e1) (void)hipEventRecord(e1, c->stream);
             return;
         }
@@ -687,15 +686,15 @@ void free_batch(scc_ctx *c) {
 }
 
 void free_tiles(scc_ctx *c) {
-    for (void *p : {(void *)c->d_toff, (void *)c->d_tile_of, (void *)c->d_inbox, (void *)c->d_tlist,
-                    (void *)c->d_tadj, (void *)c->d_txarc, (void *)c->d_tct, (void *)c->d_xflag,
-                    (void *)c->d_incnt, (void *)c->d_tstamp})
+    for (void *p : {(void *)c->d_toff, (void *)c->d_tflags, (void *)c->d_tdone, (void *)c->d_tadj,
+                    (void *)c->d_txarc, (void *)c->d_tct, (void *)c->d_txout, (void *)c->d_xflag})
         if (p) (void)hipFree(p);
-    c->d_toff = c->d_tile_of = c->d_inbox = c->d_tlist = nullptr;
+    c->d_toff = nullptr;
+    c->d_tflags = nullptr;
+    c->d_tdone = nullptr;
     c->d_tadj = c->d_txarc = c->d_tct = nullptr;
+    c->d_txout = nullptr;
     c->d_xflag = nullptr;
-    c->d_incnt = nullptr;
-    c->d_tstamp = nullptr;
     c->ntiles = 0;
 }
 
@@ -788,6 +787,29 @@ void row_indexes(scc_ctx *c) {
     c->stat_valid = true;
 }
 
+// Whether a BFS from vertex 0 over the arcs taken both ways reaches depth `depth`
+// (the graph's diameter is at least that); stops there.
+bool bfs_depth_at_least(int32_t n, const int32_t *row, const int32_t *col, const int32_t *rowT,
+                        const int32_t *colT, int32_t depth) {
+    std::vector<int32_t> lev((size_t)n, -1), cur{0}, nxt;
+    lev[0] = 0;
+    for (int32_t l = 0; !cur.empty(); l++) {
+        if (l >= depth) return true;
+        nxt.clear();
+        for (int32_t v : cur)
+            for (int d = 0; d < 2; d++) {
+                const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
+                for (int32_t e = r[v]; e < r[v + 1]; e++)
+                    if (lev[cl[e]] < 0) {
+                        lev[cl[e]] = l + 1;
+                        nxt.push_back(cl[e]);
+                    }
+            }
+        cur.swap(nxt);
+    }
+    return false;
+}
+
 // Tiles for the tile reach: regions of up to T vertices grown by BFS over the
 // arcs taken both ways, each from the lowest vertex not yet taken and, when the
 // region runs out of unvisited neighbours before T, from the next lowest, so
@@ -871,17 +893,21 @@ void cell_order(int32_t n, const int32_t *row, const int32_t *col, const int32_t
 // The tile structures of a graph renumbered by `order` (tiles toff, cells of CELL
 // vertices from each tile's first vertex): per vertex and direction its neighbours
 // in its cell as a mask (bit = lane), up to CELL_XARCS neighbours in other cells of
-// its tile as 16-bit indices from the tile's first vertex, and flags (bit d: an arc
-// of direction d leaves the tile; bit 2 + d: more than CELL_XARCS go to other
-// cells).  row .. colT: the CSR pair in the caller's ids.
+// its tile as 16-bit indices from the tile's first vertex, up to two neighbours in
+// other tiles, and flags (bit d: an arc of direction d leaves the tile; bit 2 + d:
+// more than CELL_XARCS go to other cells; bit 4 + d: more than two leave the tile).
+// row .. colT: the CSR pair in the caller's ids.
 scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const int32_t *rowT,
                        const int32_t *colT, const std::vector<int32_t> &order, const std::vector<int32_t> &toff) {
     const int32_t n = c->g.n, nt = (int32_t)toff.size() - 1;
+    for (int32_t t = 1; t < nt; t++)  // the kernel finds a vertex's tile by division
+        if (toff[t] != t * toff[1]) return fail(c, SCC_E_STATE, "tiles of unequal size");
     std::vector<int32_t> nid((size_t)n), tof((size_t)n);
     for (int32_t i = 0; i < n; i++) nid[order[i]] = i;
     for (int32_t t = 0; t < nt; t++)
         for (int32_t i = toff[t]; i < toff[t + 1]; i++) tof[i] = t;
     std::vector<unsigned long long> adj(2 * (size_t)n, 0ull), xarc(2 * (size_t)n, ~0ull);
+    std::vector<int2> xout(2 * (size_t)n, int2{-1, -1});
     std::vector<uint8_t> xf((size_t)n, 0);
     for (int d = 0; d < 2; d++) {
         const int32_t *r = d ? rowT : row, *cl = d ? colT : col;
@@ -889,12 +915,17 @@ scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const
             const int32_t o = order[i], t = tof[i], lo = toff[t];
             const int32_t cell = (i - lo) / (int32_t)CELL;
             unsigned long long m = 0ull, x = ~0ull;
-            int nx = 0;
+            int nx = 0, no = 0;
+            int2 &xo = xout[(size_t)d * n + i];
             for (int32_t e = r[o]; e < r[o + 1]; e++) {
                 const int32_t u = nid[cl[e]];
                 if (u == i) continue;  // a self-loop reaches nothing new
                 if (tof[u] != t) {
                     xf[i] |= (uint8_t)(1u << d);
+                    if (no == 0) xo.x = u;
+                    else if (no == 1) xo.y = u;
+                    else xf[i] |= (uint8_t)(16u << d);
+                    no++;
                 } else if ((u - lo) / (int32_t)CELL == cell) {
                     m |= 1ull << ((u - lo) % (int32_t)CELL);
                 } else if (nx < CELL_XARCS) {
@@ -909,28 +940,25 @@ scc_status build_tiles(scc_ctx *c, const int32_t *row, const int32_t *col, const
         }
     }
     free_tiles(c);
-    const size_t vb = sizeof(int32_t) * (size_t)n;
     HIPCHK(c, dev_alloc(c, &c->d_toff, sizeof(int32_t) * toff.size()));
-    HIPCHK(c, dev_alloc(c, &c->d_tile_of, vb));
     HIPCHK(c, dev_alloc(c, &c->d_tadj, sizeof(unsigned long long) * adj.size()));
     HIPCHK(c, dev_alloc(c, &c->d_txarc, sizeof(unsigned long long) * xarc.size()));
     HIPCHK(c, dev_alloc(c, &c->d_tct, sizeof(unsigned long long) * adj.size()));
+    HIPCHK(c, dev_alloc(c, &c->d_txout, sizeof(int2) * xout.size()));
     HIPCHK(c, dev_alloc(c, &c->d_xflag, (size_t)n));
-    HIPCHK(c, dev_alloc(c, &c->d_inbox, 4 * vb));
-    HIPCHK(c, dev_alloc(c, &c->d_incnt, sizeof(uint32_t) * 4 * (size_t)nt));
-    HIPCHK(c, dev_alloc(c, &c->d_tstamp, sizeof(unsigned long long) * (size_t)nt));
-    HIPCHK(c, dev_alloc(c, &c->d_tlist, sizeof(int32_t) * 2 * (size_t)nt));
+    HIPCHK(c, dev_alloc(c, &c->d_tflags, 3 * (size_t)nt + 4));
+    HIPCHK(c, dev_alloc(c, &c->d_tdone, sizeof(unsigned long long) * 2 * TILE_MAX / CELL * (size_t)nt));
     HIPCHK(c, hipMemcpy(c->d_toff, toff.data(), sizeof(int32_t) * toff.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_tile_of, tof.data(), vb, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tadj, adj.data(), sizeof(unsigned long long) * adj.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_txarc, xarc.data(), sizeof(unsigned long long) * xarc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_xflag, xf.data(), (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_txout, xout.data(), sizeof(int2) * xout.size(), hipMemcpyHostToDevice));
     const Launch l = launch_for(c);
-    launch_fill_i32(l, (int32_t *)c->d_incnt, 4 * (int64_t)nt, 0);
-    launch_fill_u64(l, c->d_tstamp, nt, 0ull);
+    launch_fill_i32(l, (int32_t *)c->d_tflags, (3 * (int64_t)nt + 3) / 4, 0);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ntiles = nt;
+    c->tile_size = toff.size() > 1 ? toff[1] : 0;
     return SCC_OK;
 }
 
@@ -1251,8 +1279,11 @@ scc_status scc_upload(scc_ctx *c, int32_t n, int64_t m, const int32_t *row, cons
         dmax = std::max<int64_t>(dmax, deg[v]);
     }
     const bool skewed = n >= 65536 && dmax * (int64_t)n > 64 * 2 * m;  // max above 64 x the mean (2m / n)
-    // tiles for the tile reach: when asked for (the level kernels are faster by default)
-    const bool tiles = n > 0 && c->opts.tile_reach > 0;
+    // tiles for the tile reach: when asked for, and by default for graphs of high diameter
+    // (a long BFS from vertex 0), where the level kernels pay a launch or a barrier per level
+    const bool tiles = n > 0 && (c->opts.tile_reach > 0 ||
+                                 (c->opts.tile_reach == 0 && c->opts.relabel == 0 && !skewed && n >= 65536 &&
+                                  bfs_depth_at_least(n, row, col, rowT, colT, TILE_DEPTH)));
     if (tiles) {
         const int32_t T = c->opts.tile_reach > 1 ? std::min<int32_t>(c->opts.tile_reach, (int32_t)TILE_MAX)
                                                  : (int32_t)TILE_MAX;

@@ -76,12 +76,11 @@ struct Ctl {
     unsigned long long tail_pivots;  // k_fb_tail: pivots elected
     uint32_t tail_live0;             // k_fb_tail: live vertices when it started
     uint32_t wnext;                  // k_fb_batch: the next graph to take (work counter)
-    uint32_t tcnt[4];                // tile reach: lengths of the tile lists (iteration i reads [i & 3])
     uint32_t terr;                   // tile reach: a tile of more than TILE_MAX vertices (never: the
                                      // upload cuts them smaller)
     unsigned long long st_truns, st_trounds;  // instrumented tile reach: tile runs, their rounds
     unsigned long long st_tclk[4];            // instrumented tile reach: 100 MHz ticks of the runs' phases
-                                              // (set-up and inbox, rounds, write-back, claims across)
+                                              // (set-up and seeds, fixpoint, counting, claims across)
     unsigned long long st_tbusy, st_tidle;  // instrumented tile reach: 100 MHz ticks waves spent on takes,
                                             // idle polls
     uint32_t tlog[4][3];  // instrumented tile reach, ring by iteration: tiles run, most rounds and

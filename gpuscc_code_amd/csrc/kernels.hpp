@@ -206,9 +206,11 @@ void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const Batc
 
 // kernels_tile.hip -- the reach of a first FB step over tiles (regular graphs renumbered tile by
 // tile at upload).  A tile (one workgroup) is made of cells of 64 vertices (one wave lane each).
-// Iteration 0 puts the pivots (seed list, ctl->qsize[0][0]) into their tiles' inboxes; iteration
-// i >= 1 runs every tile of list i to its fixpoint and claims across tile boundaries.
+// Iteration 0 computes the cell closures and flags the tiles of the pivots (seed list,
+// ctl->qsize[0][0]); iteration i >= 1 runs every flagged tile to its fixpoint and claims across
+// tile boundaries, flagging the tiles it claims into.
 constexpr uint32_t TILE_MAX = 4096;  // vertices per tile: 64 cells of 64
+constexpr int32_t TILE_DEPTH = 256;  // BFS depth from vertex 0 above which the upload tiles by default
 constexpr uint32_t CELL = 64;        // vertices per cell (the wavefront width)
 constexpr int CELL_XARCS = 4;        // arcs to other cells of the tile kept per vertex and direction
 struct TileArgs {
@@ -216,24 +218,25 @@ struct TileArgs {
     uint32_t *word;
     int32_t n, ntiles;
     const int32_t *toff;      // tile t = internal vertices [toff[t], toff[t + 1]); cell k of it starts at toff[t] + 64k
-    const int32_t *tile_of;   // tile of each vertex
     const unsigned long long *adj[2];  // per vertex and direction (0 out, 1 in): its neighbours in its cell, bit = lane
     unsigned long long *ct[2];         // written by iteration 0, per vertex: the vertices of its cell it is reached
                                        // from inside the cell ([0] forward: that reach it, [1] backward: that it
                                        // reaches), itself included
     const unsigned long long *xarc[2]; // up to 4 neighbours in other cells of its tile, 16 bits each (index from
                                        // the tile's first vertex; 0xFFFF none)
+    const int2 *xout[2];      // per vertex and direction: up to two neighbours in other tiles (-1 none)
     const uint8_t *xflag;     // bit d: an arc of direction d leaves the tile; bit 2 + d: more than 4 arcs of
-                              // direction d go to other cells of the tile (the row is scanned)
-    int32_t *inbox;           // [parity][direction][n]: claimed into tile t at toff[t] + k
-    uint32_t *incnt;          // [parity][direction][ntiles]
-    unsigned long long *stamp;  // [ntiles]: (run << 32) | (iteration + 2) of the last list the tile joined
-    int32_t *list[2];         // tile lists, by iteration parity
+                              // direction d go to other cells of the tile; bit 4 + d: more than 2 leave the
+                              // tile (for those, the row is scanned)
+    int32_t tsize;            // vertices of every tile but the last: tile t starts at t * tsize
+    uint8_t *flags;           // [3][ntiles]: iteration i runs the tiles flagged in [i % 3], flags tiles for
+                              // iteration i + 1 in [(i + 1) % 3] and clears [(i + 2) % 3]
+    unsigned long long *done; // [ntiles][128]: per cell and direction (claim-word order), the vertices the
+                              // tile has expanded in this reach
     const int32_t *piv;       // iteration 0: the pivot list
     Ctl *ctl;
     int iter;
     int instr;                // count work: reached vertices and their arcs, tile runs and rounds
-    unsigned long long run;   // distinct per reach (stamps are never cleared)
 };
 void launch_tile_reach(hipStream_t s, uint32_t grid, const TileArgs &a);
 

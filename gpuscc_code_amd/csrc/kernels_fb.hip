@@ -246,7 +246,6 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
     }
     if (what & 1) {
         for (int i = 0; i < 4; i++) ctl->changed[i] = 0;
-        for (int i = 0; i < 4; i++) ctl->tcnt[i] = 0;
         for (int d = 0; d < 2; d++)
             for (int i = 0; i < 4; i++) ctl->hsize[d][i] = 0;
         ctl->ran = 0;

@@ -37,6 +37,7 @@ constexpr int TILE_BLOCK = 1024;
 constexpr int TILE_WAVES = TILE_BLOCK / WAVE;                          // 16
 constexpr int TILE_CELLS = (int)(TILE_MAX / CELL);                     // 64
 constexpr int CELLS_PER_WAVE = TILE_CELLS / TILE_WAVES;                // 4
+constexpr int TILE_LIST_MAX = TILE_BLOCK / 64;  // a workgroup's tiles of one chunk of flags (grid >= 64)
 static_assert(CELL == WAVE, "a cell is one vertex per lane of a wave");
 
 // claims into cell k, direction d: [cidx(k, d)], bit = lane; a wave's own entries are
@@ -46,24 +47,19 @@ __device__ inline uint32_t cidx(uint32_t k, int d) {
 }
 
 struct TileShared {
-    unsigned long long claim[TILE_CELLS * 2];    // [cidx]: claims not yet taken by the cell's wave
-    unsigned long long reached[TILE_CELLS * 2];  // [cidx]: reached so far (written by the cell's wave only)
-    unsigned long long seeds[TILE_CELLS * 2];    // [cidx]: the inbox's vertices
-    // the tile's arcs to other cells ([direction][vertex]); after the fixpoint, the list of
-    // (vertex, direction) whose arcs leave the tile
-    union {
-        unsigned long long xa[2 * TILE_MAX];
-        uint32_t exits[2 * TILE_MAX];
-    };
+    unsigned long long claim[TILE_CELLS * 2];  // [cidx]: claims not yet taken by the cell's wave
+    unsigned long long xa[2 * TILE_MAX];       // the tile's arcs to other cells ([direction][vertex])
+    uint32_t exits[2 * TILE_MAX];  // (vertex, direction) reached in this run with arcs leaving the tile
     // claim words that are non-zero or taken and not yet processed: a depositor counts a
     // word before it can turn non-zero (and takes the count back when it was non-zero
     // already), a wave uncounts a word it took once it has made all its own claims, so the
     // count never falls below the work left and zero means the tile is at its fixpoint
     uint32_t pending;
     unsigned long long sink[WAVE];  // a lane's claim slot without a target ORs nothing into its own word
-    uint32_t n_in[2];  // inbox entries per direction
     uint32_t n_exit;
     uint32_t stuck;    // a wave gave up waiting (never: the count reaches zero)
+    unsigned long long ins[6];  // instrumented runs: the waves' busy ticks, idle polls, takes, most takes of
+                                // a wave, vertices reached, their arcs (flushed once per run)
 };
 
 // One workgroup works on a tile: a workgroup-scope barrier orders its LDS and
@@ -84,28 +80,25 @@ __device__ inline unsigned long long lane64(unsigned long long x, int l) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
-__device__ inline void tile_push_next(const TileArgs &a, int32_t t2) {
-    // join the next iteration's list once: the stamp holds the last list the tile joined
-    const unsigned long long s = (a.run << 32) | (unsigned long long)(uint32_t)(a.iter + 2);
-    if (atomicMax(&a.stamp[t2], s) < s) {
-        const uint32_t k = atomicAdd(&a.ctl->tcnt[(a.iter + 1) & 3], 1u);
-        a.list[(a.iter + 1) & 1][k] = t2;
-    }
+// a claim of u across a tile boundary in direction d, and u's tile flagged for
+// the next iteration (every tile but the last holds tsize vertices).  The claim
+// sets the bit without reading the word: that tile's next run takes the live
+// vertices whose bits are set and which it has not expanded, so a bit set twice,
+// or on an eliminated vertex, changes nothing (the F and B bits of eliminated
+// vertices are read by nothing).
+__device__ inline void claim_across(const TileArgs &a, int d, int32_t u) {
+    atomicOr(&a.word[u], d ? ST_B : ST_F);
+    a.flags[(size_t)((a.iter + 1) % 3) * a.ntiles + u / a.tsize] = 1;
 }
 
-// u, reached across an arc of direction d, into its tile's inbox for the next iteration
-__device__ inline void tile_post(const TileArgs &a, int d, int32_t u) {
-    const int32_t t2 = a.tile_of[u];
-    const int par = (a.iter + 1) & 1;
-    const uint32_t k = atomicAdd(&a.incnt[(par * 2 + d) * a.ntiles + t2], 1u);
-    a.inbox[(size_t)(par * 2 + d) * a.n + a.toff[t2] + k] = u;
-    tile_push_next(a, t2);
-}
+// orders this wave's LDS operations only (a workgroup fence on all memory would also
+// wait for its global write-backs)
+__device__ inline void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 
 // claim bit of word idx for another cell, counted in pending first
 __device__ inline void deposit(TileShared &sh, uint32_t idx, unsigned long long bit) {
     atomicAdd(&sh.pending, 1u);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // counted before the word can turn non-zero
+    lds_fence();  // counted before the word can turn non-zero
     if (atomicOr(&sh.claim[idx], bit) != 0ull) atomicSub(&sh.pending, 1u);
 }
 
@@ -122,12 +115,6 @@ __device__ __noinline__ void cross_row(TileShared &sh, const int32_t *row, const
         const uint32_t u = (uint32_t)(col[e] - lo);
         if (u < T && u / CELL != i / CELL) deposit(sh, cidx(u / CELL, d), 1ull << (u % CELL));
     }
-}
-
-// a wave's sum
-__device__ inline uint32_t wave_sum(uint32_t x) {
-    for (int o = WAVE / 2; o > 0; o >>= 1) x += (uint32_t)__shfl_xor(x, o, WAVE);
-    return x;
 }
 
 constexpr uint32_t TILE_SPIN_LIMIT = 1u << 20;  // idle polls before a wave gives up (never reached)
@@ -172,7 +159,6 @@ __device__ void tile_run(const TileArgs &a, TileShared &sh, int32_t t, bool &fou
         if (tid == 0) atomicAdd(&a.ctl->terr, 1u);
         return;
     }
-    const int par = a.iter & 1;
     unsigned long long clk[5];  // phase boundaries (instrumented runs, thread 0)
     if (a.instr) clk[0] = wall_clock64();
     for (uint32_t i = tid; i < TILE_CELLS * 2u; i += TILE_BLOCK) sh.claim[i] = 0ull;
@@ -180,13 +166,19 @@ __device__ void tile_run(const TileArgs &a, TileShared &sh, int32_t t, bool &fou
         sh.xa[i] = a.xarc[0][lo + (int32_t)i];
         sh.xa[TILE_MAX + i] = a.xarc[1][lo + (int32_t)i];
     }
-    if (tid < 2) sh.n_in[tid] = a.incnt[(par * 2 + (int)tid) * a.ntiles + t];
     if (tid == 0) {
         sh.n_exit = 0;
         sh.pending = 0;
         sh.stuck = 0;
     }
     if (tid < WAVE) sh.sink[tid] = 0ull;
+    if (tid < 6) sh.ins[tid] = 0ull;
+    const uint32_t base = wv * CELLS_PER_WAVE * 2;  // this wave's claim words
+    unsigned long long *done = a.done + (size_t)t * (TILE_CELLS * 2);
+    // lane q < 8 of a wave keeps the masks of its pair q = 2j + d (its cell j, direction d):
+    // the vertices the tile has expanded so far
+    unsigned long long reach_l = lane < 8 ? done[base + lane] : 0ull;
+    const unsigned long long done0 = reach_l;
     // this wave's cells, one vertex per lane: word, closure masks in the cell, flags
     uint32_t w[CELLS_PER_WAVE], xfl = 0;
     unsigned long long ct[CELLS_PER_WAVE][2];
@@ -200,49 +192,32 @@ __device__ void tile_run(const TileArgs &a, TileShared &sh, int32_t t, bool &fou
         ct[j][0] = ok ? a.ct[0][v] : 0ull;
         ct[j][1] = ok ? a.ct[1][v] : 0ull;
     }
-    tile_sync();
-    // the inbox (claimed into the tile across its boundary since its last run; the pivots
-    // in iteration 0) as the first claims
-    for (int d = 0; d < 2; d++) {
-        const uint32_t ni = sh.n_in[d];
-        const int32_t *in = a.inbox + (size_t)(par * 2 + d) * a.n + lo;
-        for (uint32_t k0 = wv * WAVE; k0 < ni; k0 += TILE_BLOCK) {  // wave-uniform
-            const uint32_t k = k0 + lane;
-            const bool ok = k < ni;
-            const uint32_t i = ok ? (uint32_t)(in[k] - lo) : 0u;
-            // counted for the whole wave first, taken back for words that were non-zero
-            const uint32_t cnt = (uint32_t)__popcll(__ballot(ok));
-            if (lane == 0) atomicAdd(&sh.pending, cnt);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            const unsigned long long old =
-                atomicOr(ok ? &sh.claim[cidx(i / CELL, d)] : &sh.sink[lane], ok ? 1ull << (i % CELL) : 0ull);
-            const uint32_t dup = (uint32_t)__popcll(__ballot(ok && old != 0ull));
-            if (lane == 0 && dup) atomicSub(&sh.pending, dup);
-        }
-    }
-    tile_sync();
-    if (tid < 2) a.incnt[(par * 2 + (int)tid) * a.ntiles + t] = 0;  // consumed
-    // reached before this run: the words' bits, except the inbox's vertices, which are
-    // expanded whatever their words say (a claim sets the word before the tile runs).  Lane
-    // q < 8 of a wave keeps the masks of its pair q = 2j + d (its cell j, direction d).
-    const uint32_t base = wv * CELLS_PER_WAVE * 2;  // this wave's claim words
+    // the seeds: vertices whose words are reached but which the tile has not expanded
+    // (claimed across its boundary since its last run, the pivots), live ones only; each
+    // wave puts them in its own claim words, counted
     if (a.instr) clk[1] = wall_clock64();
-    const unsigned long long seed_l = lane < 8 ? sh.claim[base + lane] : 0ull;
-    unsigned long long reach_l = 0ull;
+    unsigned long long seed_l = 0ull;
 #pragma unroll
     for (int j = 0; j < CELLS_PER_WAVE; j++) {
 #pragma unroll
         for (int d = 0; d < 2; d++) {
-            const unsigned long long wb = __ballot(w[j] & (d ? ST_B : ST_F));
-            if (lane == (uint32_t)(j * 2 + d)) reach_l = wb;
+            const unsigned long long wb = __ballot((w[j] & (d ? ST_B : ST_F)) && !(w[j] & ST_E));
+            if (lane == (uint32_t)(j * 2 + d)) seed_l = wb & ~reach_l;
         }
     }
-    reach_l &= ~seed_l;
-    tile_sync();  // every wave has its inbox claims before claims inside the tile begin
+    tile_sync();  // the claim words and the count are cleared
+    if (lane < 8 && seed_l) sh.claim[base + lane] = seed_l;
+    const uint32_t nseed = (uint32_t)__popcll(__ballot(seed_l != 0ull));
+    if (lane == 0 && nseed) atomicAdd(&sh.pending, nseed);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    tile_sync();  // every wave has counted its seeds before claims inside the tile begin
     // to the fixpoint, each wave on its own: take every claim word of its cells at once, close
-    // the claims inside their cells, claim into other cells (counted first), uncount the words
-    // taken; stop when nothing is counted
-    uint32_t polls = 0, idles = 0;
+    // the claims inside their cells, write the new bits back, list the new vertices whose arcs
+    // leave the tile, claim into other cells (counted first), uncount the words taken; stop
+    // when nothing is counted
+    uint32_t polls = 0, idles = 0, n_reached = 0;
+    uint32_t newbits = 0;  // bits reached in this run, 8 per cell j: written back at the end for vertices no
+                           // other tile can claim (no arc crosses the tile boundary at them)
     unsigned long long busy = 0, t_take = 0;
     for (uint32_t idle = 0;;) {
         unsigned long long pl = lane < 8 ? lds_load(&sh.claim[base + lane]) : 0ull;
@@ -288,9 +263,11 @@ __device__ void tile_run(const TileArgs &a, TileShared &sh, int32_t t, bool &fou
         // the words they did not turn non-zero is taken back afterwards)
         if (bound) {
             if (lane == 0) atomicAdd(&sh.pending, bound);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // counted before any word turns non-zero
+            lds_fence();  // counted before any word turns non-zero
         }
         uint32_t made = 0;  // claims that turned a word non-zero
+        unsigned long long xm_l = 0ull;  // lane q: the new vertices of pair q whose arcs leave the tile
+        uint32_t nx = 0;
 #pragma unroll
         for (int j = 0; j < CELLS_PER_WAVE; j++) {
             const uint32_t i = (wv + TILE_WAVES * j) * CELL + lane;
@@ -299,154 +276,182 @@ __device__ void tile_run(const TileArgs &a, TileShared &sh, int32_t t, bool &fou
                 const int q = j * 2 + d;
                 const unsigned long long fr = lane64(fr_l, q);
                 if (fr == 0ull) continue;
+                n_reached += (uint32_t)__popcll(fr);
                 const bool mine = (fr >> lane) & 1ull;
-                const unsigned long long x = mine ? sh.xa[d * TILE_MAX + i] : ~0ull;
+                const unsigned long long xq = mine ? sh.xa[d * TILE_MAX + i] : ~0ull;
                 // the four claims issued together (an unused slot ORs nothing into the lane's
                 // own sink word: no two lanes meet on one address for it)
                 unsigned long long old[CELL_XARCS];
 #pragma unroll
                 for (int k = 0; k < CELL_XARCS; k++) {
-                    const uint32_t u = (uint32_t)((x >> (16 * k)) & 0xFFFFull);
+                    const uint32_t u = (uint32_t)((xq >> (16 * k)) & 0xFFFFull);
                     const bool ok = u != 0xFFFFu;
                     old[k] = atomicOr(ok ? &sh.claim[cidx(u / CELL, d)] : &sh.sink[lane], ok ? 1ull << (u % CELL) : 0ull);
                 }
 #pragma unroll
                 for (int k = 0; k < CELL_XARCS; k++)
-                    made += (uint32_t)__popcll(__ballot(((x >> (16 * k)) & 0xFFFFull) != 0xFFFFull && old[k] == 0ull));
+                    made += (uint32_t)__popcll(__ballot(((xq >> (16 * k)) & 0xFFFFull) != 0xFFFFull && old[k] == 0ull));
                 if (mine && ((xfl >> (8 * j + 2 + d)) & 1u))  // more such arcs: the whole row
                     cross_row(sh, d ? a.rowT : a.row, d ? a.colT : a.col, lo, T, i, d);
+                // the new bits back: at once where other tiles may claim too (no return value
+                // awaited), at the end elsewhere
+                if (mine) {
+                    if (xfl & (3u << (8 * j))) atomicOr(&a.word[lo + (int32_t)i], d ? ST_B : ST_F);
+                    else newbits |= (d ? ST_B : ST_F) << (8 * j);
+                }
+                const unsigned long long xm = __ballot(mine && ((xfl >> (8 * j + d)) & 1u));
+                if (lane == (uint32_t)q) xm_l = xm;
+                nx += (uint32_t)__popcll(xm);
             }
         }
-        const uint32_t dups = bound - made;
-        if (lane == 0) atomicSub(&sh.pending, dups + (uint32_t)__popcll(m));  // the words taken are done
+        // the new vertices whose arcs leave the tile listed (one LDS atomic per take)
+        if (nx) {
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&sh.n_exit, nx);
+            at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+#pragma unroll
+            for (int j = 0; j < CELLS_PER_WAVE; j++) {
+                const uint32_t i = (wv + TILE_WAVES * j) * CELL + lane;
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    const unsigned long long xm = lane64(xm_l, j * 2 + d);
+                    if ((xm >> lane) & 1ull)
+                        sh.exits[at + (uint32_t)__popcll(xm & below)] =
+                            i | ((uint32_t)d << 16) | (((xfl >> (8 * j + 4 + d)) & 1u) << 17);
+                    at += (uint32_t)__popcll(xm);
+                }
+            }
+        }
+        if (lane == 0) atomicSub(&sh.pending, (bound - made) + (uint32_t)__popcll(m));  // the words taken are done
         if (a.instr) busy += wall_clock64() - t_take;
     }
-    if (lane < 8) {  // the masks, for the write-back
-        sh.reached[base + lane] = reach_l;
-        sh.seeds[base + lane] = seed_l;
+    if (lane < 8) done[base + lane] = reach_l;
+#pragma unroll
+    for (int j = 0; j < CELLS_PER_WAVE; j++) {  // only this run writes these words during the reach
+        const uint32_t nb = (newbits >> (8 * j)) & 0xFFu;
+        if (nb) a.word[lo + (int32_t)((wv + TILE_WAVES * j) * CELL + lane)] = w[j] | nb;
     }
     tile_sync();
+    if (a.instr) clk[2] = wall_clock64();
     if (tid == 0 && sh.stuck) atomicAdd(&a.ctl->terr, 1u);
-    if (a.instr) {
+    if (a.instr) {  // arcs of the vertices reached in this run (the inbox's vertices count as new)
+        uint32_t arcs = 0;
+#pragma unroll
+        for (int j = 0; j < CELLS_PER_WAVE; j++) {
+            const uint32_t i = (wv + TILE_WAVES * j) * CELL + lane;
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int q = j * 2 + d;
+                const unsigned long long fresh = lane64(reach_l, q) & ~lane64(done0, q);
+                if ((fresh >> lane) & 1ull) {
+                    const int32_t *row = d ? a.rowT : a.row;
+                    arcs += (uint32_t)(row[lo + (int32_t)i + 1] - row[lo + (int32_t)i]);
+                }
+            }
+        }
+        for (int o = WAVE / 2; o > 0; o >>= 1) arcs += (uint32_t)__shfl_xor(arcs, o, WAVE);
         if (lane == 0) {
-            atomicAdd(&a.ctl->st_tbusy, busy);
-            atomicAdd(&a.ctl->st_tidle, (unsigned long long)idles);
-            atomicAdd(&a.ctl->st_trounds, (unsigned long long)polls);
-            atomicMax(&a.ctl->tlog[a.iter & 3][1], polls);
+            atomicAdd(&sh.ins[0], busy);
+            atomicAdd(&sh.ins[1], (unsigned long long)idles);
+            atomicAdd(&sh.ins[2], (unsigned long long)polls);
+            atomicMax(&sh.ins[3], (unsigned long long)polls);
+            atomicAdd(&sh.ins[4], (unsigned long long)n_reached);
+            atomicAdd(&sh.ins[5], (unsigned long long)arcs);
         }
-        if (tid == 0) {
-            atomicAdd(&a.ctl->st_truns, 1ull);
-            atomicAdd(&a.ctl->tlog[a.iter & 3][0], 1u);
-        }
-        clk[2] = wall_clock64();
+        clk[3] = wall_clock64();
     }
-    // the new bits back (others may claim the tile's vertices meanwhile: atomics), and the
-    // vertices reached in this run whose arcs leave the tile listed (the arcs to other cells
-    // are no longer needed)
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t n_reached = 0, n_arcs = 0;  // instrumented counts of this lane
-#pragma unroll
-    for (int j = 0; j < CELLS_PER_WAVE; j++) {
-        const uint32_t i = (wv + TILE_WAVES * j) * CELL + lane;
-        // reached in this run: reached now, and not before (the inbox's vertices count as new)
-        uint32_t bits = 0;
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const uint32_t bit = d ? ST_B : ST_F;
-            const unsigned long long r = sh.reached[base + j * 2 + d], sd = sh.seeds[base + j * 2 + d];
-            if ((((r & (sd | ~__ballot(w[j] & bit))) >> lane) & 1ull) != 0) bits |= bit;
-        }
-        if (bits) atomicOr(&a.word[lo + (int32_t)i], bits);
-#pragma unroll
-        for (int d = 0; d < 2; d++) {  // one LDS atomic per wave and list
-            const bool x = (bits & (d ? ST_B : ST_F)) && ((xfl >> (8 * j + d)) & 1u);
-            const unsigned long long m = __ballot(x);
-            if (m == 0ull) continue;
-            uint32_t at = 0;
-            if (lane == (uint32_t)__builtin_ctzll(m)) at = atomicAdd(&sh.n_exit, (uint32_t)__popcll(m));
-            at = (uint32_t)__shfl(at, __builtin_ctzll(m), WAVE);
-            if (x) sh.exits[at + (uint32_t)__popcll(m & below)] = i | ((uint32_t)d << 16);
-        }
-        if (a.instr && bits) {  // each vertex reached in this run expands its row in each direction
-            const int32_t v = lo + (int32_t)i;
-            n_reached += (bits & ST_F ? 1u : 0u) + (bits & ST_B ? 1u : 0u);
-            n_arcs += (uint32_t)((bits & ST_F ? a.row[v + 1] - a.row[v] : 0) +
-                                 (bits & ST_B ? a.rowT[v + 1] - a.rowT[v] : 0));
-        }
-    }
-    if (a.instr) {
-        for (int o = WAVE / 2; o > 0; o >>= 1) {
-            n_reached += (uint32_t)__shfl_xor(n_reached, o, WAVE);
-            n_arcs += (uint32_t)__shfl_xor(n_arcs, o, WAVE);
-        }
-        if (lane == 0 && n_reached) {
-            atomicAdd(&a.ctl->st_expanded, (unsigned long long)n_reached);
-            atomicAdd(&a.ctl->st_visited, (unsigned long long)n_reached);
-            atomicAdd(&a.ctl->st_edges, (unsigned long long)n_arcs);
-        }
-    }
-    tile_sync();
-    if (a.instr) clk[3] = wall_clock64();
-    // claims across the tile boundary: the level kernels' test (live, not yet reached; in a
-    // first FB step every live vertex has the one colour), the claimed vertex into its tile's
-    // inbox.  The target's tile and the tile's first vertex are loaded with its word, ahead
-    // of the claim.
+    // claims across the tile boundary from the listed vertices: their neighbours outside the
+    // tile kept at upload, or their whole rows when they have more
     const uint32_t ne = sh.n_exit;
     for (uint32_t q = tid; q < ne; q += TILE_BLOCK) {
-        const uint32_t i = sh.exits[q] & 0xFFFFu;
-        const int d = (int)(sh.exits[q] >> 16);
+        const uint32_t ent = sh.exits[q];
+        const uint32_t i = ent & 0xFFFFu;
+        const int d = (int)((ent >> 16) & 1u);
         const int32_t v = lo + (int32_t)i;
-        const uint32_t bit = d ? ST_B : ST_F;
-        const int32_t *row = d ? a.rowT : a.row, *col = d ? a.colT : a.col;
-        const int32_t e1 = row[v + 1];
-        for (int32_t e = row[v]; e < e1; e++) {
-            const int32_t u = col[e];
-            if (u >= lo && u < hi) continue;
-            const uint32_t wu = a.word[u];
-            const int32_t t2 = a.tile_of[u];
-            if (wu & (bit | ST_E)) continue;
-            const int32_t base2 = a.toff[t2];
-            if (atomicOr(&a.word[u], bit) & bit) continue;
-            const int par2 = (a.iter + 1) & 1;
-            const uint32_t k = atomicAdd(&a.incnt[(par2 * 2 + d) * a.ntiles + t2], 1u);
-            a.inbox[(size_t)(par2 * 2 + d) * a.n + base2 + k] = u;
-            tile_push_next(a, t2);
-            found = true;
+        found = true;
+        if (!((ent >> 17) & 1u)) {
+            const int2 xo = a.xout[d][v];
+            if (xo.x >= 0) claim_across(a, d, xo.x);
+            if (xo.y >= 0) claim_across(a, d, xo.y);
+        } else {  // more than two: the row
+            const int32_t *row = d ? a.rowT : a.row, *col = d ? a.colT : a.col;
+            for (int32_t e = row[v]; e < row[v + 1]; e++) {
+                const int32_t u = col[e];
+                if (u < lo || u >= hi) claim_across(a, d, u);
+            }
         }
     }
     tile_sync();  // the shared claims are cleared for the next tile
-    if (a.instr && tid == 0) {
+    if (a.instr && tid == 0) {  // once per run (the wave counts were summed in LDS)
         clk[4] = wall_clock64();
         for (int q = 0; q < 4; q++) atomicAdd(&a.ctl->st_tclk[q], clk[q + 1] - clk[q]);
-        atomicMax(&a.ctl->tlog[a.iter & 3][2], (uint32_t)(clk[4] - clk[0]));
+        atomicMax(&a.ctl->tlog[a.iter & 3][2], (uint32_t)(clk[4] - clk[0] - (clk[3] - clk[2])));
+        atomicAdd(&a.ctl->st_tbusy, sh.ins[0]);
+        atomicAdd(&a.ctl->st_tidle, sh.ins[1]);
+        atomicAdd(&a.ctl->st_trounds, sh.ins[2]);
+        atomicMax(&a.ctl->tlog[a.iter & 3][1], (uint32_t)sh.ins[3]);
+        atomicAdd(&a.ctl->st_expanded, sh.ins[4]);
+        atomicAdd(&a.ctl->st_visited, sh.ins[4]);
+        atomicAdd(&a.ctl->st_edges, sh.ins[5]);
+        atomicAdd(&a.ctl->st_truns, 1ull);
+        atomicAdd(&a.ctl->tlog[a.iter & 3][0], 1u);
     }
 }
 
 __global__ __launch_bounds__(TILE_BLOCK) void k_tile_reach(TileArgs a) {
     __shared__ TileShared sh;
+    __shared__ uint32_t wcnt[TILE_WAVES];
+    __shared__ int32_t mine[TILE_LIST_MAX];
+    __shared__ uint32_t n_mine;
     if (chain_exit(a.ctl, a.iter)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctl->tcnt[(a.iter + 2) & 3] = 0;  // the list after next
-        if (a.instr) a.ctl->tlog[(a.iter + 2) & 3][0] = a.ctl->tlog[(a.iter + 2) & 3][1] =
-                         a.ctl->tlog[(a.iter + 2) & 3][2] = 0;
-    }
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    if (a.instr && blockIdx.x == 0 && tid == 0)  // the iteration log entry after next
+        a.ctl->tlog[(a.iter + 2) & 3][0] = a.ctl->tlog[(a.iter + 2) & 3][1] = a.ctl->tlog[(a.iter + 2) & 3][2] = 0;
     bool found = false;
     if (a.iter == 0) {
-        // the pivots (the seed list, ctl->qsize[0][0] entries), reached both ways by their
-        // election, into their tiles' inboxes
+        // the tiles of the pivots (the seed list, ctl->qsize[0][0] entries; reached both ways
+        // by their election), flagged for iteration 1
         const uint32_t np = a.ctl->qsize[0][0];
-        for (uint32_t k = blockIdx.x * TILE_BLOCK + threadIdx.x; k < np; k += gridDim.x * TILE_BLOCK) {
-            tile_post(a, 0, a.piv[k]);
-            tile_post(a, 1, a.piv[k]);
+        for (uint32_t k = blockIdx.x * TILE_BLOCK + tid; k < np; k += gridDim.x * TILE_BLOCK) {
+            a.flags[a.ntiles + a.piv[k] / a.tsize] = 1;
             found = true;
         }
-        // and every cell's closure, for the iterations that follow
-        for (int32_t t = (int32_t)blockIdx.x; t < a.ntiles; t += (int32_t)gridDim.x) cell_closure(a, t);
+        // every cell's closure, and nothing expanded yet
+        for (int32_t t = (int32_t)blockIdx.x; t < a.ntiles; t += (int32_t)gridDim.x) {
+            cell_closure(a, t);
+            for (uint32_t i = tid; i < TILE_CELLS * 2u; i += TILE_BLOCK)
+                a.done[(size_t)t * (TILE_CELLS * 2) + i] = 0ull;
+        }
     } else {
-        const uint32_t cnt = a.ctl->tcnt[a.iter & 3];
-        const int32_t *list = a.list[a.iter & 1];
-        for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x)  // block-uniform
-            tile_run(a, sh, list[k], found);
+        // the flagged tiles, ranked in tile order; rank g goes to workgroup g % grid.  The
+        // flags of iteration i + 2 are cleared meanwhile (iteration i - 1 read them)
+        if (blockIdx.x == 0)
+            for (int32_t t = (int32_t)tid; t < a.ntiles; t += TILE_BLOCK)
+                a.flags[(size_t)((a.iter + 2) % 3) * a.ntiles + t] = 0;
+        const uint8_t *flags = a.flags + (size_t)(a.iter % 3) * a.ntiles;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t ranked = 0;  // flagged tiles before this chunk
+        for (int32_t c0 = 0; c0 < a.ntiles; c0 += TILE_BLOCK) {  // block-uniform
+            const int32_t t = c0 + (int32_t)tid;
+            const bool f = t < a.ntiles && flags[t];
+            const unsigned long long m = __ballot(f);
+            if (lane == 0) wcnt[wv] = (uint32_t)__popcll(m);
+            if (tid == 0) n_mine = 0;
+            __syncthreads();
+            uint32_t pre = 0, tot = 0;
+            for (int q = 0; q < TILE_WAVES; q++) {
+                const uint32_t x = wcnt[q];
+                pre += q < (int)wv ? x : 0u;
+                tot += x;
+            }
+            if (f && (ranked + pre + (uint32_t)__popcll(m & below)) % gridDim.x == blockIdx.x)
+                mine[atomicAdd(&n_mine, 1u)] = t;  // at most TILE_LIST_MAX per chunk: grid >= 64
+            __syncthreads();
+            const uint32_t nm = n_mine;
+            for (uint32_t k = 0; k < nm; k++) tile_run(a, sh, mine[k], found);  // block-uniform
+            ranked += tot;
+            __syncthreads();  // wcnt, mine and n_mine are rewritten for the next chunk
+        }
     }
     chain_progress(a.ctl, a.iter, found);
 }

@@ -107,8 +107,11 @@ typedef struct {
                                   them) so that the reach of the first FB step (schedule h,
                                   hybrid_policy 0, single graph) closes cells in registers, tiles in
                                   one workgroup each, and crosses tile boundaries between iterations.
-                                  Results are unchanged.  0 (default) and -1: never; 1: tiles of up
-                                  to 4,096 vertices; N > 1: tiles of at most N vertices */
+                                  Results are unchanged.  0 (default): tiles of up to 4,096 vertices
+                                  when the degrees are not skewed, relabel is 0, n >= 65536 and a BFS
+                                  from vertex 0 over the arcs taken both ways is at least 256 levels
+                                  deep (high diameter: meshes, road-like graphs); -1: never; 1:
+                                  always, tiles of up to 4,096; N > 1: always, tiles of at most N */
 } scc_opts;
 
 /* Run statistics (the stdout block of scc.cu:998-1005 plus counters). */
@@ -267,8 +270,8 @@ typedef struct {
     double trim_kernel_ms;    /* sum of trim-sweep durations (flag 2) */
     int64_t tile_runs;        /* tile reach: runs of a tile by a workgroup (flag 1) */
     int64_t tile_rounds;      /* tile reach: claim words of a cell taken and closed in those runs (flag 1) */
-    double tile_phase_ms[4];  /* tile reach, summed over runs (flag 1): set-up and inbox, fixpoint,
-                                 write-back, claims across tile boundaries */
+    double tile_phase_ms[4];  /* tile reach, summed over runs (flag 1): set-up and seeds, fixpoint,
+                                 counting (instrumented runs only), claims across tile boundaries */
     double tile_busy_ms;      /* tile reach, summed over waves (flag 1): time spent on takes */
     int64_t tile_idle_polls;  /* tile reach (flag 1): polls of a wave that found nothing to take */
 } scc_profile_data;

@@ -142,13 +142,14 @@ def test_renumbered_batch(rctx, solver):
 
 
 def test_relabel_auto_and_edges(ctx):
-    """relabel 0 renumbers a skewed graph of 65,536 vertices or more and keeps
-    the numbering of a mesh or a small graph; graphs of 0, 1 and 2 vertices and
-    graphs without arcs run renumbered too."""
-    cases = [(scc.gen_rmat(16, 8, 0.57, 0.19, 0.19, seed=7), 1), (scc.gen_mesh(256, 0.6, seed=8), 0),
-             (scc.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=9), 0)]
-    for g, want in cases:
-        ctx.set_opts(scc.alg_opt(relabel=0))
+    """relabel 0 renumbers a skewed graph of 65,536 vertices or more by degree,
+    numbers a mesh of 65,536 vertices tile by tile (high diameter; with tiles off it
+    keeps its numbering) and keeps the numbering of a small graph; graphs of 0, 1
+    and 2 vertices and graphs without arcs run renumbered too."""
+    cases = [(scc.gen_rmat(16, 8, 0.57, 0.19, 0.19, seed=7), 1, 0), (scc.gen_mesh(256, 0.6, seed=8), 2, 0),
+             (scc.gen_mesh(256, 0.6, seed=8), 0, -1), (scc.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=9), 0, 0)]
+    for g, want, tile in cases:
+        ctx.set_opts(scc.alg_opt(relabel=0, tile_reach=tile))
         ctx.upload(g)
         lab, st = ctx.run("h", 0)
         assert st["renumbered"] == want

@@ -76,11 +76,17 @@ def test_tile_reach_matches_level_kernels(tctx):
 
 
 def test_tile_reach_default_large_and_batch(tctx):
-    """tile_reach 0 (the default) keeps a mesh untiled; tile_reach 1 tiles a mesh
-    of 2^20 vertices with full-size tiles; a batch drops the tiles."""
+    """tile_reach 0 (the default) tiles a mesh of 2^20 vertices (high diameter), -1
+    keeps it untiled, 1 tiles it with full-size tiles; R-MAT (skewed) is numbered by
+    degree, not tiled; a batch drops the tiles."""
     g = scc.gen_mesh(1024, 0.6, seed=10)
     ref = oracle.tarjan(g.row, g.col)[0]
-    for tile, want in ((0, 0), (1, 2)):
+    r = scc.gen_rmat(17, 8, 0.57, 0.19, 0.19, seed=12)
+    tctx.set_opts(scc.alg_opt())
+    tctx.upload(r)
+    lab, st = tctx.run("h", 0)
+    assert st["renumbered"] == 1 and np.array_equal(lab, oracle.tarjan(r.row, r.col)[0])
+    for tile, want in ((0, 2), (-1, 0), (1, 2)):
         tctx.set_opts(scc.alg_opt(tile_reach=tile))
         tctx.upload(g)
         lab, st = tctx.run("h", 0)
