@@ -268,6 +268,15 @@ def main():
     ctx.profile(0)
     runs += args.steps
 
+    # one more decomposition with the phases of the one-workgroup tail's steps timed (on
+    # the same path as the timed ones)
+    ctx.profile(8)
+    ctx.profile_read(reset=True)
+    _, st_tail = ctx.run(args.schedule, args.variant, labels=False)
+    tailp = ctx.profile_read(reset=True)
+    ctx.profile(0)
+    runs += 1
+
     ms_per_step = t_total * 1e3 / args.steps
     mteps = g.m / (ms_per_step * 1e3)
 
@@ -377,6 +386,10 @@ def main():
         "phase_ms": {k: round(statistics.mean(s[k] for s in stats), 3)
                      for k in ("ms_fwd", "ms_bwd", "ms_trim", "ms_update")},
         "decompositions_run": runs,
+        "tail": {"steps": st_tail["tail_steps"], "live_at_start": st_tail["tail_live"],
+                 "phase_ms": {k: round(x, 4) for k, x in zip(("reach", "trim", "update", "confirm"),
+                                                               tailp["tail_phase_ms"])},
+                 "lds_copy_live": tailp["tail_local_live"], "lds_copy_step": tailp["tail_local_step"]},
         "learned_pivot": learned, "roofline": roofline, "cpu_baseline": cpu,
     }
     ctx.close()

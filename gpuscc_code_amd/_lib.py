@@ -58,11 +58,16 @@ class SccProfile(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_int64) for k in _PROF_INTS] + [(k, ctypes.c_double) for k in _PROF_DBLS]
                 + [(k, ctypes.c_int64) for k in _PROF_TAIL] + [("tile_phase_ms", ctypes.c_double * 4),
                                                                ("tile_busy_ms", ctypes.c_double),
-                                                               ("tile_idle_polls", ctypes.c_int64)])
+                                                               ("tile_idle_polls", ctypes.c_int64),
+                                                               ("tail_phase_ms", ctypes.c_double * 4),
+                                                               ("tail_local_live", ctypes.c_int64),
+                                                               ("tail_local_step", ctypes.c_int64)])
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k in _PROF_INTS + _PROF_DBLS + _PROF_TAIL + ["tile_busy_ms", "tile_idle_polls"]}
+        d = {k: getattr(self, k) for k in _PROF_INTS + _PROF_DBLS + _PROF_TAIL
+             + ["tile_busy_ms", "tile_idle_polls", "tail_local_live", "tail_local_step"]}
         d["tile_phase_ms"] = list(self.tile_phase_ms)
+        d["tail_phase_ms"] = list(self.tail_phase_ms)
         return d
 
 

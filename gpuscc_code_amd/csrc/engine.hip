@@ -1404,7 +1404,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // chain may need more levels; every later kernel of the step then exits
         // at once, and the host resumes that chain and finishes the step with
         // waits.  The results are those of the waiting path.
-        const bool spec_ok = !c->opts.sequential_reach && !by_elim && c->prof_flags == 0;
+        const bool spec_ok = !c->opts.sequential_reach && !by_elim && (c->prof_flags & 7) == 0;
         // scc_w / scc_2_w (scc.cu:787-821): once n/100 vertices are FB-eliminated, the
         // pivots just elected are unset, scc_2_w peels size-2 SCCs, and one WCC pass gives
         // fresh colours and a new election.  The eliminated count is the device's (updates
@@ -1461,10 +1461,16 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
             ta.gate_live = gate;
             ta.xbits = schedule == SCC_SCHED_Q || schedule == SCC_SCHED_T ? 1 : 0;
             ta.mn = c->track_mn ? c->d_mn : nullptr;
+            ta.instr = (c->prof_flags & 8) ? 1 : 0;
             if (c->pend_stage || c->pend_bits) ctl_op(c, 0);  // queued resets go out first
             launch_fb_tail(c->stream, ta);
             CHK(read_ctl(c));
             const Ctl &h = *c->h_ctl;
+            if (c->prof_flags & 8) {
+                for (int q = 0; q < 4; q++) c->prof.tail_phase_ms[q] += (double)h.st_tailclk[q] * 1e-5;
+                c->prof.tail_local_live = h.tail_L;
+                c->prof.tail_local_step = h.tail_lstep;
+            }
             npiv = h.npiv;
             if (h.tail[1] == 3) return SCC_OK;  // not run: too many live vertices
             if (h.tail[1] == 2) return fail(c, SCC_E_STATE, "tail steps did not converge");
@@ -1762,7 +1768,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
 }
 
 scc_status scc_profile_set(scc_ctx *c, int flags) {
-    if (!c || flags < 0 || flags > 7) return SCC_E_INPUT;
+    if (!c || flags < 0 || flags > 15) return SCC_E_INPUT;
     if (flags & 4) flags |= 3;
     c->prof_flags = flags;
     c->instr = flags & 1;

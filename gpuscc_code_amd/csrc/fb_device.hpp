@@ -83,6 +83,9 @@ struct Ctl {
                                               // (set-up and seeds, fixpoint, counting, claims across)
     unsigned long long st_tbusy, st_tidle;  // instrumented tile reach: 100 MHz ticks waves spent on takes,
                                             // idle polls
+    unsigned long long st_tailclk[4];  // instrumented tail: 100 MHz ticks of its steps' reach, trim, update,
+                                       // confirm (thread 0's view)
+    uint32_t tail_L, tail_lstep;       // instrumented tail: live vertices copied to LDS, the step it happened
     uint32_t tlog[4][3];  // instrumented tile reach, ring by iteration: tiles run, most rounds and
                           // longest run (100 MHz ticks) of one tile
 };

@@ -189,6 +189,7 @@ struct TailArgs {
     int self_refill;     // before an election whose 16-bit tag would wrap, clear its slot table here
                          // (else hand back to the host, status 1)
     const int32_t *oid;  // the caller's ids (keys, roots), null: the same
+    int instr;           // time the steps' phases (ctl->st_tailclk) and note when the LDS copy was built
 };
 void launch_fb_tail(hipStream_t s, const TailArgs &t);
 // a batch in one persistent launch (k_fb_batch): graph g owns vertices [off[g], off[g + 1]);

@@ -280,6 +280,8 @@ __device__ inline void ctl_reset_body(Ctl *ctl, int what, int guard_last, int gu
         ctl->st_truns = 0;
         ctl->st_trounds = 0;
         ctl->st_tbusy = ctl->st_tidle = 0;
+        for (int i = 0; i < 4; i++) ctl->st_tailclk[i] = 0;
+        ctl->tail_L = ctl->tail_lstep = 0;
         for (int i = 0; i < 4; i++) {
             ctl->st_tclk[i] = 0;
             ctl->tlog[i][0] = ctl->tlog[i][1] = ctl->tlog[i][2] = 0;

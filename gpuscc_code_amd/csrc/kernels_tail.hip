@@ -285,6 +285,15 @@ struct BatchBook {
 template <class Sp, class Bk = TailBook>
 __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailRun &r, const Bk &bk = Bk()) {
     const uint32_t tid = threadIdx.x, nt = TAIL_BLOCK;
+    unsigned long long clk = 0;  // phase clocks of an instrumented tail (thread 0)
+    auto phase = [&](int q) {
+        if (t.instr && tid == 0) {
+            const unsigned long long now = wall_clock64();
+            if (q >= 0) atomicAdd(&t.ctl->st_tailclk[q], now - clk);
+            clk = now;
+        }
+    };
+    phase(-1);
     // ---- reach: F over out-arcs, B over in-arcs, both from the pivot list
     if (tid == 0) {
         sh.cnt[0] = sh.cnt[1] = r.npiv;
@@ -324,6 +333,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         qi ^= 1;
         tail_sync();
     }
+    phase(0);
     // ---- trim to the fixpoint (F and B without P: an SCC of two or more); a
     // round that trims nothing ends it (the count is read between two barriers,
     // so every work-item takes the same decision)
@@ -356,6 +366,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         seen = now;
         if (it > r.max_steps) return false;  // every round but the last trims a vertex
     }
+    phase(1);
     // ---- update: eliminate F and B into the pivot's SCC, split the rest, elect
     r.epoch++;
     const unsigned long long tag = elect_tag(r.epoch);
@@ -386,6 +397,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)og);
     }
     tail_sync();
+    phase(2);
     // ---- confirm: the lowest key of each slot becomes its colour's pivot (state 19)
     const uint32_t ln = sh.lcnt, cn = sp.csize(ln);
     for (uint32_t i = tid; i < cn; i += nt) {
@@ -401,6 +413,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         atomicMin(&sh.first, og);
     }
     tail_sync();
+    phase(3);
     r.npiv = sh.npiv;
     bk.end(t, sh, r, ln);  // the counts the host reads at the end
     r.pt ^= 1;
@@ -559,6 +572,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
         }
         return;
     }
+    if (t.instr && threadIdx.x == 0) {  // this launch's phase clocks (only thread 0 touches them)
+        for (int q = 0; q < 4; q++) ctl->st_tailclk[q] = 0;
+        ctl->tail_L = ctl->tail_lstep = 0;
+    }
     TailRun r{};
     r.lsel = t.lsel;
     r.pt = t.prev_tab;
@@ -608,6 +625,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_fb_tail(TailArgs t) {
                 }
                 local = b > 0;
                 try_below = L / 2;
+                if (local && t.instr && threadIdx.x == 0) {
+                    ctl->tail_L = L;
+                    ctl->tail_lstep = steps;
+                }
             }
         }
         steps++;

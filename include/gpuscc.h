@@ -256,7 +256,10 @@ scc_status scc_mlp_train(scc_ctx *ctx, const float *X, const float *y, int64_t N
  * timing.h:14).  flags: 1 = count expanded vertices / scanned edges / new
  * visits with device counters; 2 = time every level kernel with HIP events on
  * the context's stream; 4 = also log one row per reach or trim level (implies
- * 1 and 2; levels are then launched one at a time, so wall times grow). */
+ * 1 and 2; levels are then launched one at a time, so wall times grow).  Flags
+ * 1, 2 and 4 take the waiting path (no speculative steps, no one-workgroup
+ * tail); 8 = time the phases of the one-workgroup tail's steps, on the fast
+ * path. */
 typedef struct {
     int64_t reach_launches;   /* level kernels launched, including ones that exited at once */
     int64_t reach_levels;     /* level kernels that did work */
@@ -274,6 +277,10 @@ typedef struct {
                                  counting (instrumented runs only), claims across tile boundaries */
     double tile_busy_ms;      /* tile reach, summed over waves (flag 1): time spent on takes */
     int64_t tile_idle_polls;  /* tile reach (flag 1): polls of a wave that found nothing to take */
+    double tail_phase_ms[4];  /* one-workgroup tail (flag 1): its steps' reach, trim, update, confirm */
+    int64_t tail_local_live;  /* one-workgroup tail (flag 1): live vertices when its LDS copy was built
+                                 (0: never), and the step of the tail that built it */
+    int64_t tail_local_step;
 } scc_profile_data;
 scc_status scc_profile_set(scc_ctx *ctx, int flags);
 scc_status scc_profile_get(scc_ctx *ctx, scc_profile_data *out, int reset);
