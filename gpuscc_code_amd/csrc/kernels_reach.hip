@@ -1025,7 +1025,10 @@ __device__ inline uint32_t scan_rows_vis(const uint32_t *const (&bm)[T], const i
     return hit;
 }
 
-constexpr int PULL_V = 16;  // vertices per work-item per round of a pull sweep
+// vertices per work-item per round of a pull sweep: 8 rounds ahead of 16 and 32 (shorter waits
+// for each round's claims to be queued; R-MAT s24 1.03 -> 1.01 ms, the batch of 4,096 graphs
+// 8.31 -> 8.01 ms) and of 4 (the words of a round no longer cover the load latency)
+constexpr int PULL_V = 8;
 
 // Queues the vertices one sweep round claimed (pull visits, trims): bit k of
 // claimed[d] is vertex base + k * blockDim.x + threadIdx.x.  One prefix sum
