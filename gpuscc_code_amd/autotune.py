@@ -34,7 +34,7 @@ KNOBS = {
     "hybrid_arc_div": lambda n, cus: [0, 6, 30, -1],
     "hybrid_threshold": lambda n, cus: [0, max(1024, n // 10), max(1024, n // 40), max(1024, n // 80)],
     "local_rounds": lambda n, cus: [0, 16, 256, -1],
-    "huge_row": lambda n, cus: [0, 1024, 16384],
+    "huge_row": lambda n, cus: [0, 512, 1024, 2048, 16384],
     "block_size": lambda n, cus: [0, 128, 512],
     "reach_graphs": lambda n, cus: [0, 1],
     "reach_block": lambda n, cus: [0, 512],
