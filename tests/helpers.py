@@ -67,3 +67,16 @@ def is_canonical(labels):
     lab = np.asarray(labels)
     idx = np.arange(len(lab))
     return bool(np.all(lab <= idx) and np.all(lab[lab] == lab))
+
+
+def settled(words):
+    """The vertex words with the direction and expanded bits (B 1, F 2, BX 32,
+    FX 64) of eliminated vertices cleared.  Nothing reads those bits once a vertex
+    is eliminated, and they depend on timing: in the first FB step the short-row
+    expansions OR a direction bit into every target without reading it first
+    (every arc joins one colour there), eliminated targets included, and which
+    rows are expanded that way rather than by a path that reads the target first
+    depends on which workgroup reaches a vertex first.  Colours and the E, T, P
+    and pair bits stay, as in the comparisons with the oracle."""
+    w = np.asarray(words, np.uint32)
+    return np.where((w & 4) != 0, w & np.uint32(~0x63 & 0xFFFFFFFF), w)
