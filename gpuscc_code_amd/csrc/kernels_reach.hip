@@ -1160,20 +1160,55 @@ __device__ inline void pull_heads(const ReachArgs &a, bool vis, const PullTasks 
     pull_claims(a, hit, q, claimed, arcs);
 }
 
-// Entries 1.. of a batch's rows (each of at least two entries), in lockstep.
+// Entries 1.. of a batch's rows (each of at least two entries): entries 1 .. 3 of
+// every row at once from the four-entry heads (one 16-byte load per row, then
+// their visited bits or words together: most short rows settle in one round
+// trip), then entries 4.. of the longer rows in lockstep.
 __device__ inline void pull_rows(const ReachArgs &a, bool vis, const PullTasks &q, uint32_t (&claimed)[2],
                                  unsigned long long (&arcs)[2], unsigned long long &ce, uint32_t &smax) {
     int32_t e[PULL_T], e1[PULL_T];
-    uint32_t wv[PULL_T], act = 0u;
+    uint32_t wv[PULL_T], act = 0u, hit = 0u;
+    {
+        int4 h4[PULL_T];
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++) {
+            h4[t] = (((q.dm >> t) & 1u) ? a.ell[0] : a.ell[1])[q.v[t]];  // B pulls over the out-row
+            wv[t] = vis ? 0u : a.word[q.v[t]];
+            if (q.kk[t] >= 0) act |= 1u << t;
+        }
+        uint32_t x[PULL_T][3];
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++) {
+            const int32_t u3[3] = {h4[t].y, h4[t].z, h4[t].w};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int32_t u = u3[k] >= 0 ? u3[k] : 0;  // -1 past the row's end
+                x[t][k] = vis ? (((q.dm >> t) & 1u) ? a.vis[1] : a.vis[0])[u >> 5] : a.word[u];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PULL_T; t++) {
+            if (!((act >> t) & 1u)) continue;
+            const uint32_t bit = ((q.dm >> t) & 1u) ? ST_B : ST_F;
+            const int32_t u3[3] = {h4[t].y, h4[t].z, h4[t].w};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {  // counted as a scan in order counts them: up to the first hit
+                if (u3[k] < 0 || ((hit >> t) & 1u)) continue;
+                ce++;
+                const bool ok = vis ? ((x[t][k] >> (u3[k] & 31)) & 1u) != 0u
+                                    : (x[t][k] & (bit | ST_E)) == bit && colour_of(x[t][k]) == colour_of(wv[t]);
+                if (ok) hit |= 1u << t;
+            }
+        }
+        act &= ~hit;
+    }
 #pragma unroll
     for (int t = 0; t < PULL_T; t++) {
         const int32_t *rw = ((q.dm >> t) & 1u) ? a.row : a.rowT;
-        e[t] = rw[q.v[t]] + 1;  // the head is checked
+        e[t] = rw[q.v[t]] + 4;  // the first four entries are checked
         e1[t] = rw[q.v[t] + 1];
-        wv[t] = vis ? 0u : a.word[q.v[t]];
-        if (q.kk[t] >= 0) act |= 1u << t;
+        if (e[t] >= e1[t]) act &= ~(1u << t);
     }
-    uint32_t hit = 0u;
     int s = 0;
     for (; act; s++) {
         int32_t u[PULL_T];
@@ -1198,7 +1233,7 @@ __device__ inline void pull_rows(const ReachArgs &a, bool vis, const PullTasks &
             }
         }
     }
-    smax = (uint32_t)s + 1u > smax ? (uint32_t)s + 1u : smax;
+    smax = (uint32_t)s + 4u > smax ? (uint32_t)s + 4u : smax;  // the head, entries 1 .. 3, then s more
     pull_claims(a, hit, q, claimed, arcs);
 }
 
