@@ -167,8 +167,8 @@ def main():
     ap.add_argument("--relabel", type=int, default=0, choices=[-1, 0, 1],
                     help="renumbering at upload: 0 by degree skew (library default), 1 always, -1 never")
     ap.add_argument("--tile-reach", type=int, default=0,
-                    help="tile reach of the first FB step (scc_opts.tile_reach): 0 off (library default), "
-                         "1 tiles of 4096 vertices, T > 1 tiles of T vertices")
+                    help="tile reach of the first FB step (scc_opts.tile_reach): 0 the library default (tiles for "
+                         "graphs of high diameter), -1 off, 1 tiles of 4096 vertices, T > 1 tiles of T vertices")
     ap.add_argument("--no-tuned", action="store_true",
                     help="library defaults instead of the autotuned options (gpuscc_code_amd/tuned.json)")
     ap.add_argument("--no-learned", action="store_true",
