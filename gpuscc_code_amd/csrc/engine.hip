@@ -1729,7 +1729,7 @@ scc_status scc_run(scc_ctx *c, int schedule, int variant, int pivot_rule, int32_
         // already the minimum of each SCC (trimmed vertices and pairs point to
         // themselves and to the lower id); with the learned rule each root takes its
         // SCC's minimum, kept by the updates in d_mn.
-        if (learned) launch_root_to_min(l, c->d_root, c->d_mn, n);
+        if (learned) launch_root_to_min(l, c->d_root, c->d_mn, c->d_word, n);
         c->label_in_root = true;
         st.num_scc = (int64_t)c->h_ctl->pcount;
         return SCC_OK;

@@ -153,7 +153,7 @@ void launch_wcc_iter(const Launch &l, const Graph &g, const uint32_t *word, int3
 void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, int32_t n,
                          const Ctl *ctl, const int32_t *lin = nullptr, const uint32_t *lin_cnt = nullptr);
 // root[v] = mn[root[v]] in place (mn kept by the run's updates, see k_update)
-void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n);
+void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, const uint32_t *word, int32_t n);
 // out[i] = in[idx[i]], i < count
 void launch_gather_i32(const Launch &l, const int32_t *in, const int32_t *idx, int32_t *out, int64_t count);
 // renumbered CSR: row v becomes row map[v] at nrow[map[v]] in ncol, entries renamed by map

@@ -707,12 +707,19 @@ __global__ void k_wcc_recolour(uint32_t *word, const int32_t *lab, int32_t n, co
 }
 
 // root[v] = mn[root[v]], in place: each work-item reads and writes only its own
-// root, and writes only where the minimum differs from the pivot
-__global__ void k_root_to_min(int32_t *root, const int32_t *__restrict__ mn, int32_t n) {
+// root, and writes only where the minimum differs from the pivot.  Trimmed
+// vertices (T) are left out: every trim sets the root to the vertex itself, and
+// mn of a trimmed vertex is its own id -- only FB-eliminated vertices lower the
+// entry of their pivot, and a pivot that is trimmed has no such vertex (one
+// reached both ways would share a cycle with it and keep it live) -- so the
+// gather, scattered in the caller's numbering, is needed only for the rest.
+__global__ void k_root_to_min(int32_t *root, const int32_t *__restrict__ mn, const uint32_t *__restrict__ word,
+                              int32_t n) {
     int64_t beg, end;
     my_range(n, beg, end);
 #pragma unroll 8
     for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) {
+        if (word[v] & ST_T) continue;
         const int32_t r = root[v], m = mn[r];
         if (m != r) root[v] = m;
     }
@@ -842,8 +849,8 @@ void launch_wcc_recolour(const Launch &l, uint32_t *word, const int32_t *lab, in
                          const Ctl *ctl, const int32_t *lin, const uint32_t *lin_cnt) {
     hipLaunchKernelGGL(k_wcc_recolour, GRID(l), word, lab, n, ctl, lin, lin_cnt);
 }
-void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, int32_t n) {
-    hipLaunchKernelGGL(k_root_to_min, GRID(l), root, mn, n);
+void launch_root_to_min(const Launch &l, int32_t *root, const int32_t *mn, const uint32_t *word, int32_t n) {
+    hipLaunchKernelGGL(k_root_to_min, GRID(l), root, mn, word, n);
 }
 void launch_gather_i32(const Launch &l, const int32_t *in, const int32_t *idx, int32_t *out, int64_t count) {
     hipLaunchKernelGGL(k_gather_i32, GRID(l), in, idx, out, count);
