@@ -11,6 +11,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1107,6 +1108,18 @@ scc_status scc_create(int device, const scc_opts *opts, scc_ctx **out) {
         return SCC_E_INPUT;
     }
     if (hipSetDevice(device) != hipSuccess) { delete c; return SCC_E_HIP; }
+    {  // the kernels loaded onto this device once per process, outside every run
+        static std::mutex m;
+        static std::vector<bool> loaded;
+        std::lock_guard<std::mutex> lock(m);
+        if ((size_t)device >= loaded.size()) loaded.resize((size_t)device + 1, false);
+        if (!loaded[(size_t)device]) {
+            for (hipError_t (*f)() : {preload_fb, preload_reach, preload_mlp, preload_tail, preload_tile,
+                                      preload_noalloc})
+                if (f() != hipSuccess) { delete c; return SCC_E_HIP; }
+            loaded[(size_t)device] = true;
+        }
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         if (prop.multiProcessorCount > 0) c->cu_count = prop.multiProcessorCount;

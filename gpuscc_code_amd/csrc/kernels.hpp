@@ -241,6 +241,18 @@ struct TileArgs {
 };
 void launch_tile_reach(hipStream_t s, uint32_t grid, const TileArgs &a);
 
+// Each source file's kernels are loaded onto a device (its code object) the first
+// time one of them is launched there, which the first run of a process would pay
+// (R-MAT scale 24: 2.54 ms for the first run after the upload, 1.91 ms with the
+// loads done ahead); each of these loads one file's kernels on the current device
+// (scc_create, once per device).
+hipError_t preload_fb();
+hipError_t preload_reach();
+hipError_t preload_mlp();
+hipError_t preload_tail();
+hipError_t preload_tile();
+hipError_t preload_noalloc();
+
 // kernels_mlp.hip -- learned pivot scorer
 void launch_static_features(const Launch &l, const Graph &g, float4 *out);
 // live vertices per colour slot over the list lin[0 .. *lin_cnt) (lin null: all n)

@@ -864,4 +864,9 @@ void launch_collect(const Launch &l, const uint32_t *word, int32_t n, uint32_t b
     hipLaunchKernelGGL(k_collect, GRID(l), word, n, bit, q, ctl);
 }
 
+hipError_t preload_fb() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_fill_i32));
+}
+
 }  // namespace gpuscc

@@ -583,4 +583,9 @@ void launch_fill_f32(const Launch &l, float *p, int64_t count, float v) {
     hipLaunchKernelGGL(k_fill_f32, GRID(l), p, count, v);
 }
 
+hipError_t preload_mlp() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_score));
+}
+
 }  // namespace gpuscc

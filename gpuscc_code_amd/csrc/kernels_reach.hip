@@ -1691,4 +1691,9 @@ void launch_clear_bits(const Launch &l, uint32_t *word, int32_t n, uint32_t bits
     hipLaunchKernelGGL(k_clear_bits, dim3(l.grid), dim3(l.block), 0, l.stream, word, n, bits);
 }
 
+hipError_t preload_reach() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_reach_hybrid));
+}
+
 }  // namespace gpuscc

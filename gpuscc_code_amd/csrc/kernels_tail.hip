@@ -784,4 +784,9 @@ void launch_fb_batch(hipStream_t s, uint32_t grid, const TailArgs &t, const Batc
     hipLaunchKernelGGL(k_fb_batch, dim3(grid), dim3(TAIL_BLOCK), 0, s, t, b);
 }
 
+hipError_t preload_tail() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_fb_tail));
+}
+
 }  // namespace gpuscc

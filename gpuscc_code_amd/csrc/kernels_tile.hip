@@ -460,4 +460,9 @@ void launch_tile_reach(hipStream_t s, uint32_t grid, const TileArgs &a) {
     hipLaunchKernelGGL(k_tile_reach, dim3(grid), dim3(TILE_BLOCK), 0, s, a);
 }
 
+hipError_t preload_tile() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_tile_reach));
+}
+
 }  // namespace gpuscc

@@ -101,6 +101,11 @@ __global__ void k_noalloc_ctl(Ctl *ctl, int32_t n) {
 }
 
 }  // namespace
+hipError_t preload_noalloc() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_noalloc_setup));
+}
+
 }  // namespace gpuscc
 
 using namespace gpuscc;
