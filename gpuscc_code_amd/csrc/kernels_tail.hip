@@ -27,10 +27,10 @@ namespace gpuscc {
 
 constexpr int TAIL_BLOCK = 1024;
 // LDS for the local copy: words and ids (8 B per vertex), out and in row
-// offsets (8 B), four uint16 queues (8 B) and uint16 arcs
+// offsets (8 B), four uint16 queues (8 B), two uint16 live lists (4 B) and uint16 arcs
 constexpr uint32_t TAIL_ARENA = 150u * 1024u;
 
-__host__ __device__ inline uint32_t tail_fixed_bytes(uint32_t L) { return 24u * L + 8u; }
+__host__ __device__ inline uint32_t tail_fixed_bytes(uint32_t L) { return 28u * L + 16u; }
 
 // The local copy lives here; its parts are addressed by offsets from this
 // array, so every access compiles to an LDS instruction.
@@ -144,11 +144,12 @@ struct GlobalSpaceT : Slots {
     const int32_t *lst;
     uint32_t lc;
     int32_t *out;
-    __device__ uint32_t size() const { return lc; }
-    __device__ int32_t item(uint32_t i) const { return lst[i]; }
+    __device__ uint32_t size(int) const { return lc; }
+    __device__ int32_t item(int, uint32_t i) const { return lst[i]; }
     __device__ uint32_t csize(uint32_t ln) const { return ln; }
-    __device__ int32_t citem(uint32_t i) const { return out[i]; }
-    __device__ void keep(uint32_t k, int32_t v) const { out[k] = v; }
+    __device__ int32_t citem(int, uint32_t i) const { return out[i]; }
+    __device__ void keep(int, uint32_t k, int32_t v) const { out[k] = v; }
+    __device__ void kept(int, uint32_t, uint32_t) const {}  // the host reads the length from the control block
     __device__ uint32_t ld(int32_t v) const { return t->word[v]; }
     __device__ void st(int32_t v, uint32_t w) const { t->word[v] = w; }
     __device__ uint32_t orw(int32_t v, uint32_t b) const { return atomicOr(&t->word[v], b); }
@@ -177,7 +178,10 @@ using GlobalSpace = GlobalSpaceT<GlobalSlots>;
 // Vertices are local indexes 0..L-1 into the LDS copy; the pivot list is the
 // first two queues (a step's reach starts from them).  tail_arena holds, for L
 // live vertices and `to` out-arcs: words[L], global ids[L], out offsets[L+1],
-// in offsets[L+1], four uint16 queues[L], out arcs[to], in arcs (uint16), and,
+// in offsets[L+1], four uint16 queues[L], two uint16 live lists[L] and their
+// lengths (as the global path, the trim and update go over the list of the last
+// update, which writes the other: a step's sweeps cover the vertices still live,
+// not all L), out arcs[to], in arcs (uint16), and,
 // when there is room (H > 0, from byte t0), two election tables of H entries
 // (slot id, lowest global id), open addressing: the slot tables of the steps
 // in LDS, cleared every step instead of tagged.  Without the room, the global
@@ -230,12 +234,18 @@ struct LocalSpaceT : Slots {
     __device__ int32_t *gidv() const { return (int32_t *)(tail_arena + 4u * L); }
     __device__ uint32_t *off(int d) const { return (uint32_t *)(tail_arena + 8u * L) + (d ? L + 1 : 0); }
     __device__ uint16_t *q(int b) const { return (uint16_t *)(tail_arena + 16u * L + 8u) + b * L; }
+    __device__ uint16_t *lst(int k) const { return (uint16_t *)(tail_arena + 24u * L + 8u) + (k ? L : 0); }
+    __device__ uint32_t *lcount() const { return (uint32_t *)(tail_arena + 28u * L + 8u); }
     __device__ uint16_t *adj(int d) const { return (uint16_t *)(tail_arena + tail_fixed_bytes(L)) + (d ? to : 0); }
-    __device__ uint32_t size() const { return L; }
-    __device__ int32_t item(uint32_t i) const { return (int32_t)i; }
-    __device__ uint32_t csize(uint32_t) const { return L; }
-    __device__ int32_t citem(uint32_t i) const { return (int32_t)i; }
-    __device__ void keep(uint32_t, int32_t) const {}
+    // ls: the step's live list (TailRun::lsel, the global path's choice)
+    __device__ uint32_t size(int ls) const { return lcount()[ls]; }
+    __device__ int32_t item(int ls, uint32_t i) const { return (int32_t)lst(ls)[i]; }
+    __device__ uint32_t csize(uint32_t ln) const { return ln; }
+    __device__ int32_t citem(int ls, uint32_t i) const { return (int32_t)lst(ls ^ 1)[i]; }
+    __device__ void keep(int ls, uint32_t k, int32_t v) const { lst(ls ^ 1)[k] = (uint16_t)v; }
+    __device__ void kept(int ls, uint32_t ln, uint32_t tid) const {  // read after the confirm's barrier
+        if (tid == 0) lcount()[ls ^ 1] = ln;
+    }
     __device__ uint32_t ld(int32_t v) const { return w()[v]; }
     __device__ void st(int32_t v, uint32_t x) const { w()[v] = x; }
     __device__ uint32_t orw(int32_t v, uint32_t b) const { return atomicOr(&w()[v], b); }
@@ -337,12 +347,12 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
     // ---- trim to the fixpoint (F and B without P: an SCC of two or more); a
     // round that trims nothing ends it (the count is read between two barriers,
     // so every work-item takes the same decision)
-    const uint32_t lc = sp.size();
+    const uint32_t lc = sp.size(r.lsel);
     if (tid < 2) sh.red[tid] = 0;
     tail_sync();
     for (uint32_t it = 0, seen = 0;; it++) {
         for (uint32_t i = tid; i < lc; i += nt) {
-            const int32_t v = sp.item(i);
+            const int32_t v = sp.item(r.lsel, i);
             const uint32_t w = sp.ld(v);
             if ((w & ST_E) || (w & (ST_P | ST_F | ST_B)) == (ST_F | ST_B)) continue;
             const uint32_t c = colour_of(w);
@@ -377,7 +387,7 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
     }
     tail_sync();
     for (uint32_t i = tid; i < lc; i += nt) {
-        const int32_t v = sp.item(i);
+        const int32_t v = sp.item(r.lsel, i);
         const uint32_t w = sp.ld(v);
         if (w & ST_E) continue;
         uint32_t c = colour_of(w);
@@ -393,15 +403,16 @@ __device__ bool tail_step(const TailArgs &t, const Sp &sp, TailShared &sh, TailR
         }
         if (!r.keep_colour) c = (3u * c + f) & COLOUR_MASK;
         sp.st(v, make_word(c, 0));
-        sp.keep(atomicAdd(&sh.lcnt, 1u), v);
+        sp.keep(r.lsel, atomicAdd(&sh.lcnt, 1u), v);
         sp.offer(t, r.pt, slot_of(c, t.slot_bits), tag | (uint32_t)og);
     }
     tail_sync();
     phase(2);
     // ---- confirm: the lowest key of each slot becomes its colour's pivot (state 19)
     const uint32_t ln = sh.lcnt, cn = sp.csize(ln);
+    sp.kept(r.lsel, ln, tid);
     for (uint32_t i = tid; i < cn; i += nt) {
-        const int32_t v = sp.citem(i);
+        const int32_t v = sp.citem(r.lsel, i);
         const uint32_t w = sp.ld(v);
         if (w & ST_E) continue;
         const uint32_t c = colour_of(w);
@@ -438,6 +449,11 @@ __device__ int tail_build(const TailArgs &t, const TailRun &r, TailShared &sh, L
     ls.L = L;
     ls.to = 0;
     ls.H = 0;
+    {  // the live list of the step: every vertex of the copy
+        uint16_t *l = ls.lst(r.lsel);
+        for (uint32_t i = tid; i < L; i += nt) l[i] = (uint16_t)i;
+        if (tid == 0) ls.lcount()[r.lsel] = L;
+    }
     uint32_t *lw = ls.w(), *off0 = ls.off(0), *off1 = ls.off(1);
     int32_t *gidv = ls.gidv();
     // global id -> local index of the live vertices, indexed from r.vbase (the
